@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the LAMBDA per-block correlate + beamform + RFI-null path on
+one MI355X.
+
+Workload (default): 384 channels x 2 pols x 36 antennas (64-receiver LAMBDA
+layout), 8192 int8 complex samples per block, 16 beams of FP16 random-phase
+weights, one CW tone per channel, plain batched Jacobi on all 768 36x36
+covariances, rank-1 null, visibilities accumulated over 4 blocks; the
+adaptive pipeline outputs 2 x 16 beams (original + nulled).
+
+A step = one block through the adaptive pipeline with the voltages resident
+in HBM.  Prints one JSON line (see the contract in the repository docs).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "cuda-spatial-filtering_amd"))
+sys.path.insert(0, os.path.join(REPO, "oracle"))
+
+import lam  # noqa: E402
+import lam_synth as syn  # noqa: E402
+
+CHANNEL_RATE_HZ = 781250.0
+REF_RATE_HZ = 15500.0 * 64.0  # scripts/benchmark/README.md:48-50
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(shape, seed, n_tones=1):
+    scene = syn.make_scene(shape, seed=seed, n_tones=n_tones, amp=40.0)
+    ring, scales = syn.make_block(shape, scene, seed=seed + 1)
+    w16, w = syn.random_weights(shape, seed=seed + 2)
+    return scene, ring, scales, w16, w
+
+
+def algorithmic_bytes(shape, nbeams_out):
+    """Bytes a kernel must move at minimum (no padding, no re-reads)."""
+    C, R, T = shape.C, shape.R, shape.T
+    volt = C * R * 2 * T * 2                       # int8 complex voltages, both pols
+    vis = C * (R * (R + 1) // 2) * 8 * 4           # int32 visibilities [C][NBL][2][2][2]
+    beams = C * 2 * nbeams_out * T * 4             # fp16 complex beams
+    cov = C * 2 * R * R * 8                        # fp32 complex covariance / basis
+    return dict(volt=volt, vis=vis, beams=beams, cov=cov)
+
+
+def cpu_baseline(shape, ring, scales, w, channels):
+    """FP64 oracle on `channels` channels of the same block, all host cores."""
+    import oracle as orc
+
+    sub = syn.BlockShape(C=channels, F=shape.F, NT=shape.NT, R=shape.R, P=2, RPP=shape.RPP,
+                         NP=shape.NP, B=shape.B)
+    s = orc.Shape(C=channels, F=shape.F, NT=shape.NT, R=shape.R, P=2, RPP=shape.RPP,
+                  NP=shape.NP, B=shape.B)
+    t0 = time.perf_counter()
+    y = orc.ingest(s, ring[:channels], scales[:channels], orc.WINDOW_ADAPTIVE)
+    orc.adaptive_block(s, y, w[:channels], fixed_k=np.ones((channels, 2)))
+    dt = time.perf_counter() - t0
+    ant_samples = shape.R * channels * shape.T
+    return dict(value=ant_samples / dt, unit="ant-samples/s", cores=orc.num_threads(),
+                kind="port",
+                sample=f"{channels} of {shape.C} channels, 1 block of {shape.T} samples, "
+                       f"FP64 oracle: ingest+correlate+whiten+Jacobi+null+W.P+{2 * shape.B} beams; "
+                       f"{dt:.2f} s wall on {platform.processor() or platform.machine()}",
+                seconds=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--channels", type=int, default=384)
+    ap.add_argument("--mode", choices=["adaptive", "corr_beam"], default="adaptive")
+    ap.add_argument("--sweeps", type=int, default=10)
+    ap.add_argument("--cpu-channels", type=int, default=48)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graphs", type=int, default=0,
+                    help="time with hipGraph replay instead of per-kernel events")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
+                                   "(channels shard with no exchange; see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+
+    torch.cuda.set_device(0)
+    shape = syn.BlockShape(C=args.channels, F=1, NT=64, R=36, P=2, RPP=36, NP=128, B=16,
+                           blocks_to_accumulate=4)
+    log(f"generating {shape.C}x{shape.T} block ...")
+    scene, ring, scales, w16, w = make_inputs(shape, seed=20261203 + 2)
+    d_ring = torch.from_numpy(ring).cuda()
+    d_scales = torch.from_numpy(scales).cuda()
+    mode = lam.MODE_ADAPTIVE if args.mode == "adaptive" else lam.MODE_CORR_BEAM
+    p = lam.Pipeline(shape, mode, num_buffers=1)
+    p.set_weights(w16)
+    if mode == lam.MODE_ADAPTIVE:
+        p.set_fixed_k(np.ones((shape.C, 2), np.int32))
+        p.set_eig_params(args.sweeps, 1e-6)
+    p.set_graphs(bool(args.graphs))
+    p.set_profiling(not args.graphs)
+
+    log("warmup ...")
+    for i in range(args.warmup):
+        p.execute(d_ring, d_scales, on_device=True)
+        if (i + 1) % 4 == 0:
+            p.dump_visibilities_i64()
+    p.sync()
+    p.kernel_times()  # discard warmup timings
+    p.dump_visibilities_i64()
+
+    log(f"timing {args.steps} steps ...")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        p.execute(d_ring, d_scales, on_device=True)
+    p.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = p.kernel_times()
+    p.close()
+
+    sec = (t1 - t0) / args.steps
+    ant_samples = shape.R * shape.C * shape.T
+    value = ant_samples / sec
+    rtf = (shape.T / CHANNEL_RATE_HZ) / sec
+    nb_out = 2 * shape.B if mode == lam.MODE_ADAPTIVE else shape.B
+    ab = algorithmic_bytes(shape, nb_out)
+    per_kernel_bytes = {
+        "corr": ab["volt"] + ab["vis"],
+        "beam_2b": ab["volt"] + ab["beams"],
+        "corr_beam": ab["volt"] + ab["beams"] + ab["vis"],
+        "eig_null": ab["vis"] // 2 + ab["cov"],
+    }
+    kstats = {k: dict(ms_avg=v[0] / max(1, v[1]), launches=v[1]) for k, v in kt.items()}
+    dominant = max(kstats, key=lambda k: kstats[k]["ms_avg"] * kstats[k]["launches"]) if kstats else None
+    roofline = None
+    if dominant:
+        ms = kstats[dominant]["ms_avg"]
+        gbs = per_kernel_bytes[dominant] / (ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
+                    "kernel_ms": round(ms, 4), "algorithmic_bytes": per_kernel_bytes[dominant]}
+    for k, v in kstats.items():
+        v["GBps_algorithmic"] = round(per_kernel_bytes[k] / (v["ms_avg"] * 1e-3) / 1e9, 1)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(shape, ring, scales, w, min(args.cpu_channels, shape.C))
+
+    line = {
+        "metric": "ant-samples/s & real-time factor, corr+beam+RFI-null, learned vs plain eig",
+        "value": value,
+        "unit": "ant-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8->int32 corr, fp16 MFMA beams, fp32 eig (fp64 angles)",
+        "data": "synthetic (seeded Gaussian sigma=16 + 1 CW tone/channel, int8)",
+        "config": {"workload": f"{args.mode}: {shape.C} ch x 2 pol x 36 ant (64 rx), "
+                               f"{shape.T} samples/block, {shape.B} beams -> {nb_out} out, "
+                               f"plain Jacobi ({args.sweeps} sweeps max) rank-1 null, "
+                               "vis accumulated over 4 blocks",
+                   "channels": shape.C, "samples_per_block": shape.T, "beams": shape.B,
+                   "eig": "plain"},
+        "real_time_factor": rtf,
+        "real_time_factor_ref_convention": (shape.T / REF_RATE_HZ) / sec,
+        "roofline": roofline,
+        "kernels": kstats,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,273 @@
+"""Python binding of the C ABI in include/lam.h (liblam.so, gfx950 HIP).
+
+Host-side mirror of the reference's pipeline interface for this path
+(GPUPipeline, include/spatial/pipeline_base.hpp:9-40): a ``Pipeline`` is one
+lam_ctx; ``execute`` is execute_pipeline, ``dump_visibilities`` is
+dump_visibilities.  PyTorch is used only for device memory in tests and
+benchmarks; there is no CPU fallback: if liblam.so is missing this module
+raises on import of the library.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from lam_synth import BlockShape
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblam.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "lam.h")
+
+LAM_OK, LAM_ERR_ARG, LAM_ERR_HIP, LAM_ERR_RANGE, LAM_ERR_STATE, LAM_ERR_UNSUPPORTED, LAM_ERR_NO_DEVICE = range(7)
+MODE_CORR_BEAM, MODE_ADAPTIVE = 0, 1
+K_CORRBEAM, K_CORR, K_EIG, K_BEAM2B = range(4)
+KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b"]
+
+_lib = None
+
+
+class LamError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lam error {code}: {msg}")
+        self.code = code
+
+
+class LamShape(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "nr_channels", "nr_fpga_sources", "nr_time_steps_per_packet", "nr_receivers",
+        "nr_polarizations", "nr_receivers_per_packet", "nr_packets_for_correlation", "nr_beams",
+        "nr_padded_receivers", "nr_blocks_to_accumulate")]
+
+
+HOSTFN = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+
+
+class LamBlockIO(ctypes.Structure):
+    _fields_ = [
+        ("samples", ctypes.c_void_p), ("scales", ctypes.c_void_p),
+        ("inputs_on_device", ctypes.c_int32),
+        ("release_fn", HOSTFN), ("release_user", ctypes.c_void_p),
+        ("beams_dst", ctypes.c_void_p), ("beams_done_fn", HOSTFN), ("beams_done_user", ctypes.c_void_p),
+        ("eigenvalues_dst", ctypes.c_void_p), ("eigenvectors_dst", ctypes.c_void_p),
+        ("counts_dst", ctypes.c_void_p), ("eigen_done_fn", HOSTFN), ("eigen_done_user", ctypes.c_void_p),
+    ]
+
+
+def header_symbols() -> list[str]:
+    """Every lam_* function declared in include/lam.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(lam_[a-z0-9_]+)\s*\(", text)))
+
+
+def load_library(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is not built; run `make -C cuda-spatial-filtering_amd` (or __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    vp, i32 = ctypes.c_void_p, ctypes.c_int32
+    L.lam_create.argtypes = [ctypes.POINTER(LamShape), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+    L.lam_destroy.argtypes = [vp]
+    L.lam_last_error.argtypes = [vp]
+    L.lam_last_error.restype = ctypes.c_char_p
+    L.lam_set_weights.argtypes = [vp, vp]
+    L.lam_set_gains.argtypes = [vp, vp]
+    L.lam_set_subpacket_delays.argtypes = [vp, vp]
+    L.lam_set_fine_delays.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double, i32]
+    L.lam_set_stream_permutation.argtypes = [vp, vp, vp, ctypes.c_size_t]
+    L.lam_set_fixed_k.argtypes = [vp, vp]
+    L.lam_set_nulling.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_int]
+    L.lam_set_eig_params.argtypes = [vp, ctypes.c_int, ctypes.c_float]
+    L.lam_set_graphs.argtypes = [vp, ctypes.c_int]
+    L.lam_set_profiling.argtypes = [vp, ctypes.c_int]
+    L.lam_set_tuning.argtypes = [vp, ctypes.c_int]
+    L.lam_kernel_times.argtypes = [vp, vp, vp]
+    L.lam_execute.argtypes = [vp, ctypes.POINTER(LamBlockIO)]
+    L.lam_dump_visibilities.argtypes = [vp, vp, HOSTFN, vp]
+    L.lam_dump_visibilities_i64.argtypes = [vp, vp, HOSTFN, vp]
+    L.lam_blocks_accumulated.argtypes = [vp]
+    L.lam_sync.argtypes = [vp]
+    L.lam_next_stream.argtypes = [vp]
+    L.lam_next_stream.restype = vp
+    L.lam_eigh_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_float, vp, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def _ptr(x):
+    """Raw address of a numpy array or torch tensor (host or device)."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        assert x.flags.c_contiguous
+        return x.ctypes.data
+    assert x.is_contiguous()
+    return x.data_ptr()
+
+
+_NULLFN = HOSTFN()
+
+
+class Pipeline:
+    """One lam_ctx: LambdaCorrBeamOnlyGPUPipeline (mode 0) or
+    LambdaAdaptiveBeamformedSpectraPipeline (mode 1) on the current GPU."""
+
+    def __init__(self, shape: BlockShape, mode=MODE_CORR_BEAM, num_buffers=1):
+        self.L = load_library()
+        self.shape = shape
+        self.mode = mode
+        sh = LamShape(shape.C, shape.F, shape.NT, shape.R, shape.P, shape.RPP, shape.NP, shape.B,
+                      shape.R_pad, shape.blocks_to_accumulate)
+        h = ctypes.c_void_p()
+        rc = self.L.lam_create(ctypes.byref(sh), mode, num_buffers, ctypes.byref(h))
+        if rc != LAM_OK:
+            raise LamError(rc, "lam_create failed")
+        self.h = h
+        self._keep = []
+
+    # -- plumbing
+    def _chk(self, rc):
+        if rc != LAM_OK:
+            raise LamError(rc, self.L.lam_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lam_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_out_beams(self):
+        return self.shape.B * (2 if self.mode == MODE_ADAPTIVE else 1)
+
+    # -- setters (GPUPipeline virtuals)
+    def set_weights(self, halfs_u16):
+        a = np.ascontiguousarray(halfs_u16, np.uint16)
+        self._chk(self.L.lam_set_weights(self.h, a.ctypes.data))
+
+    def set_gains(self, gains):
+        g = np.ascontiguousarray(np.stack([np.real(gains), np.imag(gains)], -1), np.float32)
+        self._chk(self.L.lam_set_gains(self.h, g.ctypes.data))
+
+    def set_subpacket_delays(self, delays):
+        d = np.ascontiguousarray(delays, np.int32)
+        self._chk(self.L.lam_set_subpacket_delays(self.h, d.ctypes.data))
+
+    def set_fine_delays(self, delays_ns, base_freq_hz, channel_bw_hz, min_freq_ch):
+        d = None if delays_ns is None else np.ascontiguousarray(delays_ns, np.float32)
+        self._chk(self.L.lam_set_fine_delays(self.h, None if d is None else d.ctypes.data,
+                                             base_freq_hz, channel_bw_hz, min_freq_ch))
+
+    def set_stream_permutation(self, recv_perm, pol_perm):
+        r = np.ascontiguousarray(recv_perm, np.int32)
+        p = np.ascontiguousarray(pol_perm, np.int32)
+        self._chk(self.L.lam_set_stream_permutation(self.h, r.ctypes.data, p.ctypes.data, r.size))
+
+    def set_fixed_k(self, k):
+        a = np.ascontiguousarray(np.broadcast_to(k, (self.shape.C, 2)), np.int32)
+        self._chk(self.L.lam_set_fixed_k(self.h, a.ctypes.data))
+
+    def set_nulling(self, detect=False, delta=3.0, shrink=False):
+        self._chk(self.L.lam_set_nulling(self.h, int(bool(detect)), float(delta), int(bool(shrink))))
+
+    def set_eig_params(self, max_sweeps=12, tol=1e-6):
+        self._chk(self.L.lam_set_eig_params(self.h, int(max_sweeps), float(tol)))
+
+    def set_graphs(self, enable=True):
+        self._chk(self.L.lam_set_graphs(self.h, int(bool(enable))))
+
+    def set_profiling(self, enable=True):
+        self._chk(self.L.lam_set_profiling(self.h, int(bool(enable))))
+
+    def set_tuning(self, samples_per_wg=0):
+        self._chk(self.L.lam_set_tuning(self.h, int(samples_per_wg)))
+
+    # -- per block
+    def execute(self, samples, scales, beams_out=None, evals_out=None, evecs_out=None,
+                counts_out=None, on_device=None):
+        """Enqueue one block.  samples/scales: numpy (host) or torch (device)."""
+        if on_device is None:
+            on_device = not isinstance(samples, np.ndarray)
+        io = LamBlockIO()
+        io.samples = _ptr(samples)
+        io.scales = _ptr(scales)
+        io.inputs_on_device = int(on_device)
+        io.release_fn = _NULLFN
+        io.beams_dst = _ptr(beams_out)
+        io.beams_done_fn = _NULLFN
+        io.eigenvalues_dst = _ptr(evals_out)
+        io.eigenvectors_dst = _ptr(evecs_out)
+        io.counts_dst = _ptr(counts_out)
+        io.eigen_done_fn = _NULLFN
+        self._chk(self.L.lam_execute(self.h, ctypes.byref(io)))
+
+    def sync(self):
+        return self._chk(self.L.lam_sync(self.h))
+
+    def dump_visibilities(self):
+        s = self.shape
+        out = np.empty((s.C, s.nbl, 2, 2, 2), np.float32)
+        self._chk(self.L.lam_dump_visibilities(self.h, out.ctypes.data, _NULLFN, None))
+        self.sync()
+        return out
+
+    def dump_visibilities_i64(self):
+        s = self.shape
+        out = np.empty((s.C, s.nbl, 2, 2, 2), np.int64)
+        self._chk(self.L.lam_dump_visibilities_i64(self.h, out.ctypes.data, _NULLFN, None))
+        self.sync()
+        return out
+
+    def blocks_accumulated(self):
+        return self.L.lam_blocks_accumulated(self.h)
+
+    def next_stream(self):
+        return self.L.lam_next_stream(self.h)
+
+    def kernel_times(self):
+        ms = np.zeros(4, np.float64)
+        n = np.zeros(4, np.int32)
+        self._chk(self.L.lam_kernel_times(self.h, ms.ctypes.data, n.ctypes.data))
+        return {KERNEL_NAMES[i]: (float(ms[i]), int(n[i])) for i in range(4) if n[i]}
+
+    # -- output helpers
+    def beams_array(self):
+        s = self.shape
+        return np.zeros((s.C, 2, self.n_out_beams, s.T, 2), np.float16)
+
+    def run_host(self, ring, scales):
+        """Convenience: one block from host arrays, returns fp16 beams (+eigen)."""
+        beams = self.beams_array()
+        s = self.shape
+        extra = {}
+        if self.mode == MODE_ADAPTIVE:
+            extra = dict(evals_out=np.zeros((s.C, 2, s.R), np.float32),
+                         evecs_out=np.zeros((s.C, 2, s.R, s.R, 2), np.float32),
+                         counts_out=np.zeros((s.C, 2), np.int32))
+        self.execute(np.ascontiguousarray(ring, np.int8), np.ascontiguousarray(scales, np.int16),
+                     beams_out=beams, **extra)
+        self.sync()
+        out = dict(beams=beams[..., 0].astype(np.float32) + 1j * beams[..., 1].astype(np.float32))
+        out.update(extra)
+        return out
+
+
+def eigh_batched(A_dev, n, nmat, evals_dev, evecs_dev, sweeps_dev=None, whiten=False,
+                 max_sweeps=12, tol=1e-6, stream=None):
+    L = load_library()
+    rc = L.lam_eigh_batched(_ptr(A_dev), n, nmat, int(whiten), max_sweeps, tol, _ptr(evals_dev),
+                            _ptr(evecs_dev), _ptr(sweeps_dev), stream)
+    if rc != LAM_OK:
+        raise LamError(rc, "lam_eigh_batched failed")

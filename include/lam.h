@@ -1,0 +1,189 @@
+/*
+ * lam.h -- C ABI of the MI355X (gfx950) LAMBDA correlate -> beamform -> RFI-null
+ * hot path.  Plain pointers, sizes and status codes; no C++ or torch types.
+ *
+ * The ABI replaces the device half of the reference's two GPU pipelines
+ * (jdgsmallwood/cuda-spatial-filtering, paths relative to its root):
+ *   LambdaCorrBeamOnlyGPUPipeline        include/spatial/pipeline/lambda_corr_beam_only_gpu_pipeline.hpp
+ *   LambdaAdaptiveBeamformedSpectraPipeline
+ *                                        include/spatial/pipeline/lambda_adaptive_beamformed_spectra_pipeline.hpp
+ * behind the GPUPipeline seam (include/spatial/pipeline_base.hpp:9-40).  The
+ * C++ classes of the same names (cuda-spatial-filtering_amd/include/spatial/)
+ * are thin host wrappers over these entry points.
+ *
+ * Layouts (row-major; "complex" = interleaved re, im):
+ *   samples      int8     [C][NP+2][F][NT][RPP][P][2]   packet_formats.hpp:309-312
+ *   scales       int16    [C][NP+2][R][P]               packet_formats.hpp:305-306
+ *   weights      float16  [C][P][B][R][2]               pipeline/common.hpp:52-55
+ *   gains        float32  [C][P][R][2]                  packet_formats.hpp:384-387
+ *   beams        float16  [C][P][Bout][NP*NT][2]        packet_formats.hpp:346-348
+ *                (Bout = B in corr-beam mode, 2B in adaptive mode: beams
+ *                 0..B-1 original, B..2B-1 RFI-mitigated, adaptive 773-806)
+ *   visibilities float32  [C][R(R+1)/2][P][P][2]        packet_formats.hpp:354-356
+ *                V[bl(i,j)][p][q] = sum_t x_ip conj(x_jq), bl(i,j) = j(j+1)/2 + i, i <= j
+ *   eigenvalues  float32  [C][P][R]     ascending       adaptive 41-45, 914-950
+ *   eigenvectors float32  [C][P][R][R][2] column-major (eigenvector k at [k][0..R))
+ *   counts       int32    [C][P]                        adaptive 709-715
+ *
+ * Threading: a context is used from one host thread (the reference's
+ * pipeline_feeder, spatial.hpp:1275-1299).  Host callbacks run on the HIP
+ * runtime's callback thread, as cudaLaunchHostFunc callbacks do in the
+ * reference (pipeline/common.hpp:387-459).
+ */
+#ifndef LAM_H
+#define LAM_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LAM_ABI_VERSION 1
+
+/* Status codes.  The reference exits the process on CUDA errors
+ * (logging.hpp:11-27); this ABI returns them instead. */
+enum {
+  LAM_OK = 0,
+  LAM_ERR_ARG = 1,         /* invalid argument or shape (reference: std::runtime_error) */
+  LAM_ERR_HIP = 2,         /* HIP runtime failure */
+  LAM_ERR_RANGE = 3,       /* a scaled sample left the int8 range of the exact correlator */
+  LAM_ERR_STATE = 4,       /* call out of order (reference: std::logic_error, common.hpp:513-515) */
+  LAM_ERR_UNSUPPORTED = 5, /* shape outside this build's limits */
+  LAM_ERR_NO_DEVICE = 6    /* no gfx950 device */
+};
+
+/* Pipeline variants. */
+enum {
+  LAM_MODE_CORR_BEAM = 0, /* LambdaCorrBeamOnlyGPUPipeline */
+  LAM_MODE_ADAPTIVE = 1   /* LambdaAdaptiveBeamformedSpectraPipeline (corr + eig + null + 2B beams) */
+};
+
+/* Runtime form of LambdaConfig (packet_formats.hpp:265-407). */
+typedef struct lam_shape {
+  int32_t nr_channels;                /* C */
+  int32_t nr_fpga_sources;            /* F */
+  int32_t nr_time_steps_per_packet;   /* NT */
+  int32_t nr_receivers;               /* R = F * RPP, <= 64 */
+  int32_t nr_polarizations;           /* P, must be 2 */
+  int32_t nr_receivers_per_packet;    /* RPP */
+  int32_t nr_packets_for_correlation; /* NP */
+  int32_t nr_beams;                   /* B */
+  int32_t nr_padded_receivers;        /* reference TCC padding; informational */
+  int32_t nr_blocks_to_accumulate;    /* NR_CORRELATED_BLOCKS_TO_ACCUMULATE */
+} lam_shape;
+
+typedef struct lam_ctx lam_ctx;
+typedef void (*lam_host_fn)(void *user);
+
+/* Create a pipeline context on the current HIP device with `num_buffers`
+ * round-robin buffers/streams (lambda_corr_beam_only_gpu_pipeline.hpp:385-470;
+ * adaptive 1014-1210).  Weights start at zero, gains at 1+0j, delays 0. */
+int lam_create(const lam_shape *shape, int mode, int num_buffers, lam_ctx **out);
+int lam_destroy(lam_ctx *ctx);
+/* Last error text of this context ("" if none); valid until the next call. */
+const char *lam_last_error(const lam_ctx *ctx);
+int lam_abi_version(void);
+
+/* Beam weights complex<half>[C][P][B][R] (BeamWeightsT, pipeline/common.hpp:52-55);
+ * copied (lambda_corr_beam_only_gpu_pipeline.hpp:491-493). */
+int lam_set_weights(lam_ctx *ctx, const uint16_t *weights_half);
+/* Antenna gains complex<float>[C][P][R] (GPUPipeline::set_antenna_gains,
+ * pipeline_base.hpp:17-19).  Applied as g_i conj(g_j) to visibilities and
+ * folded into beam weights. */
+int lam_set_gains(lam_ctx *ctx, const float *gains);
+/* Per-FPGA integer sample delays, F ints in [-NT, NT]
+ * (set_subpacket_delays, adaptive 1216-1220; apply_delays spatial.cuh:1420-1506). */
+int lam_set_subpacket_delays(lam_ctx *ctx, const int32_t *delays);
+/* Per-receiver fine delays in ns (set_fine_delays,
+ * lambda_gpu_pipeline.hpp:1100-1131), applied as the channel-centre phase
+ * exp(-2 pi i f_c tau) folded into the beam weights and exp(i(phi_i - phi_j))
+ * on visibilities.  delays_ns == NULL clears them. */
+int lam_set_fine_delays(lam_ctx *ctx, const float *delays_ns, double base_freq_hz,
+                        double channel_bw_hz, int32_t min_freq_ch);
+/* Stream permutation (set_stream_permutation, adaptive 549-561):
+ * n must equal F*RPP*P, entries indexed by canonical (receiver, pol); recv -1
+ * gives zeros.  LAM_ERR_ARG on size mismatch. */
+int lam_set_stream_permutation(lam_ctx *ctx, const int32_t *recv_perm,
+                               const int32_t *pol_perm, size_t n);
+/* Fixed number of nulled modes per (channel, pol), int32[C][P] (adaptive 1157-1167). */
+int lam_set_fixed_k(lam_ctx *ctx, const int32_t *k);
+/* Null/shrink and detection switches (adaptive ctor 1014-1020, 687-757). */
+int lam_set_nulling(lam_ctx *ctx, int detect, float delta, int shrink);
+/* Eigensolver controls: maximum Jacobi sweeps and convergence tolerance on
+ * the relative off-diagonal Frobenius norm. */
+int lam_set_eig_params(lam_ctx *ctx, int max_sweeps, float tol);
+
+/* One block's inputs and output landing pointers (the ProcessorState ->
+ * GPUPipeline -> Output seam, pipeline/common.hpp:505-537,
+ * lambda_corr_beam_only_gpu_pipeline.hpp:259-284, adaptive 850-951). */
+typedef struct lam_block_io {
+  const void *samples;     /* int8 ring block */
+  const void *scales;      /* int16 scales */
+  int32_t inputs_on_device;/* 1: device pointers, no H2D copy */
+  lam_host_fn release_fn;  /* called once the host input is no longer read (may be NULL) */
+  void *release_user;
+  void *beams_dst;         /* host or device landing pointer for beams (NULL: skip) */
+  lam_host_fn beams_done_fn;
+  void *beams_done_user;
+  float *eigenvalues_dst;  /* adaptive only (NULL: skip) */
+  float *eigenvectors_dst;
+  int32_t *counts_dst;
+  lam_host_fn eigen_done_fn;
+  void *eigen_done_user;
+} lam_block_io;
+
+/* Enqueue one block on the next buffer's stream (execute_pipeline,
+ * pipeline_base.hpp:28-29).  Returns after enqueueing. */
+int lam_execute(lam_ctx *ctx, const lam_block_io *io);
+
+/* Enqueue a copy of the visibility accumulator into dst (float32 layout
+ * above, gains/phases applied) and reset it (dump_visibilities,
+ * lambda_corr_beam_only_gpu_pipeline.hpp:660-701).  dst may be host or
+ * device memory; done_fn runs when the copy has landed. */
+int lam_dump_visibilities(lam_ctx *ctx, float *dst, lam_host_fn done_fn, void *user);
+/* Exact form of the same: int64 [C][R(R+1)/2][P][P][2] raw integer sums
+ * (no gains or phases), then reset. */
+int lam_dump_visibilities_i64(lam_ctx *ctx, int64_t *dst, lam_host_fn done_fn, void *user);
+/* Blocks accumulated since the last dump. */
+int lam_blocks_accumulated(const lam_ctx *ctx);
+
+/* Wait for all enqueued work; reports LAM_ERR_RANGE if any sample scaled
+ * out of int8 range since the last call. */
+int lam_sync(lam_ctx *ctx);
+
+/* Device stream of the buffer that the next lam_execute will use. */
+void *lam_next_stream(lam_ctx *ctx);
+
+/* hipGraph capture of the per-block kernel sequence: 1 = use graphs
+ * (default; LAM_DISABLE_GRAPH=1 in the environment turns it off like
+ * SPATIAL_DISABLE_CUDA_GRAPH, lambda_corr_beam_only_gpu_pipeline.hpp:561). */
+int lam_set_graphs(lam_ctx *ctx, int enable);
+
+/* Per-kernel timing: when enabled, HIP events bracket every kernel of each
+ * block on its stream; lam_kernel_times returns the accumulated milliseconds
+ * and launch counts per kernel id (see LAM_K_* below) and resets them. */
+enum { LAM_K_CORRBEAM = 0, LAM_K_CORR = 1, LAM_K_EIG = 2, LAM_K_BEAM2B = 3, LAM_K_COUNT = 4 };
+int lam_set_profiling(lam_ctx *ctx, int enable);
+int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);
+
+/* Tunables picked by the autotuner: time samples per workgroup of the
+ * voltage kernels (multiple of 64; 0 = automatic). */
+int lam_set_tuning(lam_ctx *ctx, int samples_per_wg);
+
+/* ---- kernel-level entry points (device pointers, stream = NULL for the
+ * default stream); used by unit tests the way the reference tests its
+ * kernels directly (tests/test_adaptive_pipeline.cu:743-1129). ---- */
+
+/* Batched eigen / RFI-null stage on nmat covariance matrices given as
+ * float32 complex [nmat][n][n] column-major (full Hermitian), n <= 64:
+ * whitening (spatial.cuh:1065-1093), Jacobi eig, ascending sort.
+ * evals [nmat][n], evecs [nmat][n][n][2] column-major, sweeps [nmat] (may be NULL). */
+int lam_eigh_batched(const float *d_A, int n, int nmat, int whiten, int max_sweeps,
+                     float tol, float *d_evals, float *d_evecs, int32_t *d_sweeps,
+                     void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,383 @@
+"""GPU parity of the HIP path against the CPU oracle and the reference's own
+golden values, through the C ABI (liblam.so).
+
+Bars: visibilities bit-exact (integer); beams within 1e-3 normwise relative
+error of the FP64 oracle (fp16 output rounding included); eigenvalues of the
+whitened covariance within 1e-4 relative to the largest; projections within
+1e-4.  Reference tests cited per case."""
+import sys
+
+import numpy as np
+import pytest
+
+import lam
+import lam_synth as syn
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+BEAM_TOL = 1e-3
+
+
+def _o(s):
+    return orc.Shape(C=s.C, F=s.F, NT=s.NT, R=s.R, P=s.P, RPP=s.RPP, NP=s.NP, B=s.B)
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def bl(i, j):
+    return j * (j + 1) // 2 + i
+
+
+def rel_err(a, b):
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+SMALL = syn.BlockShape(C=1, F=1, NT=8, R=4, P=2, RPP=4, NP=1, B=1, R_pad=32)
+TWO_CH = syn.BlockShape(C=2, F=1, NT=8, R=4, P=2, RPP=4, NP=1, B=1, R_pad=32)
+TWO_PKT = syn.BlockShape(C=1, F=1, NT=8, R=4, P=2, RPP=4, NP=2, B=1, R_pad=32)
+ADAPT = syn.BlockShape(C=1, F=1, NT=64, R=4, P=2, RPP=4, NP=1, B=1, R_pad=32)
+
+
+def corr_beam_run(shape, sample_fn, scale_fn=None, weights=None, nblocks=1):
+    _log(f"corr-beam run {shape}")
+    ring, scales = syn.ring_from_fn(shape, sample_fn, scale_fn)
+    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM)
+    w16 = syn.unity_weights(shape)[0] if weights is None else weights
+    p.set_weights(w16)
+    for _ in range(nblocks):
+        out = p.run_host(ring, scales)
+    vis = p.dump_visibilities()
+    p.close()
+    return out["beams"], vis, ring, scales
+
+
+# ------------------------------------------------ reference goldens (corr-beam)
+def test_corr_beam_exact_values():
+    """tests/test_corr_beam_pipeline.cu:107-140."""
+    beams, vis, _, _ = corr_beam_run(SMALL, lambda *a: (2, -2))
+    assert np.all(beams == 8 - 8j)
+    assert np.all(vis[..., 0] == 64.0) and np.all(vis[..., 1] == 0.0)
+
+
+def test_corr_beam_zero_and_weights():
+    """test_corr_beam_pipeline.cu:156-247."""
+    beams, vis, _, _ = corr_beam_run(SMALL, lambda *a: (0, 0))
+    assert not beams.any() and not vis.any()
+    zero_w = np.zeros((1, 2, 1, 4, 2), np.float16).view(np.uint16)
+    beams, vis, _, _ = corr_beam_run(SMALL, lambda *a: (2, -2), weights=zero_w)
+    assert not beams.any()
+    assert np.all(vis[0, [bl(r, r) for r in range(4)], 0, 0, 0] > 0)
+    two_w = syn.unity_weights(SMALL, 2.0)[0]
+    beams, vis, _, _ = corr_beam_run(SMALL, lambda *a: (2, -2), weights=two_w)
+    assert np.all(beams == 16 - 16j) and np.all(vis[..., 0] == 64)
+
+
+def test_corr_beam_two_channels_scales_packets():
+    """test_corr_beam_pipeline.cu:254-392."""
+    beams, vis, _, _ = corr_beam_run(TWO_CH, lambda c, *a: (2, -2) if c == 0 else (3, 1))
+    assert np.all(beams[0] == 8 - 8j) and np.all(beams[1] == 12 + 4j)
+    for r in range(4):
+        assert vis[0, bl(r, r), 0, 0, 0] == 64 and vis[1, bl(r, r), 1, 1, 0] == 80
+    beams, vis, _, _ = corr_beam_run(SMALL, lambda *a: (1, 0), lambda c, f, k, r, p: (r + 1) * (p + 1))
+    assert np.all(beams[0, 0] == 10) and np.all(beams[0, 1] == 20)
+    for r in range(4):
+        for p in range(2):
+            s = (r + 1) * (p + 1)
+            assert vis[0, bl(r, r), p, p, 0] == 8 * s * s
+    beams, vis, _, _ = corr_beam_run(TWO_PKT, lambda *a: (2, -2))
+    assert np.all(vis[0, [bl(r, r) for r in range(4)], 0, 0, 0] == 128)
+
+
+def test_legacy_tcc_golden_conjugation():
+    """tests/test_spatial.cpp:77-134 through the GPU correlator: 67+29j."""
+    x4 = [2 + 3j, 5 + 1j, -2 + 3j, 2 - 3j, 1 + 1j, 0, 10 + 2j, 4 + 4j]
+    x5 = [4 + 5j, 2 + 2j, 9 + 1j, -4 - 5j, 0, 1 - 1j, 2 + 2j, 2 + 2j]
+    shape = syn.BlockShape(C=1, F=1, NT=8, R=8, P=2, RPP=8, NP=1, B=1)
+
+    def fn(c, f, k, t, r, p):
+        if p != 0 or k != -1:  # corr-beam reads ring slot 0 = packet -1
+            return (0, 0)
+        v = x4[t] if r == 4 else (x5[t] if r == 5 else 0)
+        return (int(np.real(v)), int(np.imag(v)))
+    _, vis, _, _ = corr_beam_run(shape, fn)
+    assert tuple(vis[0, bl(4, 5), 0, 0]) == (67.0, 29.0)
+    assert tuple(vis[0, bl(4, 4), 0, 0]) == (203.0, 0.0)
+    assert tuple(vis[0, bl(5, 5), 0, 0]) == (190.0, 0.0)
+
+
+# ------------------------------------------------------- random-data parity
+@pytest.mark.parametrize("shape", [
+    syn.BlockShape(C=4, F=1, NT=64, R=36, P=2, RPP=36, NP=4, B=4),
+    syn.BlockShape(C=3, F=4, NT=16, R=36, P=2, RPP=9, NP=5, B=2),
+    syn.BlockShape(C=2, F=3, NT=8, R=6, P=2, RPP=2, NP=3, B=1),
+    syn.BlockShape(C=2, F=2, NT=32, R=24, P=2, RPP=12, NP=4, B=16),
+    syn.BlockShape(C=1, F=1, NT=64, R=64, P=2, RPP=64, NP=2, B=8),
+], ids=lambda s: f"C{s.C}F{s.F}NT{s.NT}R{s.R}NP{s.NP}B{s.B}")
+def test_corr_beam_random_bit_exact(shape):
+    """Random int8 (incl. -128): visibilities bit-exact over 2 accumulated
+    blocks, beams within 1e-3 of the FP64 oracle."""
+    _log(f"random corr-beam {shape}")
+    rng = np.random.default_rng(shape.C * 1000 + shape.R)
+    ring = rng.integers(-128, 128, shape.ring_shape(), dtype=np.int16).astype(np.int8)
+    ring.reshape(-1)[::97] = -128
+    scales = np.ones(shape.scale_shape(), np.int16)
+    w16, w = syn.random_weights(shape, seed=3)
+    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM, num_buffers=2)
+    p.set_weights(w16)
+    out1 = p.run_host(ring, scales)
+    out2 = p.run_host(ring, scales)
+    vis64 = p.dump_visibilities_i64()
+    p.close()
+    s = _o(shape)
+    y = orc.ingest(s, ring, scales, orc.WINDOW_CORR_BEAM)
+    ref_vis = orc.correlate(s, y)
+    assert np.array_equal(vis64, 2 * ref_vis)
+    ref_beams = orc.beamform(s, y, w)
+    for out in (out1, out2):
+        assert rel_err(out["beams"], ref_beams) < BEAM_TOL
+
+
+def test_corr_beam_float_dump_and_reset():
+    shape = syn.BlockShape(C=8, F=1, NT=64, R=36, P=2, RPP=36, NP=2, B=4)
+    rng = np.random.default_rng(5)
+    ring = rng.integers(-127, 128, shape.ring_shape(), dtype=np.int16).astype(np.int8)
+    scales = np.ones(shape.scale_shape(), np.int16)
+    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM)
+    p.set_weights(syn.random_weights(shape, 1)[0])
+    p.run_host(ring, scales)
+    f = p.dump_visibilities()
+    ref = orc.correlate(_o(shape), orc.ingest(_o(shape), ring, scales, orc.WINDOW_CORR_BEAM))
+    assert np.array_equal(f, ref.astype(np.float32))  # |V| < 2^24: exact in float
+    assert p.blocks_accumulated() == 0
+    again = p.dump_visibilities_i64()
+    assert not again.any()
+    p.close()
+
+
+def test_scales_and_missing_packets():
+    """Per-packet int16 scales (zeroed scale = missing packet,
+    packet_formats.hpp:94-112) applied before correlation."""
+    shape = syn.BlockShape(C=2, F=2, NT=16, R=8, P=2, RPP=4, NP=4, B=2)
+    rng = np.random.default_rng(11)
+    ring = rng.integers(-30, 31, shape.ring_shape(), dtype=np.int16).astype(np.int8)
+    scales = rng.integers(0, 4, shape.scale_shape()).astype(np.int16)
+    scales[0, 2, 4:8] = 0
+    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM)
+    w16, w = syn.random_weights(shape, 2)
+    p.set_weights(w16)
+    out = p.run_host(ring, scales)
+    vis = p.dump_visibilities_i64()
+    p.close()
+    s = _o(shape)
+    y = orc.ingest(s, ring, scales, orc.WINDOW_CORR_BEAM)
+    assert np.array_equal(vis, orc.correlate(s, y))
+    assert rel_err(out["beams"], orc.beamform(s, y, w)) < BEAM_TOL
+
+
+def test_range_error_is_reported():
+    shape = SMALL
+    ring, scales = syn.ring_from_fn(shape, lambda *a: (100, 0), lambda *a: 3)
+    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM)
+    p.set_weights(syn.unity_weights(shape)[0])
+    beams = p.beams_array()
+    p.execute(ring, scales, beams_out=beams)
+    with pytest.raises(lam.LamError) as ei:
+        p.sync()
+    assert ei.value.code == lam.LAM_ERR_RANGE
+    p.close()
+
+
+def test_gains_and_fine_delays():
+    """Gains g_i conj(g_j) on visibilities and folded into the beams; fine
+    delays as the channel-centre phase exp(-2 pi i f_c tau)."""
+    shape = syn.BlockShape(C=2, F=1, NT=32, R=8, P=2, RPP=8, NP=2, B=2)
+    rng = np.random.default_rng(4)
+    ring = rng.integers(-60, 61, shape.ring_shape(), dtype=np.int16).astype(np.int8)
+    scales = np.ones(shape.scale_shape(), np.int16)
+    g = (rng.standard_normal((2, 2, 8)) + 1j * rng.standard_normal((2, 2, 8))).astype(np.complex64)
+    tau = rng.uniform(-5, 5, 8).astype(np.float32)
+    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM)
+    w16, w = syn.random_weights(shape, 6)
+    p.set_weights(w16)
+    p.set_gains(g)
+    p.set_fine_delays(tau, 0.0, 781250.0, 64)
+    out = p.run_host(ring, scales)
+    vis = p.dump_visibilities()
+    p.close()
+    s = _o(shape)
+    y = orc.ingest(s, ring, scales, orc.WINDOW_CORR_BEAM)
+    raw = orc.correlate(s, y)
+    fc = (64 + np.arange(2)) * 781250.0
+    ph = np.exp(-2j * np.pi * fc[:, None] * tau[None, :].astype(np.float64) * 1e-9)  # [C][R]
+    ge = g.astype(np.complex128) * ph[:, None, :]  # effective per-(c,p,r) factor
+    ref = np.zeros(raw.shape[:-1], complex)
+    for j in range(8):
+        for i in range(j + 1):
+            v = raw[:, bl(i, j), :, :, 0] + 1j * raw[:, bl(i, j), :, :, 1]
+            ref[:, bl(i, j)] = v * ge[:, :, i][:, :, None] * np.conj(ge[:, :, j])[:, None, :]
+    got = vis[..., 0] + 1j * vis[..., 1]
+    assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-5
+    ref_beams = orc.beamform(s, y, w * ge[:, :, None, :])
+    assert rel_err(out["beams"], ref_beams) < BEAM_TOL
+
+
+# ---------------------------------------------------------- eigensolver
+@pytest.mark.parametrize("n", [2, 4, 5, 12, 24, 36, 64])
+def test_eigh_batched_vs_oracle(n):
+    import torch
+    _log(f"eigh n={n}")
+    rng = np.random.default_rng(n)
+    nmat = 7
+    As = []
+    for _ in range(nmat):
+        X = rng.standard_normal((n, 2 * n)) + 1j * rng.standard_normal((n, 2 * n))
+        As.append(X @ X.conj().T / (2 * n))
+    As = np.array(As)
+    buf = np.stack([As.transpose(0, 2, 1).real, As.transpose(0, 2, 1).imag], -1).astype(np.float32)
+    dA = torch.from_numpy(np.ascontiguousarray(buf)).cuda()
+    dev = torch.zeros((nmat, n), dtype=torch.float32, device="cuda")
+    dvec = torch.zeros((nmat, n, n, 2), dtype=torch.float32, device="cuda")
+    dsw = torch.zeros(nmat, dtype=torch.int32, device="cuda")
+    lam.eigh_batched(dA, n, nmat, dev, dvec, dsw, max_sweeps=20, tol=1e-7)
+    torch.cuda.synchronize()
+    ev = dev.cpu().numpy()
+    V = dvec.cpu().numpy()
+    V = V[..., 0] + 1j * V[..., 1]  # [m][k][i]
+    for m in range(nmat):
+        ref_ev, ref_V, _ = orc.jacobi_eigh(As[m])
+        scale = np.max(np.abs(ref_ev))
+        assert np.max(np.abs(ev[m] - ref_ev)) / scale < 2e-5
+        U = V[m].T  # columns
+        resid = np.linalg.norm(As[m] @ U - U * ev[m][None, :]) / np.linalg.norm(As[m])
+        assert resid < 1e-4
+        assert np.allclose(U.conj().T @ U, np.eye(n), atol=1e-4)
+
+
+# ---------------------------------------------------------- adaptive path
+def adaptive_run(shape, sample_fn, K=1, shrink=False, detect=False, delta=3.0, weights=None):
+    ring, scales = syn.ring_from_fn(shape, sample_fn)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+    p.set_weights(syn.unity_weights(shape)[0] if weights is None else weights)
+    p.set_fixed_k(np.full((shape.C, 2), K))
+    p.set_nulling(detect=detect, delta=delta, shrink=shrink)
+    p.set_eig_params(20, 1e-7)
+    out = p.run_host(ring, scales)
+    p.close()
+    return out
+
+
+def test_adaptive_null_rank_one():
+    """test_adaptive_pipeline.cu:414-423, 491-511."""
+    out = adaptive_run(ADAPT, lambda *a: (2, -2), K=1)
+    b = out["beams"]
+    assert np.all(b[0, :, 0] == 8 - 8j)
+    assert np.sum(np.abs(b[0, :, 1]) ** 2) < 1.0
+    assert np.all(np.abs(b[0, :, 1].real) <= 0.25) and np.all(np.abs(b[0, :, 1].imag) <= 0.25)
+    ev = out["evals_out"][0]
+    assert np.all(np.diff(ev, axis=-1) >= -1e-3) and np.all(ev >= -1e-3)
+
+
+def test_adaptive_k0_tagged_layout():
+    """test_adaptive_pipeline.cu:390-489, 539-567 (null and shrink)."""
+    def tagged(c, f, k, t, r, p):
+        return (t + 1, -(1 + 10 * p + 20 * c)) if r == 0 else (0, 0)
+    for shape in (ADAPT, syn.BlockShape(C=2, F=1, NT=64, R=4, P=2, RPP=4, NP=1, B=1)):
+        for shrink in (False, True):
+            out = adaptive_run(shape, tagged, K=0, shrink=shrink)
+            b = out["beams"]
+            for c in range(shape.C):
+                for p in range(2):
+                    exp = np.arange(1, 65) - 1j * (1 + 10 * p + 20 * c)
+                    assert np.array_equal(b[c, p, 0], exp) and np.array_equal(b[c, p, 1], exp)
+
+
+def test_adaptive_shrink_vs_null_and_detection():
+    """test_adaptive_pipeline.cu:622-674."""
+    out = adaptive_run(ADAPT, lambda *a: (2, -2), K=1, shrink=True)
+    assert np.sum(np.abs(out["beams"][0, :, 1]) ** 2) < 1.0
+
+    def rank2(c, f, k, t, r, p):
+        if t < 32:
+            return (2, -2)
+        return (4, 0) if r == 0 else (0, 0)
+    pn = np.sum(np.abs(adaptive_run(ADAPT, rank2, K=1)["beams"][0, :, 1]) ** 2)
+    ps = np.sum(np.abs(adaptive_run(ADAPT, rank2, K=1, shrink=True)["beams"][0, :, 1]) ** 2)
+    assert ps > pn + 1e-6
+
+    def multimode(c, f, k, t, r, p):
+        if p == 1:
+            return (0, 0)
+        a = 20 if (t // 8) % 2 == 0 else -20
+        b = 18 if (t // 4) % 2 == 0 else -18
+        return (a + (1 if r % 2 == 0 else -1) * b, 0)
+    out = adaptive_run(ADAPT, multimode, K=0, detect=True, delta=0.5)
+    assert list(out["counts_out"][0]) == [2, 0]
+    assert np.sum(np.abs(out["beams"][0, 0, 1]) ** 2) < 1.0
+    assert np.sum(np.abs(out["beams"][0, 1, 1]) ** 2) == 0.0
+
+
+@pytest.mark.parametrize("shrink", [False, True])
+def test_adaptive_scene_vs_oracle(shrink):
+    """Synthetic scene (2 CW tones, 36 antennas): eigenvalues, K, projected
+    beams and the nulled subspace against the FP64 oracle."""
+    shape = syn.BlockShape(C=4, F=1, NT=64, R=36, P=2, RPP=36, NP=4, B=4)
+    _log(f"adaptive scene shrink={shrink}")
+    scene = syn.make_scene(shape, seed=21, n_tones=2, inr_db=(10, 25))
+    ring, scales = syn.make_block(shape, scene, seed=22)
+    w16, w = syn.random_weights(shape, 23)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+    p.set_weights(w16)
+    p.set_fixed_k(np.full((shape.C, 2), 2))
+    p.set_nulling(shrink=shrink)
+    p.set_eig_params(20, 1e-7)
+    out = p.run_host(ring, scales)
+    vis = p.dump_visibilities_i64()
+    p.close()
+    s = _o(shape)
+    y = orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)
+    ref = orc.adaptive_block(s, y, w, fixed_k=np.full((shape.C, 2), 2), shrink=shrink)
+    assert np.array_equal(vis, ref["vis"])
+    evg = out["evals_out"]
+    assert np.max(np.abs(evg - ref["evals"]) / np.max(np.abs(ref["evals"]), axis=-1, keepdims=True)) < 1e-4
+    assert list(out["counts_out"].ravel()) == list(ref["counts"].ravel())
+    assert rel_err(out["beams"][:, :, :4], ref["beams2b"][:, :, :4]) < BEAM_TOL
+    assert rel_err(out["beams"][:, :, 4:], ref["beams2b"][:, :, 4:]) < BEAM_TOL
+    # nulled subspace: projector onto the K largest eigenvectors
+    V = out["evecs_out"][..., 0] + 1j * out["evecs_out"][..., 1]
+    for c in range(shape.C):
+        for pp in range(2):
+            Ug = V[c, pp, -2:].T
+            Ur = ref["evecs"][c, pp, -2:].T
+            assert np.max(np.abs(Ug @ Ug.conj().T - Ur @ Ur.conj().T)) < 1e-4
+
+
+def test_adaptive_delays_and_permutation():
+    """apply_delays (spatial.cuh:1420-1506) and reorder_streams
+    (spatial.cuh:593-649) folded into the voltage loads; K=0 beams = W x."""
+    shape = syn.BlockShape(C=2, F=2, NT=16, R=8, P=2, RPP=4, NP=3, B=2)
+    rng = np.random.default_rng(9)
+    ring = rng.integers(-50, 51, shape.ring_shape(), dtype=np.int16).astype(np.int8)
+    scales = np.ones(shape.scale_shape(), np.int16)
+    delays = np.array([3, -5], np.int32)
+    recv = np.array([1, 1, 0, 2, -1, 3, 4, 5, 6, 6, 7, 7, 5, 4, 0, 0], np.int32)
+    pol = np.array([1, 0, 0, 1, 0, 0, 1, 1, 0, 1, 1, 0, 0, 1, 1, 0], np.int32)
+    w16, w = syn.random_weights(shape, 12)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+    p.set_weights(w16)
+    p.set_fixed_k(np.zeros((2, 2)))
+    p.set_subpacket_delays(delays)
+    p.set_stream_permutation(recv, pol)
+    out = p.run_host(ring, scales)
+    vis = p.dump_visibilities_i64()
+    p.close()
+    s = _o(shape)
+    y = orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE, delays=delays, recv_perm=recv, pol_perm=pol)
+    assert np.array_equal(vis, orc.correlate(s, y))
+    ref = orc.beamform(s, y, w)
+    assert rel_err(out["beams"][:, :, :2], ref) < BEAM_TOL
+    assert rel_err(out["beams"][:, :, 2:], ref) < BEAM_TOL
