@@ -95,15 +95,12 @@ def main():
                                    "(channels shard with no exchange; see DESIGN.md)"}))
         sys.exit(2)
 
-    import torch
-
-    torch.cuda.set_device(0)
     shape = syn.BlockShape(C=args.channels, F=1, NT=64, R=36, P=2, RPP=36, NP=128, B=16,
                            blocks_to_accumulate=4)
     log(f"generating {shape.C}x{shape.T} block ...")
     scene, ring, scales, w16, w = make_inputs(shape, seed=20261203 + 2)
-    d_ring = torch.from_numpy(ring).cuda()
-    d_scales = torch.from_numpy(scales).cuda()
+    d_ring = lam.DeviceArray.from_numpy(ring)
+    d_scales = lam.DeviceArray.from_numpy(scales)
     mode = lam.MODE_ADAPTIVE if args.mode == "adaptive" else lam.MODE_CORR_BEAM
     p = lam.Pipeline(shape, mode, num_buffers=1)
     p.set_weights(w16)
@@ -123,12 +120,11 @@ def main():
     p.dump_visibilities_i64()
 
     log(f"timing {args.steps} steps ...")
-    torch.cuda.synchronize()
+    p.sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
         p.execute(d_ring, d_scales, on_device=True)
     p.sync()
-    torch.cuda.synchronize()
     t1 = time.perf_counter()
     kt = p.kernel_times()
     p.close()

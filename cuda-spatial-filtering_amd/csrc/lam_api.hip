@@ -123,6 +123,9 @@ VoltParams base_volt(lam_ctx *ctx, const void *samples, const void *scales) {
   p.RPP = ctx->RPP;
   p.NP = ctx->NP;
   p.T = ctx->T;
+  p.nt_log2 = -1;
+  for (int l = 0; l < 30; ++l)
+    if ((1 << l) == ctx->NT) p.nt_log2 = l;
   pick_split(ctx, p.samples_per_wg, p.nsplit);
   p.xcd_map = (ctx->C % 8 == 0) ? 1 : 0;
   p.err = ctx->d_err;
@@ -637,6 +640,51 @@ int lam_sync(lam_ctx *ctx) {
                 "a scaled sample left the int8 range of the exact correlator (flag %d)", flag);
   }
   return LAM_OK;
+}
+
+void *lam_device_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes ? bytes : 16) != hipSuccess) return nullptr;
+  if (hipMemset(p, 0, bytes ? bytes : 16) != hipSuccess) {
+    (void)hipFree(p);
+    return nullptr;
+  }
+  return p;
+}
+
+void lam_device_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+int lam_memcpy(void *dst, const void *src, size_t bytes) {
+  if (!dst || !src) return LAM_ERR_ARG;
+  return hipMemcpy(dst, src, bytes, hipMemcpyDefault) == hipSuccess ? LAM_OK : LAM_ERR_HIP;
+}
+
+int lam_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  int count = 0;
+  for (int d = 0; d < n; ++d) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+      ++count;
+  }
+  return count;
+}
+
+void *lam_host_alloc(size_t bytes) {
+  void *p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) return nullptr;
+  std::memset(p, 0, bytes);
+  return p;
+}
+
+void lam_host_free(void *p) {
+  if (p) (void)hipHostFree(p);
 }
 
 int lam_eigh_batched(const float *d_A, int n, int nmat, int whiten, int max_sweeps, float tol,

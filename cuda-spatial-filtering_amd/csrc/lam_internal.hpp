@@ -28,6 +28,7 @@ struct VoltParams {
   const int16_t *scales;   // [C][NP+2][R][2]
   int C, F, NT, R, RPP, NP, T;
   int window_base;         // 0: corr-beam window (slots 0..NP-1); NT: adaptive
+  int nt_log2;             // log2(NT) when NT is a power of two, else -1
   const int *delays;       // [F] per-FPGA sample delays or nullptr
   const int *recv_perm;    // [R*2] or nullptr
   const int *pol_perm;     // [R*2] or nullptr

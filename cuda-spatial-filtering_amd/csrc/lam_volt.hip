@@ -57,6 +57,9 @@ __device__ __forceinline__ uint32_t scale_pair(uint32_t pair, int s, bool &bad) 
 // Per-thread view of the ring block: a loader thread owns one antenna and
 // fetches its dword (both pols, re/im) for a strided set of time rows.  The
 // FPGA, delay, permutation and channel base are resolved once per thread.
+// fetch() only issues loads (sample and scale dwords, no data-dependent
+// branches, so all of a chunk's loads are in flight together); apply() scales
+// them when the chunk is written to LDS.
 template <bool PERM>
 struct ThreadLoader {
   const uint32_t *chan32;  // channel base of the samples, dwords
@@ -87,33 +90,45 @@ struct ThreadLoader {
     }
   }
 
-  // Dword of sample s (window index) for this antenna, scaled.
-  __device__ uint32_t load(const VoltParams &P, int s, bool &bad) const {
-    if (!PERM) {
-      const int g = s + off;
-      const int slot = g / P.NT, tt = g - slot * P.NT;
-      const uint32_t didx = (uint32_t)((slot * P.F + f) * P.NT + tt) * P.RPP + n;
-      uint32_t x = chan32[didx];
-      const uint32_t sc = reinterpret_cast<const uint32_t *>(cscale)[slot * P.R + a];
-      if (sc != 0x00010001u) {
-        const int s0 = (int)(int16_t)(sc & 0xffffu), s1 = (int)(int16_t)(sc >> 16);
-        x = scale_pair(x & 0xffffu, s0, bad) | (scale_pair(x >> 16, s1, bad) << 16);
-      }
-      return x;
+  __device__ __forceinline__ void slot_of(const VoltParams &P, int g, int &slot, int &tt) const {
+    if (P.nt_log2 >= 0) {
+      slot = g >> P.nt_log2;
+      tt = g & (P.NT - 1);
+    } else {
+      slot = g / P.NT;
+      tt = g - slot * P.NT;
     }
-    uint32_t out = 0;
+  }
+
+  // Issues the loads of window sample s: x = sample dword, sc = scale dword
+  // (identity path) or x = two 16-bit samples, sc = two int16 scales (permutation).
+  __device__ __forceinline__ void fetch(const VoltParams &P, int s, uint32_t &x, uint32_t &sc) const {
+    if (!PERM) {
+      int slot, tt;
+      slot_of(P, s + off, slot, tt);
+      x = chan32[(uint32_t)((slot * P.F + f) * P.NT + tt) * P.RPP + n];
+      sc = reinterpret_cast<const uint32_t *>(cscale)[slot * P.R + a];
+      return;
+    }
+    x = 0;
+    sc = 0;
 #pragma unroll
     for (int po = 0; po < 2; ++po) {
-      if (src[po] < 0) continue;
-      const int g = s + poff[po];
-      const int slot = g / P.NT, tt = g - slot * P.NT;
+      const int ss = src[po] < 0 ? 0 : src[po];
+      int slot, tt;
+      slot_of(P, s + poff[po], slot, tt);
       const uint32_t hidx = ((uint32_t)((slot * P.F + pf[po]) * P.NT + tt) * P.RPP + pn[po]) * 2 + sp[po];
-      uint32_t x = chan16[hidx];
-      const int sc = cscale[(slot * P.R + src[po]) * 2 + sp[po]];
-      if (sc != 1) x = scale_pair(x, sc, bad);
-      out |= x << (16 * po);
+      const uint32_t v = chan16[hidx];
+      const uint32_t s16 = (uint16_t)cscale[(slot * P.R + ss) * 2 + sp[po]];
+      x |= (src[po] < 0 ? 0u : v) << (16 * po);
+      sc |= (src[po] < 0 ? 0u : s16) << (16 * po);
     }
-    return out;
+  }
+
+  __device__ __forceinline__ static uint32_t apply(uint32_t x, uint32_t sc, bool &bad) {
+    if (sc == 0x00010001u) return x;
+    const int s0 = (int)(int16_t)(sc & 0xffffu), s1 = (int)(int16_t)(sc >> 16);
+    return scale_pair(x & 0xffffu, s0, bad) | (scale_pair(x >> 16, s1, bad) << 16);
   }
 };
 
@@ -226,7 +241,7 @@ volt_kernel(VoltParams P) {
   constexpr int LTMIN = TR::NT / TR::NA;  // LT >= LTMIN since R <= NA
   constexpr int MAXLOAD = (CH + LTMIN - 1) / LTMIN;
   const int nload = (CH + LT - 1) / LT;
-  uint32_t pre[MAXLOAD];
+  uint32_t pre[MAXLOAD], psc[MAXLOAD];
   auto fetch = [&](int ch) {
     const int s0 = s_begin + ch * CH + tr;
 #pragma unroll
@@ -234,7 +249,12 @@ volt_kernel(VoltParams P) {
       if (k < nload) {
         const int t = tr + k * LT;
         const int s = s0 + k * LT;
-        pre[k] = (loader && t < CH && s < s_end) ? ld.load(P, s, bad) : 0u;
+        if (loader && t < CH && s < s_end) {
+          ld.fetch(P, s, pre[k], psc[k]);
+        } else {
+          pre[k] = 0u;
+          psc[k] = 0x00010001u;
+        }
       }
     }
   };
@@ -243,7 +263,7 @@ volt_kernel(VoltParams P) {
     for (int k = 0; k < MAXLOAD; ++k) {
       if (k < nload) {
         const int t = tr + k * LT;
-        if (loader && t < CH) lds[buf][t * RSD + ld.a] = pre[k];
+        if (loader && t < CH) lds[buf][t * RSD + ld.a] = ThreadLoader<PERM>::apply(pre[k], psc[k], bad);
       }
     }
   };

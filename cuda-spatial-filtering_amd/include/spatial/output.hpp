@@ -1,0 +1,119 @@
+// Output sink contract of the pipelines: the reference's Output interface
+// (include/spatial/output.hpp:22-58) verbatim, and SingleHostMemoryOutput
+// (output.hpp:60-164) with pinned buffers from lam_host_alloc.  The buffered
+// writer machinery (BufferedOutput, HDF5/PSRDADA writers) is not part of this path.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+
+#include "lam.h"
+#include "spatial/packet_formats.hpp"
+
+class Output {
+ public:
+  virtual ~Output() = default;
+  virtual size_t register_beam_data_block(const size_t start_seq_num, const size_t end_seq_num) = 0;
+  virtual size_t register_visibilities_block(const size_t start_seq_num, const size_t end_seq_num,
+                                             const int num_missing_packets,
+                                             const int num_total_packets) = 0;
+  virtual size_t register_eigendecomposition_data_block(const size_t start_seq_num,
+                                                        const size_t end_seq_num) = 0;
+  virtual size_t register_fft_block(const size_t start_seq_num, const size_t end_seq_num) = 0;
+
+  virtual void *get_beam_data_landing_pointer(const size_t block_num) = 0;
+  virtual void *get_visibilities_landing_pointer(const size_t block_num) = 0;
+  virtual void *get_arrivals_data_landing_pointer(const size_t block_num) = 0;
+  virtual void *get_eigenvalues_data_landing_pointer(const size_t block_num) = 0;
+  virtual void *get_eigenvectors_data_landing_pointer(const size_t block_num) = 0;
+  virtual void *get_nulled_eigenmode_counts_landing_pointer(const size_t block_num) = 0;
+  virtual void *get_beam_counts_landing_pointer(const size_t block_num) = 0;
+  virtual void *get_fft_landing_pointer(const size_t block_num) = 0;
+
+  virtual void register_beam_data_transfer_complete(const size_t block_num) = 0;
+  virtual void register_visibilities_transfer_complete(const size_t block_num) = 0;
+  virtual void register_arrivals_transfer_complete(const size_t block_num) = 0;
+  virtual void register_eigendecomposition_data_transfer_complete(const size_t block_num) = 0;
+  virtual void register_beam_counts_transfer_complete(const size_t block_num) = 0;
+  virtual void register_fft_transfer_complete(const size_t block_num) = 0;
+};
+
+// One landing area per product; every register_* returns block 1 (output.hpp:86-111).
+template <typename T, size_t NR_OUT_BEAMS = T::NR_BEAMS>
+class SingleHostMemoryOutput : public Output {
+ public:
+  using BeamOutput = spatial::half_t[T::NR_CHANNELS][T::NR_POLARIZATIONS][NR_OUT_BEAMS]
+                                    [T::NR_PACKETS_FOR_CORRELATION * T::NR_TIME_STEPS_PER_PACKET][2];
+  using Visibilities = float[T::NR_CHANNELS][T::NR_BASELINES_UNPADDED][T::NR_POLARIZATIONS]
+                            [T::NR_POLARIZATIONS][2];
+  using Arrivals = bool[T::NR_CHANNELS][T::NR_PACKETS_FOR_CORRELATION + 2][T::NR_FPGA_SOURCES];
+  // The adaptive pipeline writes eigenvalues [C][P][R] and eigenvectors
+  // [C][P][R][R] into these (lambda_adaptive_beamformed_spectra_pipeline.hpp:914-950).
+  using Eigenvalues = float[T::NR_CHANNELS][T::NR_POLARIZATIONS][T::NR_RECEIVERS];
+  using Eigenvectors =
+      std::complex<float>[T::NR_CHANNELS][T::NR_POLARIZATIONS][T::NR_RECEIVERS][T::NR_RECEIVERS];
+  using EigenmodeCounts = int32_t[T::NR_CHANNELS][T::NR_POLARIZATIONS];
+
+  BeamOutput *beam_data;
+  Visibilities *visibilities;
+  Arrivals *arrivals;
+  Eigenvalues *eigenvalues;
+  Eigenvectors *eigenvectors;
+  EigenmodeCounts *nulled_eigenmode_counts;
+
+  size_t register_beam_data_block(const size_t, const size_t) override { return 1; }
+  size_t register_visibilities_block(const size_t, const size_t, const int missing,
+                                     const int total) override {
+    last_missing_packets = missing;
+    last_total_packets = total;
+    return 1;
+  }
+  size_t register_eigendecomposition_data_block(const size_t, const size_t) override { return 1; }
+  size_t register_fft_block(const size_t, const size_t) override { return 1; }
+
+  void *get_beam_data_landing_pointer(const size_t) override { return beam_data; }
+  void *get_visibilities_landing_pointer(const size_t) override { return visibilities; }
+  void *get_arrivals_data_landing_pointer(const size_t) override { return arrivals; }
+  void *get_eigenvalues_data_landing_pointer(const size_t) override { return eigenvalues; }
+  void *get_eigenvectors_data_landing_pointer(const size_t) override { return eigenvectors; }
+  void *get_nulled_eigenmode_counts_landing_pointer(const size_t) override {
+    return nulled_eigenmode_counts;
+  }
+  void *get_beam_counts_landing_pointer(const size_t) override { return nulled_eigenmode_counts; }
+  void *get_fft_landing_pointer(const size_t) override { return nullptr; }
+
+  void register_beam_data_transfer_complete(const size_t) override { ++beam_blocks_done; }
+  void register_visibilities_transfer_complete(const size_t) override { ++vis_blocks_done; }
+  void register_arrivals_transfer_complete(const size_t) override {}
+  void register_eigendecomposition_data_transfer_complete(const size_t) override { ++eig_blocks_done; }
+  void register_beam_counts_transfer_complete(const size_t) override {}
+  void register_fft_transfer_complete(const size_t) override {}
+
+  int last_missing_packets = 0, last_total_packets = 0;
+  int beam_blocks_done = 0, vis_blocks_done = 0, eig_blocks_done = 0;
+
+  SingleHostMemoryOutput() {
+    beam_data = alloc<BeamOutput>();
+    visibilities = alloc<Visibilities>();
+    arrivals = alloc<Arrivals>();
+    eigenvalues = alloc<Eigenvalues>();
+    eigenvectors = alloc<Eigenvectors>();
+    nulled_eigenmode_counts = alloc<EigenmodeCounts>();
+  }
+  ~SingleHostMemoryOutput() override {
+    lam_host_free(beam_data);
+    lam_host_free(visibilities);
+    lam_host_free(arrivals);
+    lam_host_free(eigenvalues);
+    lam_host_free(eigenvectors);
+    lam_host_free(nulled_eigenmode_counts);
+  }
+
+ private:
+  template <typename A>
+  static A *alloc() {
+    A *p = static_cast<A *>(lam_host_alloc(sizeof(A)));
+    if (!p) throw std::runtime_error("lam_host_alloc failed");
+    return p;
+  }
+};

@@ -94,19 +94,77 @@ def load_library(path: str = LIB_PATH):
     L.lam_sync.argtypes = [vp]
     L.lam_next_stream.argtypes = [vp]
     L.lam_next_stream.restype = vp
+    L.lam_device_alloc.argtypes = [ctypes.c_size_t]
+    L.lam_device_alloc.restype = vp
+    L.lam_device_free.argtypes = [vp]
+    L.lam_memcpy.argtypes = [vp, vp, ctypes.c_size_t]
+    L.lam_device_count.restype = ctypes.c_int
+    L.lam_host_alloc.argtypes = [ctypes.c_size_t]
+    L.lam_host_alloc.restype = vp
+    L.lam_host_free.argtypes = [vp]
     L.lam_eigh_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_float, vp, vp, vp, vp]
     _lib = L
     return L
 
 
+class DeviceArray:
+    """Device buffer owned by liblam (hipMalloc through the C ABI) with numpy
+    shape/dtype metadata; avoids a second HIP runtime in the process."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.L = load_library()
+        self.ptr = self.L.lam_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise LamError(LAM_ERR_HIP, f"lam_device_alloc({self.nbytes}) failed")
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        if self.L.lam_memcpy(self.ptr, a.ctypes.data, self.nbytes) != LAM_OK:
+            raise LamError(LAM_ERR_HIP, "upload failed")
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        if self.L.lam_memcpy(out.ctypes.data, self.ptr, self.nbytes) != LAM_OK:
+            raise LamError(LAM_ERR_HIP, "download failed")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.L.lam_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    return load_library().lam_device_count()
+
+
 def _ptr(x):
-    """Raw address of a numpy array or torch tensor (host or device)."""
+    """Raw address of a numpy array (host), DeviceArray or torch tensor (device)."""
     if x is None:
         return None
     if isinstance(x, np.ndarray):
         assert x.flags.c_contiguous
         return x.ctypes.data
+    if isinstance(x, DeviceArray):
+        return x.ptr
     assert x.is_contiguous()
     return x.data_ptr()
 

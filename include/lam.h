@@ -171,6 +171,22 @@ int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);
  * voltage kernels (multiple of 64; 0 = automatic). */
 int lam_set_tuning(lam_ctx *ctx, int samples_per_wg);
 
+/* Pinned host memory for block buffers and output landing areas
+ * (cudaHostAlloc in LambdaFinalPacketData, packet_formats.hpp:131-142, and
+ * SingleHostMemoryOutput, output.hpp:146-154).  Returns NULL on failure. */
+void *lam_host_alloc(size_t bytes);
+void lam_host_free(void *p);
+
+/* Device memory for callers without their own HIP runtime (the Python
+ * binding, tests and benchmarks): plain hipMalloc/hipFree/hipMemcpy on the
+ * current device.  lam_memcpy is synchronous and accepts host or device
+ * pointers on either side. */
+void *lam_device_alloc(size_t bytes);
+void lam_device_free(void *p);
+int lam_memcpy(void *dst, const void *src, size_t bytes);
+/* Number of visible gfx950 devices (0 when none, e.g. on a build host). */
+int lam_device_count(void);
+
 /* ---- kernel-level entry points (device pointers, stream = NULL for the
  * default stream); used by unit tests the way the reference tests its
  * kernels directly (tests/test_adaptive_pipeline.cu:743-1129). ---- */

@@ -229,7 +229,6 @@ def test_gains_and_fine_delays():
 # ---------------------------------------------------------- eigensolver
 @pytest.mark.parametrize("n", [2, 4, 5, 12, 24, 36, 64])
 def test_eigh_batched_vs_oracle(n):
-    import torch
     _log(f"eigh n={n}")
     rng = np.random.default_rng(n)
     nmat = 7
@@ -239,14 +238,13 @@ def test_eigh_batched_vs_oracle(n):
         As.append(X @ X.conj().T / (2 * n))
     As = np.array(As)
     buf = np.stack([As.transpose(0, 2, 1).real, As.transpose(0, 2, 1).imag], -1).astype(np.float32)
-    dA = torch.from_numpy(np.ascontiguousarray(buf)).cuda()
-    dev = torch.zeros((nmat, n), dtype=torch.float32, device="cuda")
-    dvec = torch.zeros((nmat, n, n, 2), dtype=torch.float32, device="cuda")
-    dsw = torch.zeros(nmat, dtype=torch.int32, device="cuda")
+    dA = lam.DeviceArray.from_numpy(buf)
+    dev = lam.DeviceArray((nmat, n), np.float32)
+    dvec = lam.DeviceArray((nmat, n, n, 2), np.float32)
+    dsw = lam.DeviceArray((nmat,), np.int32)
     lam.eigh_batched(dA, n, nmat, dev, dvec, dsw, max_sweeps=20, tol=1e-7)
-    torch.cuda.synchronize()
-    ev = dev.cpu().numpy()
-    V = dvec.cpu().numpy()
+    ev = dev.numpy()
+    V = dvec.numpy()
     V = V[..., 0] + 1j * V[..., 1]  # [m][k][i]
     for m in range(nmat):
         ref_ev, ref_V, _ = orc.jacobi_eigh(As[m])
