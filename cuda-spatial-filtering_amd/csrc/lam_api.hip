@@ -31,7 +31,7 @@ struct Buf {
   int8_t *d_samples = nullptr;  // staging for host inputs
   int16_t *d_scales = nullptr;
   __half *d_beams = nullptr;
-  int *d_vis32 = nullptr;       // adaptive: this block's visibilities
+  int *d_vis_part = nullptr;    // this block's partial visibilities [split][C][NBL][8]
   v8h *d_wop2 = nullptr;        // adaptive: this block's 2B-beam weight operands
   float *d_evals = nullptr, *d_evecs = nullptr;
   int *d_counts = nullptr, *d_sweeps = nullptr;
@@ -107,8 +107,10 @@ void pick_split(lam_ctx *ctx, int &spw, int &nsplit) {
     const int target = std::max(1, (1024 + ctx->C - 1) / ctx->C);
     spw = (T + target - 1) / target;
   }
-  spw = ((spw + 127) / 128) * 128;
-  if (spw > ((T + 127) / 128) * 128) spw = ((T + 127) / 128) * 128;
+  spw = ((spw + kChunk - 1) / kChunk) * kChunk;
+  const int min_spw = (((T + kMaxSplits - 1) / kMaxSplits + kChunk - 1) / kChunk) * kChunk;
+  if (spw < min_spw) spw = min_spw;
+  if (spw > ((T + kChunk - 1) / kChunk) * kChunk) spw = ((T + kChunk - 1) / kChunk) * kChunk;
   nsplit = (T + spw - 1) / spw;
 }
 
@@ -154,16 +156,21 @@ struct Prof {
 // The per-block kernel sequence of one buffer (captured into a graph when enabled).
 int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
   hipStream_t s = b.stream;
+  const long long vis_count = (long long)ctx->C * ctx->NBL * 8;
+  const bool atomic_acc = ctx->nbuf > 1;
   if (ctx->mode == LAM_MODE_CORR_BEAM) {
     VoltParams p = base_volt(ctx, samples, scales);
     p.window_base = 0;  // ring slots 0..NP-1 (spatial.cuh:254-264)
-    p.vis64 = ctx->d_vis_acc;
+    p.vis_part = b.d_vis_part;
     p.wop = ctx->d_wop;
     p.nbeams_out = ctx->B;
     p.NMT = ctx->NMT1;
     p.beams = b.d_beams;
-    Prof pr(ctx, s, LAM_K_CORRBEAM);
-    HIPC(ctx, launch_volt(p, true, true, s));
+    {
+      Prof pr(ctx, s, LAM_K_CORRBEAM);
+      HIPC(ctx, launch_volt(p, true, true, s));
+    }
+    HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, ctx->d_vis_acc, atomic_acc, s));
     return LAM_OK;
   }
   // adaptive: correlate -> eig/null -> 2B beams
@@ -174,7 +181,7 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
   p.pol_perm = ctx->perm_set ? ctx->d_pol_perm : nullptr;
   {
     VoltParams pc = p;
-    pc.vis32 = b.d_vis32;
+    pc.vis_part = b.d_vis_part;
     Prof pr(ctx, s, LAM_K_CORR);
     HIPC(ctx, launch_volt(pc, true, false, s));
   }
@@ -183,9 +190,8 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
     e.C = ctx->C;
     e.R = ctx->R;
     e.B = ctx->B;
-    e.vis32 = b.d_vis32;
-    e.vis64 = ctx->d_vis_acc;
-    e.vis32_clear = b.d_vis32;
+    e.vis_part = b.d_vis_part;
+    e.nsplit = p.nsplit;
     e.weights = ctx->d_weights;
     e.fixed_k = ctx->d_fixed_k;
     e.detect = ctx->detect;
@@ -212,6 +218,7 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
     Prof pr(ctx, s, LAM_K_BEAM2B);
     HIPC(ctx, launch_volt(pb, false, true, s));
   }
+  HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, ctx->d_vis_acc, atomic_acc, s));
   return LAM_OK;
 }
 
@@ -344,8 +351,8 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
     chk(dalloc(ctx, &b.d_samples, ctx->samples_bytes));
     chk(dalloc(ctx, &b.d_scales, ctx->scales_bytes));
     chk(dalloc(ctx, &b.d_beams, ctx->beams_bytes));
+    chk(dalloc(ctx, &b.d_vis_part, (size_t)kMaxSplits * vis_elems * sizeof(int)));
     if (mode == LAM_MODE_ADAPTIVE) {
-      chk(dalloc(ctx, &b.d_vis32, vis_elems * sizeof(int)));
       chk(dalloc(ctx, &b.d_wop2, (size_t)ctx->C * 2 * ctx->NMT2 * ctx->NK * 2 * 64 * sizeof(v8h)));
       chk(dalloc(ctx, &b.d_evals, (size_t)ctx->C * 2 * ctx->R * sizeof(float)));
       chk(dalloc(ctx, &b.d_evecs, (size_t)ctx->C * 2 * ctx->R * ctx->R * 2 * sizeof(float)));
@@ -399,7 +406,7 @@ int lam_destroy(lam_ctx *ctx) {
     hipFree(b.d_samples);
     hipFree(b.d_scales);
     hipFree(b.d_beams);
-    hipFree(b.d_vis32);
+    hipFree(b.d_vis_part);
     hipFree(b.d_wop2);
     hipFree(b.d_evals);
     hipFree(b.d_evecs);

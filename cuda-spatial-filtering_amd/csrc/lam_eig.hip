@@ -226,8 +226,13 @@ __global__ void __launch_bounds__(kEigThreads) eig_null_kernel(EigParams P) {
     while ((long)j * (j + 1) / 2 > idx) --j;
     while ((long)(j + 1) * (j + 2) / 2 <= idx) ++j;
     const int i = idx - j * (j + 1) / 2;
-    const int *v = P.vis32 + (((long)c * nbl + idx) * 4 + p * 2 + p) * 2;
-    const float2 a = make_float2((float)v[0], (float)v[1]);
+    long re = 0, im = 0;
+    for (int k = 0; k < P.nsplit; ++k) {
+      const int *v = P.vis_part + ((((long)k * P.C + c) * nbl + idx) * 4 + p * 2 + p) * 2;
+      re += v[0];
+      im += v[1];
+    }
+    const float2 a = make_float2((float)re, (float)im);
     S.A[j * n + i] = a;
     S.A[i * n + j] = make_float2(a.x, -a.y);
   }
@@ -356,16 +361,6 @@ __global__ void __launch_bounds__(kEigThreads) eig_null_kernel(EigParams P) {
     v8h *dst = P.wop + ((cp * P.NMT + mt) * P.NK + j) * 2 * 64;
     dst[lane] = hi;
     dst[64 + lane] = lo;
-  }
-  // 9. fold this block's visibilities into the accumulator and clear them
-  for (int idx = tid; idx < nbl * 4; idx += nthr) {
-    const long bl = idx >> 2;
-    const int q = (idx >> 1) & 1, z = idx & 1;
-    const long o = (((long)c * nbl + bl) * 4 + p * 2 + q) * 2 + z;
-    if (P.vis64)
-      atomicAdd(reinterpret_cast<unsigned long long *>(&P.vis64[o]),
-                (unsigned long long)(long long)P.vis32[o]);
-    if (P.vis32_clear) P.vis32_clear[o] = 0;
   }
 }
 

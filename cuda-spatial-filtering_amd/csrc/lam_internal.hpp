@@ -11,12 +11,10 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-// Fixed launch geometry of the voltage kernels: 4 wave64 per workgroup and
-// 64-sample chunks staged through LDS.
-constexpr int kWaves = 4;
-constexpr int kThreads = 64 * kWaves;
-constexpr int kChunk = 64;
+// Voltage kernels stage 128-sample chunks through LDS.
+constexpr int kChunk = 128;
 constexpr int kMaxReceivers = 64;  // 4 antenna groups of 16
+constexpr int kMaxSplits = 16;     // time splits per channel (partial visibility buffers)
 constexpr int kEigThreads = 256;
 
 // Error bits reported through the device flag word.
@@ -35,9 +33,8 @@ struct VoltParams {
   int samples_per_wg;      // multiple of kChunk
   int nsplit;              // workgroups per channel
   int xcd_map;             // 1: keep all workgroups of a channel on one XCD
-  // correlator outputs (one of them, or neither)
-  int *vis32;              // [C][NBL][2][2][2] per-block buffer
-  long long *vis64;        // [C][NBL][2][2][2] accumulator
+  // correlator output: one int32 partial per time split [nsplit][C][NBL][2][2][2]
+  int *vis_part;
   // beamformer
   const v8h *wop;          // [C][2][NMT][NK][2][64] weight operands (hi, lo)
   int nbeams_out;          // beams per (channel, pol) written
@@ -50,9 +47,8 @@ struct VoltParams {
 // Parameters of the eigen / RFI-null kernel, one workgroup per (channel, pol).
 struct EigParams {
   int C, R, B;
-  const int *vis32;        // per-block visibilities [C][NBL][2][2][2]
-  long long *vis64;        // accumulator (block added, block buffer cleared) or nullptr
-  int *vis32_clear;        // == vis32 when the block buffer is to be zeroed
+  const int *vis_part;     // per-block partial visibilities [nsplit][C][NBL][2][2][2]
+  int nsplit;
   const __half *weights;   // [C][2][B][R][2]
   const int *fixed_k;      // [C][2]
   int detect;
@@ -74,6 +70,8 @@ hipError_t launch_eig(const EigParams &p, hipStream_t s);
 hipError_t launch_prep_weights(const __half *w, const float2 *gains, const float2 *phase,
                                int C, int B, int R, int NMT, int NK, v8h *wop,
                                hipStream_t s);
+hipError_t launch_accumulate_vis(const int *vis_part, int nsplit, long long count, long long *acc,
+                                 bool atomic, hipStream_t s);
 hipError_t launch_dump_vis(long long *acc, int C, int R, const float2 *gains,
                            const float2 *phase, float *out_f32, long long *out_i64,
                            hipStream_t s);

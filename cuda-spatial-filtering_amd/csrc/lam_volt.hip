@@ -1,4 +1,5 @@
-// Voltage kernel: fused ingest -> int8-MFMA correlator -> FP16-MFMA beamformer.
+// Voltage kernels: ingest -> int8-MFMA correlator -> FP16-MFMA beamformer in
+// one pass over the int8 voltages.
 //
 // Replaces, for one block, the reference chain
 //   scale_and_convert_to_half (spatial.cuh:420-486) -> packet_to_corr_input
@@ -8,33 +9,42 @@
 //   (lambda_corr_beam_only_gpu_pipeline.hpp:335-338) -> beam_ccglib_to_half_output
 //   (spatial.cuh:376-418)
 // and the adaptive pipeline's apply_delays / reorder_streams
-// (spatial.cuh:1420-1506, 593-649) with a single pass over the int8 voltages.
+// (spatial.cuh:1420-1506, 593-649).
 //
-// Workgroup = (channel, time split).  Each 64-sample chunk of the ring block is
-// copied into LDS as [time][antenna] dwords (one dword = 2 pols x re/im int8),
-// with the int16 scale, FPGA delay and stream permutation applied on the way.
+// Workgroup = (channel, time split).  128-sample chunks are staged in LDS as
+// rows of [antenna] dwords (one dword = 2 pols x re/im int8) with a row stride
+// of NA + 4 dwords (NA = antennas padded to a multiple of 16); that stride
+// keeps both MFMA operand reads free of bank conflicts.  Padding columns hold
+// finite garbage that only reaches discarded outputs or zero weights.
 //
 // Correlator: real rows are (antenna, component v = 2*pol + re/im).  Antennas
-// form groups of 16; an MFMA tile pairs component v of antenna group A with
-// component w of group B, so one v_mfma_i32_16x16x64_i8 consumes 64 time
-// samples.  Per (A <= B, p, q) three int32 accumulators hold
-//   re = sum Re_a Re_b + Im_a Im_b,  x = sum Im_a Re_b,  y = sum Re_a Im_b
-// and V = re + i (x - y) = sum x_a conj(x_b): exact integer arithmetic, no
-// negated operands (so -128 is representable).
+// form groups of 16; one wave per group pair (A <= B) runs
+// v_mfma_i32_16x16x64_i8 on operands built with a 4x4 byte transpose.  Per
+// (p, q) three int32 accumulators hold re = sum Re_a Re_b + Im_a Im_b,
+// x = sum Im_a Re_b, y = sum Re_a Im_b; V = re + i (x - y) = sum x_a conj(x_b),
+// exact, without negating operands.  Each workgroup stores its partial sums;
+// a later pass adds the time splits.
 //
-// Beamformer: per pol, [Re;Im] rows of 2B beams x K = (antenna, re/im) with
-// v_mfma_f32_32x32x16_f16; weights arrive pre-split into hi + lo FP16
-// operands so that the product keeps ~FP32 weight precision.
+// Beamformer: per (pol, 16-beam tile), [Re;Im] rows x K = (antenna, re/im) with
+// v_mfma_f32_32x32x16_f16; weights arrive as hi + lo FP16 operands.
+//
+// Two loaders:
+//   * fast (F == 1, R % 4 == 0, no permutation): a dedicated loader wave moves
+//     the contiguous ring bytes straight into a 3-stage LDS ring with
+//     global_load_lds_dwordx4 (no registers held, two chunks in flight);
+//     non-unit scales and partial chunks are fixed up in LDS.
+//   * general: every thread owns one antenna and prefetches its dwords into
+//     registers (FPGA segments, per-FPGA delays, permutation, scales).
 #include "lam_internal.hpp"
 
 namespace lam {
 
-// 4x4 byte transpose: x[k] holds bytes (c0,c1,c2,c3) of time k; y[c] holds
-// bytes (t0,t1,t2,t3) of component c.  v_perm_b32(hi, lo, sel) selects bytes
-// of the 64-bit value {hi, lo}.
+// ------------------------------------------------------------ small helpers
 __device__ __forceinline__ void transpose4x4(uint32_t x0, uint32_t x1, uint32_t x2,
                                              uint32_t x3, uint32_t &y0, uint32_t &y1,
                                              uint32_t &y2, uint32_t &y3) {
+  // x[k] holds bytes (c0,c1,c2,c3) of time k; y[c] = bytes (t0,t1,t2,t3) of
+  // component c.  v_perm_b32(hi, lo, sel) picks bytes of the 64-bit {hi, lo}.
   const uint32_t a_lo = __builtin_amdgcn_perm(x1, x0, 0x05010400u);
   const uint32_t a_hi = __builtin_amdgcn_perm(x1, x0, 0x07030602u);
   const uint32_t b_lo = __builtin_amdgcn_perm(x3, x2, 0x05010400u);
@@ -45,8 +55,7 @@ __device__ __forceinline__ void transpose4x4(uint32_t x0, uint32_t x1, uint32_t 
   y3 = __builtin_amdgcn_perm(b_hi, a_hi, 0x07060302u);
 }
 
-// Multiply the two int8 (re, im) of one pol by an int16 scale; flags values
-// that leave [-128, 127].
+// int8 (re, im) of one pol times an int16 scale; flags results outside [-128, 127].
 __device__ __forceinline__ uint32_t scale_pair(uint32_t pair, int s, bool &bad) {
   const int re = (int)(int8_t)(pair & 0xffu) * s;
   const int im = (int)(int8_t)((pair >> 8) & 0xffu) * s;
@@ -54,20 +63,312 @@ __device__ __forceinline__ uint32_t scale_pair(uint32_t pair, int s, bool &bad) 
   return ((uint32_t)re & 0xffu) | (((uint32_t)im & 0xffu) << 8);
 }
 
+__device__ __forceinline__ uint32_t apply_scale(uint32_t x, uint32_t sc, bool &bad) {
+  if (sc == 0x00010001u) return x;
+  const int s0 = (int)(int16_t)(sc & 0xffffu), s1 = (int)(int16_t)(sc >> 16);
+  return scale_pair(x & 0xffffu, s0, bad) | (scale_pair(x >> 16, s1, bad) << 16);
+}
+
+__device__ __forceinline__ void slot_of(const VoltParams &P, int g, int &slot, int &tt) {
+  if (P.nt_log2 >= 0) {
+    slot = g >> P.nt_log2;
+    tt = g & (P.NT - 1);
+  } else {
+    slot = g / P.NT;
+    tt = g - slot * P.NT;
+  }
+}
+
+// s_waitcnt with only vmcnt constrained (gfx9 encoding: vmcnt[3:0] | vmcnt[5:4] << 14).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// ------------------------------------------------------------- geometry
+template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT, bool LOADER_WAVE>
+struct Geo {
+  static constexpr int NA = 16 * NG;              // padded antennas
+  static constexpr int RSD = NA + 4;              // LDS row stride (dwords), == 4 mod 16
+  static constexpr int NK = 2 * NG - KODD;        // beam K steps of 8 antennas
+  static constexpr int NPAIRS = NG * (NG + 1) / 2;
+  static constexpr int NWC = DO_CORR ? NPAIRS : 0;        // correlator waves
+  static constexpr int NWB = DO_BEAM ? 2 * NMT : 0;       // beam waves: pol x 16-beam tile
+  static constexpr int NW = NWC + NWB + (LOADER_WAVE ? 1 : 0);
+  static constexpr int NT = 64 * NW;
+  static constexpr int STAGE = kChunk * RSD;      // dwords per staged chunk
+  static_assert(STAGE % 256 == 0, "stage must be whole dwordx4 wave loads");
+};
+
+// ------------------------------------------------------------ correlator
+// Operands of antenna group G: op[v] = 16 bytes of component v of antenna
+// 16G + (lane & 15) at 16 times of the 64-row k-step t0.  Lane group g = lane >> 4
+// reads rows t0 + 16*(e >> 2) + (e & 3) + 4g: with RSD == 4 (mod 16) the four
+// groups hit disjoint banks.
+template <int RSD>
+__device__ __forceinline__ void load_group(const uint32_t *L, int G, int t0, int lane, v4i op[4]) {
+  const int a = 16 * G + (lane & 15);
+  const int tb = t0 + 4 * (lane >> 4);
+  uint32_t x[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) x[e] = L[(tb + 16 * (e >> 2) + (e & 3)) * RSD + a];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint32_t y0, y1, y2, y3;
+    transpose4x4(x[4 * j], x[4 * j + 1], x[4 * j + 2], x[4 * j + 3], y0, y1, y2, y3);
+    op[0][j] = (int)y0;
+    op[1][j] = (int)y1;
+    op[2][j] = (int)y2;
+    op[3][j] = (int)y3;
+  }
+}
+
+__device__ __forceinline__ v4i mfma_i8(v4i a, v4i b, v4i c) {
+  return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+}
+
+template <int NG>
+__device__ __forceinline__ void pair_groups(int pi, int &A, int &B) {
+  A = 0;
+  int rem = pi;
+  while (rem >= NG - A) {
+    rem -= NG - A;
+    ++A;
+  }
+  B = A + rem;
+}
+
+// One chunk of correlation for the wave's group pair.
+template <int RSD>
+__device__ __forceinline__ void corr_chunk(const uint32_t *L, int gA, int gB, int lane, v4i acc[12]) {
+#pragma unroll 1
+  for (int t0 = 0; t0 < kChunk; t0 += 64) {
+    v4i opA[4], opB[4];
+    load_group<RSD>(L, gA, t0, lane, opA);
+    if (gB != gA) {
+      load_group<RSD>(L, gB, t0, lane, opB);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) opB[v] = opA[v];
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        v4i *a3 = &acc[(p * 2 + q) * 3];
+        a3[0] = mfma_i8(opA[2 * p], opB[2 * q], a3[0]);
+        a3[0] = mfma_i8(opA[2 * p + 1], opB[2 * q + 1], a3[0]);
+        a3[1] = mfma_i8(opA[2 * p + 1], opB[2 * q], a3[1]);
+        a3[2] = mfma_i8(opA[2 * p], opB[2 * q + 1], a3[2]);
+      }
+  }
+}
+
+// Partial visibilities of this workgroup: vis_part[split][c][bl(a,b)][p][q][re,im].
+__device__ __forceinline__ void corr_store(const VoltParams &P, const v4i acc[12], int gA, int gB,
+                                           int lane, int c, int split) {
+  const long nbl = (long)P.R * (P.R + 1) / 2;
+  int *base = P.vis_part + ((long)split * P.C + c) * nbl * 8;
+  const int b = 16 * gB + (lane & 15);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int a = 16 * gA + 4 * (lane >> 4) + v;
+    if (a > b || b >= P.R) continue;
+    int *o = base + ((long)b * (b + 1) / 2 + a) * 8;
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const v4i *a3 = &acc[(p * 2 + q) * 3];
+        reinterpret_cast<int2 *>(o)[p * 2 + q] = make_int2(a3[0][v], a3[1][v] - a3[2][v]);
+      }
+  }
+}
+
+// ------------------------------------------------------------ beamformer
+// One chunk of beams for (pol, 16-beam tile mt): 4 time tiles of 32 samples.
+template <int RSD, int NK>
+__device__ __forceinline__ void beam_chunk(const VoltParams &P, const uint32_t *L, const v8h wop[2 * NK],
+                                           int lane, int c, int pol, int mt, int s_chunk, int s_end) {
+  const int kg = lane >> 5;
+#pragma unroll 1
+  for (int tt = 0; tt < kChunk / 32; ++tt) {
+    const int t = 32 * tt + (lane & 31);
+    v16f acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+      // 4 antennas x both pols: 16-byte aligned since RSD and 8j + 4kg are multiples of 4
+      const uint4 d = *reinterpret_cast<const uint4 *>(L + t * RSD + 8 * j + 4 * kg);
+      const uint32_t w4[4] = {d.x, d.y, d.z, d.w};
+      v8h xop;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t pr = w4[i] >> (16 * pol);
+        xop[2 * i] = (_Float16)(int)(int8_t)(pr & 0xffu);
+        xop[2 * i + 1] = (_Float16)(int)(int8_t)((pr >> 8) & 0xffu);
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[2 * j], xop, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[2 * j + 1], xop, acc, 0, 0, 0);
+    }
+    const int s = s_chunk + t;
+    if (s < s_end) {
+      __half2 *row = reinterpret_cast<__half2 *>(P.beams) + ((long)c * 2 + pol) * P.nbeams_out * P.T + s;
+#pragma unroll
+      for (int v = 0; v < 16; v += 2) {
+        const int r = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+        const int e = 16 * mt + (r >> 1);
+        if (e < P.nbeams_out) row[(long)e * P.T] = __floats2half2_rn(acc[v], acc[v + 1]);
+      }
+    }
+  }
+}
+
+template <int NK>
+__device__ __forceinline__ void load_weights(const VoltParams &P, int c, int pol, int mt, int NMT, int lane,
+                                             v8h wop[2 * NK]) {
+  const v8h *src = P.wop + ((((long)c * 2 + pol) * NMT + mt) * NK) * 2 * 64;
+#pragma unroll
+  for (int j = 0; j < NK; ++j) {
+    wop[2 * j] = src[(j * 2 + 0) * 64 + lane];
+    wop[2 * j + 1] = src[(j * 2 + 1) * 64 + lane];
+  }
+}
+
+__device__ __forceinline__ void wg_coords(const VoltParams &P, int &c, int &split) {
+  const int bid = blockIdx.x;
+  if (P.xcd_map) {
+    const int xcd = bid & 7, k = bid >> 3;
+    c = (k / P.nsplit) * 8 + xcd;
+    split = k % P.nsplit;
+  } else {
+    c = bid / P.nsplit;
+    split = bid % P.nsplit;
+  }
+}
+
+// ================================================================ fast path
+template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT>
+__global__ void __launch_bounds__((Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, true>::NT))
+volt_fast_kernel(VoltParams P) {
+  using G = Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, true>;
+  constexpr int RSD = G::RSD, NK = G::NK, STAGE = G::STAGE, NS = 3;
+  constexpr int NI = STAGE / 256;  // dwordx4 wave loads per chunk
+  static_assert(NI < 64, "loads of one chunk must fit the vmcnt counter");
+  __shared__ __align__(16) uint32_t lds[NS * STAGE];
+  __shared__ int s_flags;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int c, split;
+  wg_coords(P, c, split);
+  const int s_begin = split * P.samples_per_wg;
+  const int s_end = min(P.T, s_begin + P.samples_per_wg);
+  if (s_begin >= s_end) return;
+  const int nchunks = (s_end - s_begin + kChunk - 1) / kChunk;
+  const bool is_loader = wave == G::NW - 1;
+  const bool is_corr = wave < G::NWC;
+  const int bw = wave - G::NWC;
+  const int off = P.window_base + (P.delays ? P.delays[0] : 0);
+  const long ring_rows = (long)(P.NP + 2) * P.NT;
+  const uint32_t *chan32 = reinterpret_cast<const uint32_t *>(P.samples) + (long)c * ring_rows * P.R;
+  const uint32_t *cscale = reinterpret_cast<const uint32_t *>(P.scales) + (long)c * (P.NP + 2) * P.R;
+  bool bad = false;
+
+  // any non-unit scale in this workgroup's slot range?
+  if (tid == 0) s_flags = 0;
+  __syncthreads();
+  {
+    const int slot0 = (s_begin + off) / P.NT, slot1 = (s_end - 1 + off) / P.NT;
+    const int n = (slot1 - slot0 + 1) * P.R;
+    int any = 0;
+    for (int i = tid; i < n; i += G::NT) any |= cscale[(long)slot0 * P.R + i] != 0x00010001u;
+    if (any) atomicOr(&s_flags, 1);
+  }
+  __syncthreads();
+  const bool scaled = s_flags != 0;
+
+  auto issue = [&](int ch) {
+    const int s0 = s_begin + ch * kChunk;
+    const int valid = min(kChunk, s_end - s0);
+    uint32_t *dst = lds + (ch % NS) * STAGE;
+#pragma unroll 2
+    for (int j = 0; j < NI; ++j) {
+      const int pos = j * 256 + lane * 4;
+      const int t = pos / RSD, a = pos - t * RSD;
+      const uint32_t *src = chan32;
+      if (t < valid && a < P.R) src = chan32 + (long)(s0 + t + off) * P.R + a;
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(src),
+                                       (__attribute__((address_space(3))) void *)(dst + j * 256), 16, 0, 0);
+    }
+  };
+
+  v4i acc[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) acc[i] = v4i{0, 0, 0, 0};
+  int gA = 0, gB = 0;
+  if (DO_CORR && is_corr) pair_groups<NG>(wave, gA, gB);
+  const int bpol = bw & 1, bmt = bw >> 1;
+  v8h wop[DO_BEAM ? 2 * NK : 1];
+  if constexpr (DO_BEAM) {
+    if (!is_corr && !is_loader) load_weights<NK>(P, c, bpol, bmt, NMT, lane, wop);
+  }
+
+  if (is_loader) {
+    issue(0);
+    if (nchunks > 1) issue(1);
+  }
+  for (int ch = 0; ch < nchunks; ++ch) {
+    if (is_loader) {
+      if (ch + 1 < nchunks)
+        wait_vmcnt<NI>();
+      else
+        wait_vmcnt<0>();
+    }
+    uint32_t *L = lds + (ch % NS) * STAGE;
+    const int s0 = s_begin + ch * kChunk;
+    const int valid = min(kChunk, s_end - s0);
+    if (scaled || (DO_CORR && valid < kChunk)) {
+      __syncthreads();  // raw chunk visible to all waves
+      for (int i = tid; i < kChunk * P.R; i += G::NT) {
+        const int t = i / P.R, a = i - t * P.R;
+        uint32_t &w = L[t * RSD + a];
+        if (t >= valid) {
+          w = 0u;
+        } else if (scaled) {
+          int slot, tt;
+          slot_of(P, s0 + t + off, slot, tt);
+          w = apply_scale(w, cscale[(long)slot * P.R + a], bad);
+        }
+      }
+    }
+    __syncthreads();
+    if (is_loader && ch + 2 < nchunks) issue(ch + 2);
+    if constexpr (DO_CORR) {
+      if (is_corr) corr_chunk<RSD>(L, gA, gB, lane, acc);
+    }
+    if constexpr (DO_BEAM) {
+      if (!is_corr && !is_loader) beam_chunk<RSD, NK>(P, L, wop, lane, c, bpol, bmt, s0, s_end);
+    }
+  }
+  if (bad) atomicOr(P.err, kErrRange);
+  if constexpr (DO_CORR) {
+    if (is_corr) corr_store(P, acc, gA, gB, lane, c, split);
+  }
+}
+
+// ============================================================= general path
 // Per-thread view of the ring block: a loader thread owns one antenna and
-// fetches its dword (both pols, re/im) for a strided set of time rows.  The
-// FPGA, delay, permutation and channel base are resolved once per thread.
-// fetch() only issues loads (sample and scale dwords, no data-dependent
-// branches, so all of a chunk's loads are in flight together); apply() scales
-// them when the chunk is written to LDS.
+// fetches its dword (both pols, re/im) for a strided set of time rows; FPGA,
+// delay, permutation and channel base are resolved once per thread.
 template <bool PERM>
 struct ThreadLoader {
-  const uint32_t *chan32;  // channel base of the samples, dwords
-  const uint16_t *chan16;  // same, 16-bit units (permutation path)
-  const int16_t *cscale;   // channel base of the scales
-  int a;                   // output antenna
-  int f, n, off;           // identity path: FPGA, receiver in packet, sample offset
-  int src[2], sp[2], pf[2], pn[2], poff[2];  // permutation path, per output pol
+  const uint32_t *chan32;
+  const uint16_t *chan16;
+  const int16_t *cscale;
+  int a, f, n, off;
+  int src[2], sp[2], pf[2], pn[2], poff[2];
 
   __device__ void init(const VoltParams &P, int c, int a_) {
     a = a_;
@@ -90,18 +391,7 @@ struct ThreadLoader {
     }
   }
 
-  __device__ __forceinline__ void slot_of(const VoltParams &P, int g, int &slot, int &tt) const {
-    if (P.nt_log2 >= 0) {
-      slot = g >> P.nt_log2;
-      tt = g & (P.NT - 1);
-    } else {
-      slot = g / P.NT;
-      tt = g - slot * P.NT;
-    }
-  }
-
-  // Issues the loads of window sample s: x = sample dword, sc = scale dword
-  // (identity path) or x = two 16-bit samples, sc = two int16 scales (permutation).
+  // Issues the loads of window sample s (no data-dependent branches).
   __device__ __forceinline__ void fetch(const VoltParams &P, int s, uint32_t &x, uint32_t &sc) const {
     if (!PERM) {
       int slot, tt;
@@ -124,132 +414,55 @@ struct ThreadLoader {
       sc |= (src[po] < 0 ? 0u : s16) << (16 * po);
     }
   }
-
-  __device__ __forceinline__ static uint32_t apply(uint32_t x, uint32_t sc, bool &bad) {
-    if (sc == 0x00010001u) return x;
-    const int s0 = (int)(int16_t)(sc & 0xffffu), s1 = (int)(int16_t)(sc >> 16);
-    return scale_pair(x & 0xffffu, s0, bad) | (scale_pair(x >> 16, s1, bad) << 16);
-  }
 };
 
-template <int NG, bool DO_CORR, bool DO_BEAM, int CH>
-struct VoltTraits {
-  static constexpr int NA = 16 * NG;                       // padded antennas
-  static constexpr int RSD = NA + 1;                       // LDS row stride in dwords (odd)
-  static constexpr int NPAIRS = NG * (NG + 1) / 2;
-  static constexpr int NWC = DO_CORR ? NPAIRS : 0;         // correlator waves, one per pair
-  static constexpr int NWB = DO_BEAM ? (DO_CORR ? 2 : 4) : 0;  // beam waves: pol x time-tile parity
-  static constexpr int NW = NWC + NWB;
-  static constexpr int NT = 64 * NW;
-};
-
-// Loads component operands of antenna group G for this lane: op[v] = 16 bytes
-// of component v (2*pol + re/im) of antenna 16G + (lane & 15) at times
-// t0 + 16*(lane >> 4) + 0..15 of the chunk.
-template <int RSD>
-__device__ __forceinline__ void load_group(const uint32_t *lds, int G, int t0, int lane, v4i op[4]) {
-  const int a = 16 * G + (lane & 15);
-  const int tb = t0 + 16 * (lane >> 4);
-  uint32_t x[16];
-#pragma unroll
-  for (int e = 0; e < 16; ++e) x[e] = lds[(tb + e) * RSD + a];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    uint32_t y0, y1, y2, y3;
-    transpose4x4(x[4 * j], x[4 * j + 1], x[4 * j + 2], x[4 * j + 3], y0, y1, y2, y3);
-    op[0][j] = (int)y0;
-    op[1][j] = (int)y1;
-    op[2][j] = (int)y2;
-    op[3][j] = (int)y3;
-  }
-}
-
-__device__ __forceinline__ v4i mfma_i8(v4i a, v4i b, v4i c) {
-  return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
-}
-
-// Pair index -> antenna groups (A, B), A <= B, row-major over the upper triangle.
-template <int NG>
-__device__ __forceinline__ void pair_groups(int pi, int &A, int &B) {
-  A = 0;
-  int rem = pi;
-  while (rem >= NG - A) {
-    rem -= NG - A;
-    ++A;
-  }
-  B = A + rem;
-}
-
-template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT, int CH, bool PERM>
-__global__ void __launch_bounds__((VoltTraits<NG, DO_CORR, DO_BEAM, CH>::NT))
-volt_kernel(VoltParams P) {
-  using TR = VoltTraits<NG, DO_CORR, DO_BEAM, CH>;
-  constexpr int RSD = TR::RSD;
-  constexpr int NK = 2 * NG - KODD;  // beam K steps of 8 antennas
-  __shared__ uint32_t lds[2][CH * RSD];
+template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT, bool PERM>
+__global__ void __launch_bounds__((Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, false>::NT))
+volt_general_kernel(VoltParams P) {
+  using G = Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, false>;
+  constexpr int RSD = G::RSD, NK = G::NK, STAGE = G::STAGE;
+  __shared__ __align__(16) uint32_t lds[2 * STAGE];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int c, split;
-  {
-    const int bid = blockIdx.x;
-    if (P.xcd_map) {
-      const int xcd = bid & 7, k = bid >> 3;
-      c = (k / P.nsplit) * 8 + xcd;
-      split = k % P.nsplit;
-    } else {
-      c = bid / P.nsplit;
-      split = bid % P.nsplit;
-    }
-  }
+  wg_coords(P, c, split);
   const int s_begin = split * P.samples_per_wg;
   const int s_end = min(P.T, s_begin + P.samples_per_wg);
   if (s_begin >= s_end) return;
-  const int nchunks = (s_end - s_begin + CH - 1) / CH;
+  const int nchunks = (s_end - s_begin + kChunk - 1) / kChunk;
   bool bad = false;
-  const bool is_corr = wave < TR::NWC;
-  const int bw = wave - TR::NWC;  // beam wave index 0..3 when >= 0
+  const bool is_corr = wave < G::NWC;
+  const int bw = wave - G::NWC;
 
-  // correlator accumulators of this wave's pair: [p][q][re, x, y]
   v4i acc[12];
 #pragma unroll
   for (int i = 0; i < 12; ++i) acc[i] = v4i{0, 0, 0, 0};
   int gA = 0, gB = 0;
   if (DO_CORR && is_corr) pair_groups<NG>(wave, gA, gB);
-
-  // beam weight operands of this wave's pol, held for the whole workgroup
-  const int bpol = bw & 1;
-  v8h wop[DO_BEAM ? NMT : 1][DO_BEAM ? 2 * NK : 1];
+  const int bpol = bw & 1, bmt = bw >> 1;
+  v8h wop[DO_BEAM ? 2 * NK : 1];
   if constexpr (DO_BEAM) {
-    if (!is_corr) {
-      const v8h *wsrc = P.wop + (((long)c * 2 + bpol) * NMT) * NK * 2 * 64;
-#pragma unroll
-      for (int mt = 0; mt < NMT; ++mt)
-#pragma unroll
-        for (int j = 0; j < NK; ++j) {
-          wop[mt][2 * j] = wsrc[((mt * NK + j) * 2 + 0) * 64 + lane];
-          wop[mt][2 * j + 1] = wsrc[((mt * NK + j) * 2 + 1) * 64 + lane];
-        }
-    }
+    if (!is_corr) load_weights<NK>(P, c, bpol, bmt, NMT, lane, wop);
   }
 
   // loader geometry: thread -> (antenna a = tid % R, row tr = tid / R)
-  const int LT = TR::NT / P.R;  // time rows per pass (R <= 64 <= NT)
+  const int LT = G::NT / P.R;
   const bool loader = tid < LT * P.R;
   ThreadLoader<PERM> ld;
   const int tr = tid / P.R;
   ld.init(P, c, tid - tr * P.R);
-  constexpr int LTMIN = TR::NT / TR::NA;  // LT >= LTMIN since R <= NA
-  constexpr int MAXLOAD = (CH + LTMIN - 1) / LTMIN;
-  const int nload = (CH + LT - 1) / LT;
+  constexpr int LTMIN = G::NT / G::NA;
+  constexpr int MAXLOAD = (kChunk + LTMIN - 1) / LTMIN;
+  const int nload = (kChunk + LT - 1) / LT;
   uint32_t pre[MAXLOAD], psc[MAXLOAD];
   auto fetch = [&](int ch) {
-    const int s0 = s_begin + ch * CH + tr;
+    const int s0 = s_begin + ch * kChunk + tr;
 #pragma unroll
     for (int k = 0; k < MAXLOAD; ++k) {
       if (k < nload) {
         const int t = tr + k * LT;
         const int s = s0 + k * LT;
-        if (loader && t < CH && s < s_end) {
+        if (loader && t < kChunk && s < s_end) {
           ld.fetch(P, s, pre[k], psc[k]);
         } else {
           pre[k] = 0u;
@@ -263,148 +476,52 @@ volt_kernel(VoltParams P) {
     for (int k = 0; k < MAXLOAD; ++k) {
       if (k < nload) {
         const int t = tr + k * LT;
-        if (loader && t < CH) lds[buf][t * RSD + ld.a] = ThreadLoader<PERM>::apply(pre[k], psc[k], bad);
+        if (loader && t < kChunk) lds[buf * STAGE + t * RSD + ld.a] = apply_scale(pre[k], psc[k], bad);
       }
     }
   };
-  // zero the padding antennas of both buffers once (the loader never writes them)
-  for (int i = tid; i < 2 * CH * RSD; i += TR::NT) {
-    const int a = (i % (CH * RSD)) % RSD;
-    if (a >= P.R) (&lds[0][0])[i] = 0u;
-  }
+  // padding antennas are never loaded: clear them once in both buffers
+  for (int i = tid; i < 2 * STAGE; i += G::NT)
+    if ((i % RSD) >= P.R) lds[i] = 0u;
 
   fetch(0);
   stash(0);
   __syncthreads();
-
   for (int ch = 0; ch < nchunks; ++ch) {
     const int buf = ch & 1;
     if (ch + 1 < nchunks) fetch(ch + 1);
-    const uint32_t *L = lds[buf];
-
+    const uint32_t *L = lds + buf * STAGE;
+    const int s0 = s_begin + ch * kChunk;
     if constexpr (DO_CORR) {
-      if (is_corr) {
-#pragma unroll 1
-        for (int t0 = 0; t0 < CH; t0 += 64) {
-          v4i opA[4], opB[4];
-          load_group<RSD>(L, gA, t0, lane, opA);
-          if (gB != gA) {
-            load_group<RSD>(L, gB, t0, lane, opB);
-          } else {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) opB[v] = opA[v];
-          }
-#pragma unroll
-          for (int p = 0; p < 2; ++p)
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-              v4i *a3 = &acc[(p * 2 + q) * 3];
-              a3[0] = mfma_i8(opA[2 * p], opB[2 * q], a3[0]);
-              a3[0] = mfma_i8(opA[2 * p + 1], opB[2 * q + 1], a3[0]);
-              a3[1] = mfma_i8(opA[2 * p + 1], opB[2 * q], a3[1]);
-              a3[2] = mfma_i8(opA[2 * p], opB[2 * q + 1], a3[2]);
-            }
-        }
-      }
+      if (is_corr) corr_chunk<RSD>(L, gA, gB, lane, acc);
     }
-
     if constexpr (DO_BEAM) {
-      if (!is_corr) {
-        const int kg = lane >> 5;
-#pragma unroll 1
-        for (int tt = bw >> 1; tt < CH / 32; tt += TR::NWB / 2) {
-          // time tile of 32 samples
-          const int t = 32 * tt + (lane & 31);
-          v16f bacc[NMT];
-#pragma unroll
-          for (int mt = 0; mt < NMT; ++mt)
-#pragma unroll
-            for (int i = 0; i < 16; ++i) bacc[mt][i] = 0.0f;
-#pragma unroll
-          for (int j = 0; j < NK; ++j) {
-            {
-              const int a0 = 8 * j + 4 * kg;
-              v8h xop;
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                const uint32_t pr = L[t * RSD + a0 + i] >> (16 * bpol);
-                xop[2 * i] = (_Float16)(int)(int8_t)(pr & 0xffu);
-                xop[2 * i + 1] = (_Float16)(int)(int8_t)((pr >> 8) & 0xffu);
-              }
-#pragma unroll
-              for (int mt = 0; mt < NMT; ++mt) {
-                bacc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[mt][2 * j], xop, bacc[mt], 0, 0, 0);
-                bacc[mt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[mt][2 * j + 1], xop, bacc[mt], 0, 0, 0);
-              }
-            }
-          }
-          const int s = s_begin + ch * CH + t;
-          if (s < s_end) {
-#pragma unroll
-            for (int mt = 0; mt < NMT; ++mt)
-#pragma unroll
-              for (int v = 0; v < 16; v += 2) {
-                const int row = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                const int e = 16 * mt + (row >> 1);
-                if (e < P.nbeams_out) {
-                  const __half2 h = __floats2half2_rn(bacc[mt][v], bacc[mt][v + 1]);
-                  reinterpret_cast<__half2 *>(P.beams)
-                      [(((long)c * 2 + bpol) * P.nbeams_out + e) * P.T + s] = h;
-                }
-              }
-          }
-        }
-      }
+      if (!is_corr) beam_chunk<RSD, NK>(P, L, wop, lane, c, bpol, bmt, s0, s_end);
     }
-
     if (ch + 1 < nchunks) stash(buf ^ 1);
     __syncthreads();
   }
-
   if (bad) atomicOr(P.err, kErrRange);
-
   if constexpr (DO_CORR) {
-    if (is_corr) {
-      const long nbl = (long)P.R * (P.R + 1) / 2;
-      const int b = 16 * gB + (lane & 15);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int a = 16 * gA + 4 * (lane >> 4) + v;
-        if (a > b || b >= P.R) continue;
-        const long bl = (long)b * (b + 1) / 2 + a;
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const v4i *a3 = &acc[(p * 2 + q) * 3];
-            const int re = a3[0][v];
-            const int im = a3[1][v] - a3[2][v];
-            const long o = (((long)c * nbl + bl) * 4 + p * 2 + q) * 2;
-            if (P.vis32) {
-              atomicAdd(&P.vis32[o], re);
-              atomicAdd(&P.vis32[o + 1], im);
-            }
-            if (P.vis64) {
-              atomicAdd(reinterpret_cast<unsigned long long *>(&P.vis64[o]),
-                        (unsigned long long)(long long)re);
-              atomicAdd(reinterpret_cast<unsigned long long *>(&P.vis64[o + 1]),
-                        (unsigned long long)(long long)im);
-            }
-          }
-      }
-    }
+    if (is_corr) corr_store(P, acc, gA, gB, lane, c, split);
   }
 }
 
+// ================================================================ launchers
 template <int NG, int KODD, bool C_, bool B_, int NMT>
 static hipError_t launch_t(const VoltParams &p, hipStream_t s) {
-  constexpr int CH = 128;
-  using TR = VoltTraits<NG, C_, B_, CH>;
-  dim3 grid(p.C * p.nsplit), block(TR::NT);
-  if (p.recv_perm)
-    hipLaunchKernelGGL((volt_kernel<NG, KODD, C_, B_, NMT, CH, true>), grid, block, 0, s, p);
-  else
-    hipLaunchKernelGGL((volt_kernel<NG, KODD, C_, B_, NMT, CH, false>), grid, block, 0, s, p);
+  dim3 grid(p.C * p.nsplit);
+  const bool fast = p.F == 1 && (p.R % 4) == 0 && p.recv_perm == nullptr;
+  if (fast) {
+    using G = Geo<NG, KODD, C_, B_, NMT, true>;
+    hipLaunchKernelGGL((volt_fast_kernel<NG, KODD, C_, B_, NMT>), grid, dim3(G::NT), 0, s, p);
+  } else {
+    using G = Geo<NG, KODD, C_, B_, NMT, false>;
+    if (p.recv_perm)
+      hipLaunchKernelGGL((volt_general_kernel<NG, KODD, C_, B_, NMT, true>), grid, dim3(G::NT), 0, s, p);
+    else
+      hipLaunchKernelGGL((volt_general_kernel<NG, KODD, C_, B_, NMT, false>), grid, dim3(G::NT), 0, s, p);
+  }
   return hipGetLastError();
 }
 
@@ -428,9 +545,10 @@ hipError_t launch_volt(const VoltParams &p, bool do_corr, bool do_beam, hipStrea
   const int nk = (p.R + 7) / 8;
   const int kodd = 2 * ng - nk;  // 0 or 1
   if (do_beam && p.NK != nk) return hipErrorInvalidValue;
-#define LAM_NG(G)                                                   \
-  case G:                                                           \
-    return kodd ? launch_ng<G, 1>(p, do_corr, do_beam, s)           \
+  if (p.samples_per_wg % kChunk != 0) return hipErrorInvalidValue;
+#define LAM_NG(G)                                         \
+  case G:                                                 \
+    return kodd ? launch_ng<G, 1>(p, do_corr, do_beam, s) \
                 : launch_ng<G, 0>(p, do_corr, do_beam, s);
   switch (ng) {
     LAM_NG(1)
@@ -441,6 +559,31 @@ hipError_t launch_volt(const VoltParams &p, bool do_corr, bool do_beam, hipStrea
       return hipErrorInvalidValue;
   }
 #undef LAM_NG
+}
+
+// Sum of the per-split partial visibilities into the int64 accumulator.
+// (atomic when several buffers' blocks may be in flight on different streams)
+template <bool ATOMIC>
+__global__ void accumulate_vis_kernel(const int *part, int nsplit, long long count, long long *acc) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x) {
+    long long s = 0;
+    for (int k = 0; k < nsplit; ++k) s += part[(long long)k * count + i];
+    if (ATOMIC)
+      atomicAdd(reinterpret_cast<unsigned long long *>(acc + i), (unsigned long long)s);
+    else
+      acc[i] += s;
+  }
+}
+
+hipError_t launch_accumulate_vis(const int *vis_part, int nsplit, long long count, long long *acc,
+                                 bool atomic, hipStream_t s) {
+  const int blocks = (int)std::min<long long>((count + 255) / 256, 8192);
+  if (atomic)
+    hipLaunchKernelGGL(accumulate_vis_kernel<true>, dim3(blocks), dim3(256), 0, s, vis_part, nsplit, count, acc);
+  else
+    hipLaunchKernelGGL(accumulate_vis_kernel<false>, dim3(blocks), dim3(256), 0, s, vis_part, nsplit, count, acc);
+  return hipGetLastError();
 }
 
 }  // namespace lam
