@@ -44,15 +44,74 @@ __device__ float block_sum(float v, float *red) {
   return t;
 }
 
+// 2x2 block of a Hermitian update: Y = J_P^H X J_Q with X = A[{p1,q1}][{p2,q2}].
+__device__ __forceinline__ void rotate_block(const Rot &RP, const Rot &RQ, float2 &x00, float2 &x01,
+                                             float2 &x10, float2 &x11) {
+  {  // columns: [x_p, x_q] J_Q
+    const float2 eb = make_float2(RQ.er, -RQ.ei);
+    const float2 y0 = cmul(eb, x01), y1 = cmul(eb, x11);
+    const float2 n00 = make_float2(RQ.c * x00.x - RQ.s * y0.x, RQ.c * x00.y - RQ.s * y0.y);
+    const float2 n01 = make_float2(RQ.s * x00.x + RQ.c * y0.x, RQ.s * x00.y + RQ.c * y0.y);
+    const float2 n10 = make_float2(RQ.c * x10.x - RQ.s * y1.x, RQ.c * x10.y - RQ.s * y1.y);
+    const float2 n11 = make_float2(RQ.s * x10.x + RQ.c * y1.x, RQ.s * x10.y + RQ.c * y1.y);
+    x00 = n00; x01 = n01; x10 = n10; x11 = n11;
+  }
+  {  // rows: J_P^H [x_p; x_q]
+    const float2 e = make_float2(RP.er, RP.ei);
+    const float2 y0 = cmul(e, x10), y1 = cmul(e, x11);
+    const float2 n00 = make_float2(RP.c * x00.x - RP.s * y0.x, RP.c * x00.y - RP.s * y0.y);
+    const float2 n01 = make_float2(RP.c * x01.x - RP.s * y1.x, RP.c * x01.y - RP.s * y1.y);
+    const float2 n10 = make_float2(RP.s * x00.x + RP.c * y0.x, RP.s * x00.y + RP.c * y0.y);
+    const float2 n11 = make_float2(RP.s * x01.x + RP.c * y1.x, RP.s * x01.y + RP.c * y1.y);
+    x00 = n00; x01 = n01; x10 = n10; x11 = n11;
+  }
+}
+
+// FP64 reciprocal square root / reciprocal: hardware estimate plus one Newton step.
+__device__ __forceinline__ double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+__device__ __forceinline__ double rcp64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  return y * (2.0 - x * y);
+}
+
+// Rotation that annihilates a_pq (angles in FP64):
+// J = [[c, s], [-s e^{-i phi}, c e^{-i phi}]], a_pq = |b| e^{i phi},
+// tau = (a_qq - a_pp) / (2|b|), t = sgn(tau) / (|tau| + sqrt(1 + tau^2)).
+__device__ __forceinline__ Rot make_rotation(float2 b, float app, float aqq) {
+  Rot R{1.0f, 0.0f, 1.0f, 0.0f};
+  const double br = b.x, bi = b.y;
+  const double b2 = br * br + bi * bi;
+  if (b2 > 0.0 && b2 > 1e-60 * ((double)app * app + (double)aqq * aqq)) {
+    const double inv_b = rsq64(b2);
+    const double tau = ((double)aqq - (double)app) * 0.5 * inv_b;
+    const double at = fabs(tau);
+    const double u = 1.0 + tau * tau;
+    const double t = rcp64(at + u * rsq64(u));
+    const double cc = rsq64(1.0 + t * t);
+    R.c = (float)cc;
+    R.s = (float)((tau >= 0.0 ? t : -t) * cc);
+    R.er = (float)(br * inv_b);
+    R.ei = (float)(bi * inv_b);
+  }
+  return R;
+}
+
 // Cyclic Jacobi on the Hermitian matrix A (column-major, leading dim n) in LDS;
-// V receives the eigenvectors (columns).  Returns the number of sweeps.
+// V receives the eigenvectors (columns).  Each round rotates n/2 disjoint pairs
+// (round-robin order); the update visits only blocks kp <= kq and writes the
+// mirrored block as its conjugate transpose.  Returns the number of sweeps.
 __device__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol,
-                          int2 *rpq, Rot *rot, float *red) {
+                          int2 *rpq, Rot *rot, float *red, bool init_v = true) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int ne = n + (n & 1), h = ne / 2;
+  const int nblk = h * (h + 1) / 2;
+  const float inv_n = 1.0f / (float)n;
   for (int i = tid; i < n * n; i += nthr) {
-    const int r = i % n, c = i / n;
-    V[i] = make_float2(r == c ? 1.0f : 0.0f, 0.0f);
+    const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
+    if (init_v) V[i] = make_float2(r == c ? 1.0f : 0.0f, 0.0f);
     if (r == c) A[i].y = 0.0f;
   }
   __syncthreads();
@@ -60,10 +119,11 @@ __device__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol
   for (; sweep < max_sweeps; ++sweep) {
     float off = 0.0f, tot = 0.0f;
     for (int i = tid; i < n * n; i += nthr) {
+      const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
       const float2 a = A[i];
       const float m = a.x * a.x + a.y * a.y;
       tot += m;
-      if (i % n != i / n) off += m;
+      if (r != c) off += m;
     }
     off = block_sum(off, red);
     tot = block_sum(tot, red);
@@ -77,69 +137,58 @@ __device__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol
           q = t;
         }
         Rot R{1.0f, 0.0f, 1.0f, 0.0f};
-        if (q < n) {
-          const float2 b = A[q * n + p];
-          const double br = b.x, bi = b.y;
-          const double babs = sqrt(br * br + bi * bi);
-          const double a = A[p * n + p].x, d = A[q * n + q].x;
-          if (babs > 1e-30 * (fabs(a) + fabs(d)) && babs > 0.0) {
-            const double tau = (d - a) / (2.0 * babs);
-            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-            const double cc = 1.0 / sqrt(1.0 + t * t);
-            R.c = (float)cc;
-            R.s = (float)(t * cc);
-            R.er = (float)(br / babs);
-            R.ei = (float)(bi / babs);
-          }
-        }
+        if (q < n) R = make_rotation(A[q * n + p], A[p * n + p].x, A[q * n + q].x);
         rpq[tid] = make_int2(p, q);
         rot[tid] = R;
       }
       __syncthreads();
-      // A <- J^H A J on 2x2 blocks (row pair kp, column pair kq)
-      for (int idx = tid; idx < h * h; idx += nthr) {
-        const int kp = idx % h, kq = idx / h;
+      // A <- J^H A J on blocks (kp <= kq); the (kq, kp) block is the conjugate transpose
+      for (int idx = tid; idx < nblk; idx += nthr) {
+        // idx -> (kp, kq) over the upper triangle, row-major: kq >= kp
+        int kp = (int)((2.0f * h + 1.0f - sqrtf((2.0f * h + 1.0f) * (2.0f * h + 1.0f) - 8.0f * idx)) * 0.5f);
+        while (kp > 0 && kp * (2 * h - kp + 1) / 2 > idx) --kp;
+        while ((kp + 1) * (2 * h - kp) / 2 <= idx) ++kp;
+        const int kq = kp + idx - kp * (2 * h - kp + 1) / 2;
         const int2 P = rpq[kp], Q = rpq[kq];
         const Rot RP = rot[kp], RQ = rot[kq];
-        const bool pv = P.y < n, qv = Q.y < n;  // second index of a pair may be the dummy
+        const bool pv = P.y < n, qv = Q.y < n;  // the second index of a pair may be the dummy
         float2 x00 = A[Q.x * n + P.x];
         float2 x01 = qv ? A[Q.y * n + P.x] : make_float2(0.f, 0.f);
         float2 x10 = pv ? A[Q.x * n + P.y] : make_float2(0.f, 0.f);
         float2 x11 = (pv && qv) ? A[Q.y * n + P.y] : make_float2(0.f, 0.f);
-        // columns: [x_p, x_q] J_Q
-        {
-          const float2 eb = make_float2(RQ.er, -RQ.ei);
-          const float2 y0 = cmul(eb, x01), y1 = cmul(eb, x11);
-          const float2 n00 = make_float2(RQ.c * x00.x - RQ.s * y0.x, RQ.c * x00.y - RQ.s * y0.y);
-          const float2 n01 = make_float2(RQ.s * x00.x + RQ.c * y0.x, RQ.s * x00.y + RQ.c * y0.y);
-          const float2 n10 = make_float2(RQ.c * x10.x - RQ.s * y1.x, RQ.c * x10.y - RQ.s * y1.y);
-          const float2 n11 = make_float2(RQ.s * x10.x + RQ.c * y1.x, RQ.s * x10.y + RQ.c * y1.y);
-          x00 = n00; x01 = n01; x10 = n10; x11 = n11;
-        }
-        // rows: J_P^H [x_p; x_q]
-        {
-          const float2 e = make_float2(RP.er, RP.ei);
-          const float2 y0 = cmul(e, x10), y1 = cmul(e, x11);
-          const float2 n00 = make_float2(RP.c * x00.x - RP.s * y0.x, RP.c * x00.y - RP.s * y0.y);
-          const float2 n01 = make_float2(RP.c * x01.x - RP.s * y1.x, RP.c * x01.y - RP.s * y1.y);
-          const float2 n10 = make_float2(RP.s * x00.x + RP.c * y0.x, RP.s * x00.y + RP.c * y0.y);
-          const float2 n11 = make_float2(RP.s * x01.x + RP.c * y1.x, RP.s * x01.y + RP.c * y1.y);
-          x00 = n00; x01 = n01; x10 = n10; x11 = n11;
-        }
+        rotate_block(RP, RQ, x00, x01, x10, x11);
         if (kp == kq) {  // the annihilated pair: exact zeros, real diagonal
           x01 = make_float2(0.f, 0.f);
           x10 = make_float2(0.f, 0.f);
           x00.y = 0.f;
           x11.y = 0.f;
+          A[P.x * n + P.x] = x00;
+          if (pv) {
+            A[P.y * n + P.x] = x01;
+            A[P.x * n + P.y] = x10;
+            A[P.y * n + P.y] = x11;
+          }
+        } else {
+          A[Q.x * n + P.x] = x00;
+          A[P.x * n + Q.x] = make_float2(x00.x, -x00.y);
+          if (qv) {
+            A[Q.y * n + P.x] = x01;
+            A[P.x * n + Q.y] = make_float2(x01.x, -x01.y);
+          }
+          if (pv) {
+            A[Q.x * n + P.y] = x10;
+            A[P.y * n + Q.x] = make_float2(x10.x, -x10.y);
+          }
+          if (pv && qv) {
+            A[Q.y * n + P.y] = x11;
+            A[P.y * n + Q.y] = make_float2(x11.x, -x11.y);
+          }
         }
-        A[Q.x * n + P.x] = x00;
-        if (qv) A[Q.y * n + P.x] = x01;
-        if (pv) A[Q.x * n + P.y] = x10;
-        if (pv && qv) A[Q.y * n + P.y] = x11;
       }
       // V <- V J
+      const float inv_nn = inv_n;
       for (int idx = tid; idx < n * h; idx += nthr) {
-        const int row = idx % n, k = idx / n;
+        const int k = (int)(((float)idx + 0.5f) * inv_nn), row = idx - k * n;
         const int2 Q = rpq[k];
         if (Q.y >= n) continue;
         const Rot RQ = rot[k];
