@@ -88,7 +88,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graphs", type=int, default=0,
                     help="time with hipGraph replay instead of per-kernel events")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_probe_mib:
+        nbytes = args.copy_probe_mib << 20
+        src = lam.DeviceArray((nbytes,), np.uint8)
+        dst = lam.DeviceArray((nbytes,), np.uint8)
+        for _ in range(3):
+            lam.memcpy(dst, src, nbytes)
+        print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}), flush=True)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "

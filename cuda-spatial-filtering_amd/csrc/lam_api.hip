@@ -6,11 +6,14 @@
 // library glue: one fused voltage kernel replaces ~15 launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -34,10 +37,37 @@ struct Buf {
   int *d_vis_part = nullptr;    // this block's partial visibilities [split][C][NBL][8]
   v8h *d_wop2 = nullptr;        // adaptive: this block's 2B-beam weight operands
   float *d_evals = nullptr, *d_evecs = nullptr;
-  int *d_counts = nullptr, *d_sweeps = nullptr;
+  int *d_counts = nullptr, *d_sweeps = nullptr, *d_fallback = nullptr, *d_nsig = nullptr;
   hipGraphExec_t graph = nullptr;
   const void *graph_samples = nullptr, *graph_scales = nullptr;
   hipEvent_t done = nullptr;
+};
+
+// Learned warm start: MLP parameters, Adam state, per-block buffers and the
+// captured training set.
+struct Learn {
+  bool ready = false;
+  int KW = 4, n = 0, Dx = 0, Dp = 0, H = 256, Dy = 0, Dyp = 0;
+  long P = 0, off_b1 = 0, off_W2 = 0, off_b2 = 0;
+  float *d_prm = nullptr, *d_grad = nullptr, *d_m1 = nullptr, *d_m2 = nullptr;
+  uint16_t *d_W1bf = nullptr, *d_W2bf = nullptr;
+  int adam_t = 0;
+  // per block (M = C * 2 rows)
+  uint16_t *d_X = nullptr, *d_H = nullptr;
+  float *d_Y = nullptr;
+  float2 *d_basis = nullptr;
+  // captured set
+  int cap_max = 0, cap_count = 0;
+  uint16_t *d_dsX = nullptr;
+  float2 *d_dsU = nullptr;
+  int *d_dsK = nullptr;
+  // training batch scratch
+  int Mb = 0;
+  uint16_t *d_bX = nullptr, *d_bH = nullptr;
+  float2 *d_bU = nullptr;
+  int *d_bK = nullptr, *d_idx = nullptr;
+  long idx_cap = 0;
+  float *d_bZ = nullptr, *d_bY = nullptr, *d_bdY = nullptr, *d_bdH = nullptr, *d_loss = nullptr;
 };
 
 }  // namespace
@@ -68,6 +98,12 @@ struct lam_ctx {
   bool use_graphs = true, graphs_dirty = true, profiling = false;
   int samples_per_wg = 0;
   std::vector<ProfEvent> prof;
+  // eigen-stage method
+  int eig_method = LAM_EIG_PLAIN, refine_sweeps = 2, power = 0;
+  float res_tol = 0.1f;
+  int eig_plain_threads = 256, eig_fallback_threads = 1024;
+  Learn lr;
+  int last_buf = -1;
   double prof_ms[LAM_K_COUNT] = {0};
   int prof_n[LAM_K_COUNT] = {0};
   std::string err;
@@ -153,6 +189,18 @@ struct Prof {
   }
 };
 
+// Skip connection of the output layer: the previous block's vectors stored in
+// the features (scaled by sqrt(n)) are added back unscaled.
+Skip skip_of(const Learn &L, const uint16_t *X) {
+  Skip k;
+  k.X = X;
+  k.ld = L.Dp;
+  k.off = L.n * (L.n - 1);
+  k.len = L.Dy;
+  k.scale = 1.0f / std::sqrt((float)L.n);
+  return k;
+}
+
 // The per-block kernel sequence of one buffer (captured into a graph when enabled).
 int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
   hipStream_t s = b.stream;
@@ -185,6 +233,30 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
     Prof pr(ctx, s, LAM_K_CORR);
     HIPC(ctx, launch_volt(pc, true, false, s));
   }
+  const int M = ctx->C * 2;
+  Learn &L = ctx->lr;
+  const bool learned = ctx->eig_method == LAM_EIG_LEARNED && L.ready;
+  const bool previous = ctx->eig_method == LAM_EIG_PREVIOUS && L.ready;
+  const bool capture = L.ready && ctx->eig_method == LAM_EIG_PLAIN && L.cap_count < L.cap_max;
+  if (learned || capture) {
+    FeatParams f{};
+    f.vis_part = b.d_vis_part;
+    f.nsplit = p.nsplit;
+    f.C = ctx->C;
+    f.n = ctx->R;
+    f.KW = L.KW;
+    f.Dp = L.Dp;
+    f.basis = L.d_basis;
+    f.X = capture ? L.d_dsX + (size_t)L.cap_count * M * L.Dp : L.d_X;
+    Prof pr(ctx, s, LAM_K_FEAT);
+    HIPC(ctx, launch_features(f, s));
+  }
+  if (learned) {
+    Prof pr(ctx, s, LAM_K_MLP);
+    HIPC(ctx, launch_dense_fwd(L.d_X, M, L.Dp, L.d_W1bf, L.d_prm + L.off_b1, L.H, nullptr, L.d_H, Skip{}, s));
+    HIPC(ctx, launch_dense_fwd(L.d_H, M, L.H, L.d_W2bf, L.d_prm + L.off_b2, L.Dyp, L.d_Y, nullptr,
+                               skip_of(L, L.d_X), s));
+  }
   {
     EigParams e{};
     e.C = ctx->C;
@@ -206,8 +278,34 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
     e.evecs = b.d_evecs;
     e.counts = b.d_counts;
     e.sweeps = b.d_sweeps;
+    e.fallback = b.d_fallback;
+    e.nsig = b.d_nsig;
+    e.plain_threads = ctx->eig_plain_threads;
+    e.fallback_threads = ctx->eig_fallback_threads;
+    if (L.ready) {
+      e.KW = L.KW;
+      e.basis_out = L.d_basis;
+      e.refine_sweeps = ctx->refine_sweeps;
+      e.power = ctx->power;
+      e.res_tol = ctx->res_tol;
+      if (learned) {
+        e.proposal = reinterpret_cast<const float2 *>(L.d_Y);
+        e.prop_stride = L.Dyp / 2;
+      } else if (previous) {
+        e.proposal = L.d_basis;
+        e.prop_stride = L.KW * ctx->R;
+      }
+    }
     Prof pr(ctx, s, LAM_K_EIG);
     HIPC(ctx, launch_eig(e, s));
+  }
+  if (capture) {  // labels: this block's plain-Jacobi vectors and mode counts
+    const size_t nu = (size_t)L.KW * ctx->R;
+    HIPC(ctx, hipMemcpyAsync(L.d_dsU + (size_t)L.cap_count * M * nu, L.d_basis, M * nu * sizeof(float2),
+                             hipMemcpyDeviceToDevice, s));
+    HIPC(ctx, hipMemcpyAsync(L.d_dsK + (size_t)L.cap_count * M, b.d_nsig, M * sizeof(int),
+                             hipMemcpyDeviceToDevice, s));
+    L.cap_count += 1;
   }
   {
     VoltParams pb = p;
@@ -223,7 +321,9 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
 }
 
 int run_block(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
-  const bool graphs = ctx->use_graphs && !ctx->profiling;
+  const Learn &L = ctx->lr;
+  const bool capturing = L.ready && ctx->eig_method == LAM_EIG_PLAIN && L.cap_count < L.cap_max;
+  const bool graphs = ctx->use_graphs && !ctx->profiling && !capturing;
   if (!graphs) return enqueue_kernels(ctx, b, samples, scales);
   if (ctx->graphs_dirty) {
     for (auto &bb : ctx->bufs)
@@ -279,6 +379,121 @@ int update_wop(lam_ctx *ctx) {
   HIPC(ctx, launch_prep_weights(ctx->d_weights, ctx->d_gains, ctx->d_phase, ctx->C, ctx->B,
                                 ctx->R, ctx->NMT1, ctx->NK, ctx->d_wop, s));
   HIPC(ctx, hipStreamSynchronize(s));
+  return LAM_OK;
+}
+
+
+void free_learn(Learn &L) {
+  for (void *q : {(void *)L.d_prm, (void *)L.d_grad, (void *)L.d_m1, (void *)L.d_m2,
+                  (void *)L.d_W1bf, (void *)L.d_W2bf, (void *)L.d_X, (void *)L.d_H, (void *)L.d_Y,
+                  (void *)L.d_basis, (void *)L.d_dsX, (void *)L.d_dsU, (void *)L.d_dsK,
+                  (void *)L.d_bX, (void *)L.d_bH, (void *)L.d_bU, (void *)L.d_bK, (void *)L.d_idx,
+                  (void *)L.d_bZ, (void *)L.d_bY, (void *)L.d_bdY, (void *)L.d_bdH, (void *)L.d_loss})
+    if (q) hipFree(q);
+  L = Learn{};
+}
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+
+// Allocate the MLP and its per-block buffers; hidden-layer weights
+// U(-1/sqrt(fan_in), +1/sqrt(fan_in)), zero output layer, biases and padding.
+int ensure_learn(lam_ctx *ctx, int KW, int H, uint64_t seed, bool reinit) {
+  Learn &L = ctx->lr;
+  if (L.ready && !reinit) return LAM_OK;
+  if (KW < 2 || KW > kMaxWarm || (KW & 1) || H < 32 || H % 32)
+    return fail(ctx, LAM_ERR_ARG, "learn: warm_vectors must be even in [2, %d], hidden a multiple of 32", kMaxWarm);
+  if (int r = sync_all(ctx)) return r;
+  free_learn(L);
+  const int n = ctx->R, M = ctx->C * 2;
+  L.KW = KW;
+  L.n = n;
+  L.H = H;
+  L.Dx = n * (n - 1) + 2 * KW * n;
+  L.Dp = round_up(L.Dx, 32);
+  L.Dy = 2 * KW * n;
+  L.Dyp = round_up(L.Dy, 32);
+  L.off_b1 = (long)H * L.Dp;
+  L.off_W2 = L.off_b1 + H;
+  L.off_b2 = L.off_W2 + (long)L.Dyp * H;
+  L.P = L.off_b2 + L.Dyp;
+  int rc = LAM_OK;
+  auto chk = [&](int r) {
+    if (r != LAM_OK && rc == LAM_OK) rc = r;
+  };
+  chk(dalloc(ctx, &L.d_prm, L.P * sizeof(float)));
+  chk(dalloc(ctx, &L.d_grad, L.P * sizeof(float)));
+  chk(dalloc(ctx, &L.d_m1, L.P * sizeof(float)));
+  chk(dalloc(ctx, &L.d_m2, L.P * sizeof(float)));
+  chk(dalloc(ctx, &L.d_W1bf, (size_t)H * L.Dp * sizeof(uint16_t)));
+  chk(dalloc(ctx, &L.d_W2bf, (size_t)L.Dyp * H * sizeof(uint16_t)));
+  chk(dalloc(ctx, &L.d_X, (size_t)M * L.Dp * sizeof(uint16_t)));
+  chk(dalloc(ctx, &L.d_H, (size_t)M * H * sizeof(uint16_t)));
+  chk(dalloc(ctx, &L.d_Y, (size_t)M * L.Dyp * sizeof(float)));
+  chk(dalloc(ctx, &L.d_basis, (size_t)M * KW * n * sizeof(float2)));
+  chk(dalloc(ctx, &L.d_loss, sizeof(float)));
+  if (rc != LAM_OK) {
+    free_learn(L);
+    return rc;
+  }
+  std::vector<float> prm(L.P, 0.0f);
+  std::mt19937_64 gen(seed);
+  std::uniform_real_distribution<float> u1(-1.0f / std::sqrt((float)L.Dx), 1.0f / std::sqrt((float)L.Dx));
+  for (int h = 0; h < H; ++h)
+    for (int d = 0; d < L.Dx; ++d) prm[(size_t)h * L.Dp + d] = u1(gen);
+  // the output layer starts at zero: the untrained network proposes the
+  // previous block's vectors through the skip connection
+  HIPC(ctx, hipMemcpy(L.d_prm, prm.data(), L.P * sizeof(float), hipMemcpyHostToDevice));
+  hipStream_t st = ctx->bufs[0].stream;
+  HIPC(ctx, launch_to_bf16(L.d_prm, (long)H * L.Dp, L.d_W1bf, st));
+  HIPC(ctx, launch_to_bf16(L.d_prm + L.off_W2, (long)L.Dyp * H, L.d_W2bf, st));
+  HIPC(ctx, hipStreamSynchronize(st));
+  L.ready = true;
+  ctx->graphs_dirty = true;
+  return LAM_OK;
+}
+
+int ensure_batch(lam_ctx *ctx, int Mb) {
+  Learn &L = ctx->lr;
+  if (L.Mb >= Mb) return LAM_OK;
+  for (void *q : {(void *)L.d_bX, (void *)L.d_bH, (void *)L.d_bU, (void *)L.d_bK, (void *)L.d_bZ,
+                  (void *)L.d_bY, (void *)L.d_bdY, (void *)L.d_bdH})
+    if (q) hipFree(q);
+  L.d_bX = nullptr;
+  L.d_bH = nullptr;
+  L.d_bU = nullptr;
+  L.d_bK = nullptr;
+  L.d_bZ = L.d_bY = L.d_bdY = L.d_bdH = nullptr;
+  L.Mb = 0;
+  if (int r = dalloc(ctx, &L.d_bX, (size_t)Mb * L.Dp * sizeof(uint16_t))) return r;
+  if (int r = dalloc(ctx, &L.d_bH, (size_t)Mb * L.H * sizeof(uint16_t))) return r;
+  if (int r = dalloc(ctx, &L.d_bU, (size_t)Mb * L.KW * L.n * sizeof(float2))) return r;
+  if (int r = dalloc(ctx, &L.d_bK, (size_t)Mb * sizeof(int))) return r;
+  if (int r = dalloc(ctx, &L.d_bZ, (size_t)Mb * L.H * sizeof(float))) return r;
+  if (int r = dalloc(ctx, &L.d_bdH, (size_t)Mb * L.H * sizeof(float))) return r;
+  if (int r = dalloc(ctx, &L.d_bY, (size_t)Mb * L.Dyp * sizeof(float))) return r;
+  if (int r = dalloc(ctx, &L.d_bdY, (size_t)Mb * L.Dyp * sizeof(float))) return r;
+  L.Mb = Mb;
+  return LAM_OK;
+}
+
+// Forward, loss and backward of one batch held in the batch buffers; fills d_grad.
+int forward_backward(lam_ctx *ctx, int nb, hipStream_t s) {
+  Learn &L = ctx->lr;
+  HIPC(ctx, launch_dense_fwd(L.d_bX, nb, L.Dp, L.d_W1bf, L.d_prm + L.off_b1, L.H, L.d_bZ, L.d_bH, Skip{}, s));
+  HIPC(ctx, launch_dense_fwd(L.d_bH, nb, L.H, L.d_W2bf, L.d_prm + L.off_b2, L.Dyp, L.d_bY, nullptr,
+                             skip_of(L, L.d_bX), s));
+  HIPC(ctx, launch_loss(L.d_bY, L.Dyp, L.d_bU, L.d_bK, nb, L.n, L.KW, L.d_bdY, L.d_loss, s));
+  // dW2[o][h] = sum_m dY[m][o] H[m][h];  db2 = colsum(dY)
+  HIPC(ctx, launch_gemm_bwd(true, L.d_bdY, L.Dyp, L.d_bH, L.H, L.d_grad + L.off_W2, L.H, L.Dyp, L.H, nb,
+                            nullptr, 0, s));
+  HIPC(ctx, launch_colsum(L.d_bdY, nb, L.Dyp, L.d_grad + L.off_b2, s));
+  // dH[m][h] = sum_o dY[m][o] W2[o][h] * (Z1[m][h] > 0)
+  HIPC(ctx, launch_gemm_bwd(false, L.d_bdY, L.Dyp, L.d_W2bf, L.H, L.d_bdH, L.H, nb, L.H, L.Dyp, L.d_bZ,
+                            L.H, s));
+  // dW1[h][d] = sum_m dH[m][h] X[m][d];  db1 = colsum(dH)
+  HIPC(ctx, launch_gemm_bwd(true, L.d_bdH, L.H, L.d_bX, L.Dp, L.d_grad, L.Dp, L.H, L.Dp, nb, nullptr, 0, s));
+  HIPC(ctx, launch_colsum(L.d_bdH, nb, L.H, L.d_grad + L.off_b1, s));
   return LAM_OK;
 }
 
@@ -358,6 +573,8 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
       chk(dalloc(ctx, &b.d_evecs, (size_t)ctx->C * 2 * ctx->R * ctx->R * 2 * sizeof(float)));
       chk(dalloc(ctx, &b.d_counts, (size_t)ctx->C * 2 * sizeof(int)));
       chk(dalloc(ctx, &b.d_sweeps, (size_t)ctx->C * 2 * sizeof(int)));
+      chk(dalloc(ctx, &b.d_fallback, (size_t)ctx->C * 2 * sizeof(int)));
+      chk(dalloc(ctx, &b.d_nsig, (size_t)ctx->C * 2 * sizeof(int)));
     }
   }
   chk(dalloc(ctx, &ctx->d_vis_acc, vis_elems * sizeof(long long)));
@@ -412,6 +629,8 @@ int lam_destroy(lam_ctx *ctx) {
     hipFree(b.d_evecs);
     hipFree(b.d_counts);
     hipFree(b.d_sweeps);
+    hipFree(b.d_fallback);
+    hipFree(b.d_nsig);
     if (b.stream) hipStreamDestroy(b.stream);
   }
   hipFree(ctx->d_vis_acc);
@@ -426,6 +645,7 @@ int lam_destroy(lam_ctx *ctx) {
   hipFree(ctx->d_err);
   hipFree(ctx->d_dump_f32);
   hipFree(ctx->d_dump_i64);
+  free_learn(ctx->lr);
   if (ctx->reset_done) hipEventDestroy(ctx->reset_done);
   delete ctx;
   return LAM_OK;
@@ -539,6 +759,15 @@ int lam_set_tuning(lam_ctx *ctx, int samples_per_wg) {
   return LAM_OK;
 }
 
+int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads, int fallback_threads) {
+  auto ok = [](int t) { return t == 256 || t == 512 || t == 1024; };
+  if (!ctx || !ok(plain_threads) || !ok(fallback_threads)) return LAM_ERR_ARG;
+  ctx->eig_plain_threads = plain_threads;
+  ctx->eig_fallback_threads = fallback_threads;
+  ctx->graphs_dirty = true;
+  return LAM_OK;
+}
+
 int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches) {
   if (!ctx) return LAM_ERR_ARG;
   if (int r = sync_all(ctx)) return r;
@@ -580,6 +809,9 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
   }
   // never accumulate into a visibility buffer that a dump has not reset yet
   HIPC(ctx, hipStreamWaitEvent(s, ctx->reset_done, 0));
+  // the warm-start chain (and the capture counter) orders consecutive blocks
+  if (ctx->lr.ready && ctx->last_buf >= 0 && ctx->last_buf != ctx->cur)
+    HIPC(ctx, hipStreamWaitEvent(s, ctx->bufs[ctx->last_buf].done, 0));
   if (int r = run_block(ctx, b, smp, scl)) return r;
   if (io->inputs_on_device && io->release_fn)
     HIPC(ctx, hipLaunchHostFunc(s, io->release_fn, io->release_user));
@@ -607,6 +839,7 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
     if (any && io->eigen_done_fn) HIPC(ctx, hipLaunchHostFunc(s, io->eigen_done_fn, io->eigen_done_user));
   }
   HIPC(ctx, hipEventRecord(b.done, s));
+  ctx->last_buf = ctx->cur;
   ctx->blocks_acc += 1;
   ctx->cur = (ctx->cur + 1) % ctx->nbuf;
   return LAM_OK;
@@ -692,6 +925,200 @@ void *lam_host_alloc(size_t bytes) {
 
 void lam_host_free(void *p) {
   if (p) (void)hipHostFree(p);
+}
+
+int lam_set_eig_method(lam_ctx *ctx, int method, int refine_sweeps, int power, float residual_tol) {
+  if (!ctx) return LAM_ERR_ARG;
+  if (method < LAM_EIG_PLAIN || method > LAM_EIG_PREVIOUS || refine_sweeps < 0 || refine_sweeps > 16 ||
+      !(residual_tol >= 0.0f))
+    return fail(ctx, LAM_ERR_ARG, "lam_set_eig_method: bad argument");
+  if (ctx->mode != LAM_MODE_ADAPTIVE) return fail(ctx, LAM_ERR_STATE, "eigen method needs the adaptive mode");
+  if (method != LAM_EIG_PLAIN)
+    if (int r = ensure_learn(ctx, 4, 256, 1, false)) return r;
+  if (int r = sync_all(ctx)) return r;
+  ctx->eig_method = method;
+  ctx->refine_sweeps = refine_sweeps;
+  ctx->power = power ? 1 : 0;
+  ctx->res_tol = residual_tol;
+  ctx->graphs_dirty = true;
+  return LAM_OK;
+}
+
+int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback) {
+  if (!ctx) return LAM_ERR_ARG;
+  if (ctx->mode != LAM_MODE_ADAPTIVE || ctx->last_buf < 0) return fail(ctx, LAM_ERR_STATE, "no adaptive block run yet");
+  if (int r = sync_all(ctx)) return r;
+  Buf &b = ctx->bufs[ctx->last_buf];
+  const size_t nb = (size_t)ctx->C * 2 * sizeof(int);
+  if (sweeps) HIPC(ctx, hipMemcpy(sweeps, b.d_sweeps, nb, hipMemcpyDeviceToHost));
+  if (fallback) HIPC(ctx, hipMemcpy(fallback, b.d_fallback, nb, hipMemcpyDeviceToHost));
+  return LAM_OK;
+}
+
+int lam_learn_init(lam_ctx *ctx, int warm_vectors, int hidden, uint64_t seed) {
+  if (!ctx) return LAM_ERR_ARG;
+  if (ctx->mode != LAM_MODE_ADAPTIVE) return fail(ctx, LAM_ERR_STATE, "learning needs the adaptive mode");
+  if (ctx->R < 2 * warm_vectors) return fail(ctx, LAM_ERR_ARG, "need receivers >= 2 * warm_vectors");
+  return ensure_learn(ctx, warm_vectors, hidden, seed, true);
+}
+
+int lam_learn_dims(lam_ctx *ctx, int32_t *dims, int64_t *params) {
+  if (!ctx) return LAM_ERR_ARG;
+  if (int r = ensure_learn(ctx, 4, 256, 1, false)) return r;
+  const Learn &L = ctx->lr;
+  if (dims) {
+    const int32_t d[7] = {L.KW, L.n, L.Dx, L.Dp, L.H, L.Dy, L.Dyp};
+    std::memcpy(dims, d, sizeof d);
+  }
+  if (params) *params = L.P;
+  return LAM_OK;
+}
+
+int lam_learn_capture(lam_ctx *ctx, int max_blocks) {
+  if (!ctx || max_blocks < 0) return LAM_ERR_ARG;
+  if (int r = ensure_learn(ctx, 4, 256, 1, false)) return r;
+  if (int r = sync_all(ctx)) return r;
+  Learn &L = ctx->lr;
+  if (L.d_dsX) hipFree(L.d_dsX);
+  if (L.d_dsU) hipFree(L.d_dsU);
+  if (L.d_dsK) hipFree(L.d_dsK);
+  L.d_dsX = nullptr;
+  L.d_dsU = nullptr;
+  L.d_dsK = nullptr;
+  L.cap_max = L.cap_count = 0;
+  if (max_blocks == 0) return LAM_OK;
+  const size_t M = (size_t)ctx->C * 2;
+  if (int r = dalloc(ctx, &L.d_dsX, max_blocks * M * L.Dp * sizeof(uint16_t))) return r;
+  if (int r = dalloc(ctx, &L.d_dsU, max_blocks * M * L.KW * L.n * sizeof(float2))) return r;
+  if (int r = dalloc(ctx, &L.d_dsK, max_blocks * M * sizeof(int))) return r;
+  L.cap_max = max_blocks;
+  ctx->graphs_dirty = true;
+  return LAM_OK;
+}
+
+int lam_learn_captured(lam_ctx *ctx) { return ctx ? ctx->lr.cap_count : 0; }
+
+int lam_learn_train(lam_ctx *ctx, int epochs, int batch, float lr, uint64_t seed, float *loss_per_epoch) {
+  if (!ctx || epochs < 0 || batch < 1 || !(lr > 0.0f)) return LAM_ERR_ARG;
+  Learn &L = ctx->lr;
+  if (!L.ready || L.cap_count == 0) return fail(ctx, LAM_ERR_STATE, "no captured training blocks");
+  if (int r = sync_all(ctx)) return r;
+  if (int r = ensure_batch(ctx, batch)) return r;
+  const long N = (long)L.cap_count * ctx->C * 2;
+  if (L.idx_cap < N) {
+    if (L.d_idx) hipFree(L.d_idx);
+    L.d_idx = nullptr;
+    L.idx_cap = 0;
+    if (int r = dalloc(ctx, &L.d_idx, N * sizeof(int))) return r;
+    L.idx_cap = N;
+  }
+  hipStream_t s = ctx->bufs[0].stream;
+  std::vector<int> perm(N);
+  std::mt19937_64 gen(seed);
+  const int nu = L.KW * L.n;
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::iota(perm.begin(), perm.end(), 0);
+    std::shuffle(perm.begin(), perm.end(), gen);
+    HIPC(ctx, hipMemcpyAsync(L.d_idx, perm.data(), N * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPC(ctx, launch_zero(L.d_loss, sizeof(float), s));
+    int steps = 0;
+    for (long b0 = 0; b0 < N; b0 += batch) {
+      const int nb = (int)std::min<long>(batch, N - b0);
+      HIPC(ctx, launch_gather(L.d_idx + b0, nb, L.d_dsX, L.Dp, L.d_dsU, nu, L.d_dsK, L.d_bX, L.d_bU, L.d_bK, s));
+      if (int r = forward_backward(ctx, nb, s)) return r;
+      L.adam_t += 1;
+      HIPC(ctx, launch_adam(L.d_prm, L.d_grad, L.d_m1, L.d_m2, L.P, lr, 0.9f, 0.999f, 1e-8f, L.adam_t, 0,
+                            (long)L.H * L.Dp, L.d_W1bf, L.off_W2, (long)L.Dyp * L.H, L.d_W2bf, s));
+      ++steps;
+    }
+    float tot = 0.0f;
+    HIPC(ctx, hipMemcpyAsync(&tot, L.d_loss, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPC(ctx, hipStreamSynchronize(s));
+    if (loss_per_epoch) loss_per_epoch[ep] = tot / (float)std::max(1, steps);
+  }
+  ctx->graphs_dirty = true;
+  return LAM_OK;
+}
+
+int lam_learn_params(lam_ctx *ctx, float *params, int64_t count, int set) {
+  if (!ctx || !params) return LAM_ERR_ARG;
+  if (int r = ensure_learn(ctx, 4, 256, 1, false)) return r;
+  Learn &L = ctx->lr;
+  if (count != L.P) return fail(ctx, LAM_ERR_ARG, "parameter count %lld != %ld", (long long)count, L.P);
+  if (int r = sync_all(ctx)) return r;
+  if (!set) {
+    HIPC(ctx, hipMemcpy(params, L.d_prm, L.P * sizeof(float), hipMemcpyDeviceToHost));
+    return LAM_OK;
+  }
+  HIPC(ctx, hipMemcpy(L.d_prm, params, L.P * sizeof(float), hipMemcpyHostToDevice));
+  hipStream_t st = ctx->bufs[0].stream;
+  HIPC(ctx, launch_to_bf16(L.d_prm, (long)L.H * L.Dp, L.d_W1bf, st));
+  HIPC(ctx, launch_to_bf16(L.d_prm + L.off_W2, (long)L.Dyp * L.H, L.d_W2bf, st));
+  HIPC(ctx, hipStreamSynchronize(st));
+  L.adam_t = 0;
+  HIPC(ctx, hipMemset(L.d_m1, 0, L.P * sizeof(float)));
+  HIPC(ctx, hipMemset(L.d_m2, 0, L.P * sizeof(float)));
+  return LAM_OK;
+}
+
+int lam_learn_grad(lam_ctx *ctx, const uint16_t *x, const float *labels, const int32_t *k, int m,
+                   float *loss, float *grad) {
+  if (!ctx || !x || !labels || !k || m < 1) return LAM_ERR_ARG;
+  if (int r = ensure_learn(ctx, 4, 256, 1, false)) return r;
+  if (int r = sync_all(ctx)) return r;
+  if (int r = ensure_batch(ctx, m)) return r;
+  Learn &L = ctx->lr;
+  hipStream_t s = ctx->bufs[0].stream;
+  HIPC(ctx, hipMemcpy(L.d_bX, x, (size_t)m * L.Dp * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIPC(ctx, hipMemcpy(L.d_bU, labels, (size_t)m * L.KW * L.n * sizeof(float2), hipMemcpyHostToDevice));
+  HIPC(ctx, hipMemcpy(L.d_bK, k, (size_t)m * sizeof(int), hipMemcpyHostToDevice));
+  HIPC(ctx, launch_zero(L.d_loss, sizeof(float), s));
+  if (int r = forward_backward(ctx, m, s)) return r;
+  HIPC(ctx, hipStreamSynchronize(s));
+  if (loss) HIPC(ctx, hipMemcpy(loss, L.d_loss, sizeof(float), hipMemcpyDeviceToHost));
+  if (grad) HIPC(ctx, hipMemcpy(grad, L.d_grad, L.P * sizeof(float), hipMemcpyDeviceToHost));
+  return LAM_OK;
+}
+
+int lam_synth_block(const lam_shape *sh, uint64_t seed, int32_t block_index, float sigma, int32_t n_sources,
+                    const float *d_tones, const float *d_steer, const float *d_gain, int8_t *d_samples,
+                    int16_t *d_scales, void *stream) {
+  if (!sh || !d_gain || !d_samples || !d_scales || n_sources < 0 || n_sources > kGenMaxSources ||
+      (n_sources > 0 && (!d_tones || !d_steer)))
+    return LAM_ERR_ARG;
+  if (sh->nr_receivers != sh->nr_fpga_sources * sh->nr_receivers_per_packet) return LAM_ERR_ARG;
+  GenParams g{};
+  g.C = sh->nr_channels;
+  g.F = sh->nr_fpga_sources;
+  g.NT = sh->nr_time_steps_per_packet;
+  g.R = sh->nr_receivers;
+  g.RPP = sh->nr_receivers_per_packet;
+  g.NP = sh->nr_packets_for_correlation;
+  g.S = n_sources;
+  g.seed = seed;
+  g.block_index = block_index;
+  g.sigma = sigma;
+  g.tones = d_tones;
+  g.steer = reinterpret_cast<const float2 *>(d_steer);
+  g.gain = d_gain;
+  g.samples = d_samples;
+  g.scales = d_scales;
+  if (launch_synth(g, static_cast<hipStream_t>(stream)) != hipSuccess) return LAM_ERR_HIP;
+  // NULL stream: return only when the block is written (the pipeline's own
+  // streams are non-blocking and would not wait for the legacy stream)
+  if (!stream && hipStreamSynchronize(nullptr) != hipSuccess) return LAM_ERR_HIP;
+  return LAM_OK;
+}
+
+int lam_eigh_warm(const float *d_A, const float *d_prop, int n, int nmat, int warm_vectors, int k,
+                  int steps, float tol, float *d_evals, float *d_evecs, float *d_ratio, void *stream) {
+  if (!d_A || !d_prop || n < 1 || n > kMaxReceivers || nmat < 1 || warm_vectors < 2 ||
+      warm_vectors > kMaxWarm || (warm_vectors & 1) || n < 2 * warm_vectors || k < 0 || k > warm_vectors ||
+      steps < 0 || !d_evals || !d_evecs || !d_ratio)
+    return LAM_ERR_ARG;
+  const hipError_t e = launch_eigh_warm(d_A, d_prop, n, nmat, warm_vectors, k, steps, tol, d_evals, d_evecs,
+                                        d_ratio, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? LAM_OK : LAM_ERR_HIP;
 }
 
 int lam_eigh_batched(const float *d_A, int n, int nmat, int whiten, int max_sweeps, float tol,

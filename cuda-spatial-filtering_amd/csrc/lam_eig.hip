@@ -29,10 +29,13 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
 
 // Round-robin position of slot j in round r (circle method, ne even).
 __device__ __forceinline__ int rr_pos(int j, int r, int ne) {
-  return j == 0 ? 0 : 1 + (j - 1 + r) % (ne - 1);
+  // j in [0, ne), r in [0, ne - 1): (j - 1 + r) mod (ne - 1) needs one conditional subtract
+  int v = j - 1 + r;
+  if (v >= ne - 1) v -= ne - 1;
+  return j == 0 ? 0 : 1 + v;
 }
 
-__device__ float block_sum(float v, float *red) {
+__device__ __forceinline__ float block_sum(float v, float *red) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -82,6 +85,22 @@ __device__ __forceinline__ double rcp64(double x) {
 // tau = (a_qq - a_pp) / (2|b|), t = sgn(tau) / (|tau| + sqrt(1 + tau^2)).
 __device__ __forceinline__ Rot make_rotation(float2 b, float app, float aqq) {
   Rot R{1.0f, 0.0f, 1.0f, 0.0f};
+#ifdef LAM_EIG_ROT32
+  {
+    const float b2 = b.x * b.x + b.y * b.y;
+    if (b2 > 0.0f) {
+      const float inv_b = rsqrtf(b2);
+      const float tau = (aqq - app) * 0.5f * inv_b;
+      const float t = 1.0f / (fabsf(tau) + sqrtf(1.0f + tau * tau));
+      const float cc = rsqrtf(1.0f + t * t);
+      R.c = cc;
+      R.s = (tau >= 0.0f ? t : -t) * cc;
+      R.er = b.x * inv_b;
+      R.ei = b.y * inv_b;
+    }
+    return R;
+  }
+#endif
   const double br = b.x, bi = b.y;
   const double b2 = br * br + bi * bi;
   if (b2 > 0.0 && b2 > 1e-60 * ((double)app * app + (double)aqq * aqq)) {
@@ -99,20 +118,44 @@ __device__ __forceinline__ Rot make_rotation(float2 b, float app, float aqq) {
   return R;
 }
 
+// Per-thread block list of one Jacobi round for NTHR threads and n <= 64:
+// upper-triangle 2x2 blocks (at most 32 * 33 / 2).
+template <int NTHR> struct JacobiLists {
+  static constexpr int kMaxBlk = (32 * 33 / 2 + NTHR - 1) / NTHR;
+};
+
 // Cyclic Jacobi on the Hermitian matrix A (column-major, leading dim n) in LDS;
 // V receives the eigenvectors (columns).  Each round rotates n/2 disjoint pairs
 // (round-robin order); the update visits only blocks kp <= kq and writes the
 // mirrored block as its conjugate transpose.  Returns the number of sweeps.
-__device__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol,
+template <int NTHR>
+__device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol,
                           int2 *rpq, Rot *rot, float *red, bool init_v = true) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int ne = n + (n & 1), h = ne / 2;
   const int nblk = h * (h + 1) / 2;
   const float inv_n = 1.0f / (float)n;
+  const int ld = n | 1;
   for (int i = tid; i < n * n; i += nthr) {
     const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
     if (init_v) V[i] = make_float2(r == c ? 1.0f : 0.0f, 0.0f);
-    if (r == c) A[i].y = 0.0f;
+    if (r == c) A[c * ld + r].y = 0.0f;
+  }
+  // this thread's blocks (kp <= kq) of the upper triangle, decoded once
+  constexpr int kMaxBlk = JacobiLists<NTHR>::kMaxBlk;
+  static_assert(kMaxBlk <= 3, "block list holds three entries per thread");
+  int blk0 = 0, blk1 = 0, blk2 = 0;
+  int nmine = 0;
+  for (int idx = tid; idx < nblk && nmine < kMaxBlk; idx += nthr) {
+    int kp = (int)((2.0f * h + 1.0f - sqrtf((2.0f * h + 1.0f) * (2.0f * h + 1.0f) - 8.0f * idx)) * 0.5f);
+    while (kp > 0 && kp * (2 * h - kp + 1) / 2 > idx) --kp;
+    while ((kp + 1) * (2 * h - kp) / 2 <= idx) ++kp;
+    const int kq = kp + idx - kp * (2 * h - kp + 1) / 2;
+    const int code = kp | (kq << 8);
+    if (nmine == 0) blk0 = code;
+    else if (nmine == 1) blk1 = code;
+    else blk2 = code;
+    ++nmine;
   }
   __syncthreads();
   int sweep = 0;
@@ -120,7 +163,7 @@ __device__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol
     float off = 0.0f, tot = 0.0f;
     for (int i = tid; i < n * n; i += nthr) {
       const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
-      const float2 a = A[i];
+      const float2 a = A[c * ld + r];
       const float m = a.x * a.x + a.y * a.y;
       tot += m;
       if (r != c) off += m;
@@ -137,58 +180,55 @@ __device__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol
           q = t;
         }
         Rot R{1.0f, 0.0f, 1.0f, 0.0f};
-        if (q < n) R = make_rotation(A[q * n + p], A[p * n + p].x, A[q * n + q].x);
+        if (q < n) R = make_rotation(A[q * ld + p], A[p * ld + p].x, A[q * ld + q].x);
         rpq[tid] = make_int2(p, q);
         rot[tid] = R;
       }
       __syncthreads();
       // A <- J^H A J on blocks (kp <= kq); the (kq, kp) block is the conjugate transpose
-      for (int idx = tid; idx < nblk; idx += nthr) {
-        // idx -> (kp, kq) over the upper triangle, row-major: kq >= kp
-        int kp = (int)((2.0f * h + 1.0f - sqrtf((2.0f * h + 1.0f) * (2.0f * h + 1.0f) - 8.0f * idx)) * 0.5f);
-        while (kp > 0 && kp * (2 * h - kp + 1) / 2 > idx) --kp;
-        while ((kp + 1) * (2 * h - kp) / 2 <= idx) ++kp;
-        const int kq = kp + idx - kp * (2 * h - kp + 1) / 2;
+#pragma unroll 1
+      for (int j = 0; j < nmine; ++j) {
+        const int code = j == 0 ? blk0 : (j == 1 ? blk1 : blk2);
+        const int kp = code & 0xff, kq = code >> 8;
         const int2 P = rpq[kp], Q = rpq[kq];
         const Rot RP = rot[kp], RQ = rot[kq];
         const bool pv = P.y < n, qv = Q.y < n;  // the second index of a pair may be the dummy
-        float2 x00 = A[Q.x * n + P.x];
-        float2 x01 = qv ? A[Q.y * n + P.x] : make_float2(0.f, 0.f);
-        float2 x10 = pv ? A[Q.x * n + P.y] : make_float2(0.f, 0.f);
-        float2 x11 = (pv && qv) ? A[Q.y * n + P.y] : make_float2(0.f, 0.f);
+        float2 x00 = A[Q.x * ld + P.x];
+        float2 x01 = qv ? A[Q.y * ld + P.x] : make_float2(0.f, 0.f);
+        float2 x10 = pv ? A[Q.x * ld + P.y] : make_float2(0.f, 0.f);
+        float2 x11 = (pv && qv) ? A[Q.y * ld + P.y] : make_float2(0.f, 0.f);
         rotate_block(RP, RQ, x00, x01, x10, x11);
         if (kp == kq) {  // the annihilated pair: exact zeros, real diagonal
           x01 = make_float2(0.f, 0.f);
           x10 = make_float2(0.f, 0.f);
           x00.y = 0.f;
           x11.y = 0.f;
-          A[P.x * n + P.x] = x00;
+          A[P.x * ld + P.x] = x00;
           if (pv) {
-            A[P.y * n + P.x] = x01;
-            A[P.x * n + P.y] = x10;
-            A[P.y * n + P.y] = x11;
+            A[P.y * ld + P.x] = x01;
+            A[P.x * ld + P.y] = x10;
+            A[P.y * ld + P.y] = x11;
           }
         } else {
-          A[Q.x * n + P.x] = x00;
-          A[P.x * n + Q.x] = make_float2(x00.x, -x00.y);
+          A[Q.x * ld + P.x] = x00;
+          A[P.x * ld + Q.x] = make_float2(x00.x, -x00.y);
           if (qv) {
-            A[Q.y * n + P.x] = x01;
-            A[P.x * n + Q.y] = make_float2(x01.x, -x01.y);
+            A[Q.y * ld + P.x] = x01;
+            A[P.x * ld + Q.y] = make_float2(x01.x, -x01.y);
           }
           if (pv) {
-            A[Q.x * n + P.y] = x10;
-            A[P.y * n + Q.x] = make_float2(x10.x, -x10.y);
+            A[Q.x * ld + P.y] = x10;
+            A[P.y * ld + Q.x] = make_float2(x10.x, -x10.y);
           }
           if (pv && qv) {
-            A[Q.y * n + P.y] = x11;
-            A[P.y * n + Q.y] = make_float2(x11.x, -x11.y);
+            A[Q.y * ld + P.y] = x11;
+            A[P.y * ld + Q.y] = make_float2(x11.x, -x11.y);
           }
         }
       }
       // V <- V J
-      const float inv_nn = inv_n;
-      for (int idx = tid; idx < n * h; idx += nthr) {
-        const int k = (int)(((float)idx + 0.5f) * inv_nn), row = idx - k * n;
+      for (int idx = tid; idx < n * h; idx += NTHR) {
+        const int k = (int)(((float)idx + 0.5f) * inv_n), row = idx - k * n;
         const int2 Q = rpq[k];
         if (Q.y >= n) continue;
         const Rot RQ = rot[k];
@@ -204,13 +244,13 @@ __device__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol
 }
 
 // Ascending order of the diagonal of A: perm[rank] = column, ev[rank] = value.
-__device__ void sort_eigen(const float2 *A, int n, int *perm, float *ev) {
-  const int j = threadIdx.x;
+__device__ __forceinline__ void sort_eigen(const float2 *A, int n, int *perm, float *ev) {
+  const int j = threadIdx.x, ld = n | 1;
   if (j < n) {
-    const float lj = A[j * n + j].x;
+    const float lj = A[j * ld + j].x;
     int rank = 0;
     for (int k = 0; k < n; ++k) {
-      const float lk = A[k * n + k].x;
+      const float lk = A[k * ld + k].x;
       rank += (lk < lj) || (lk == lj && k < j);
     }
     perm[rank] = j;
@@ -220,7 +260,7 @@ __device__ void sort_eigen(const float2 *A, int n, int *perm, float *ev) {
 }
 
 // Linear-interpolated percentile (pipeline/common.hpp:461-478), FP32 like the reference.
-__device__ float pct_linear(const float *v, int n, float q) {
+__device__ __forceinline__ float pct_linear(const float *v, int n, float q) {
   if (n <= 0) return 0.0f;
   if (n == 1) return v[0];
   const float pos = fminf(1.0f, fmaxf(0.0f, q)) * (float)(n - 1);
@@ -231,42 +271,404 @@ __device__ float pct_linear(const float *v, int n, float q) {
 
 struct EigSmem {
   float2 *A, *V, *W, *G;
-  float *ev, *d, *f, *red;
+  float2 *A0, *Y, *hv, *t1, *t2;  // warm start: copy of A, proposal, reflectors, scratch
+  float *ev, *d, *f, *red, *htau, *res;
   int *perm;
   int2 *rpq;
   Rot *rot;
-  int *kdet;
+  int *kdet, *mark;
 };
 
-__device__ EigSmem carve(char *base, int n, int B) {
+__device__ __forceinline__ EigSmem carve(char *base, int n, int B) {
   EigSmem s;
   s.A = reinterpret_cast<float2 *>(base);
-  s.V = s.A + n * n;
+  const int ld = n | 1;  // odd column stride of A: conflict-free strided LDS access
+  s.V = s.A + ld * n;
   s.W = s.V + n * n;
   s.G = s.W + 2 * B * n;
-  float *fp = reinterpret_cast<float *>(s.G + B * n);
+  s.A0 = s.G + B * n;
+  s.Y = s.A0 + ld * n;
+  s.hv = s.Y + kMaxWarm * n;
+  s.t1 = s.hv + kMaxWarm * n;
+  s.t2 = s.t1 + 64;
+  float *fp = reinterpret_cast<float *>(s.t2 + 64);
   s.ev = fp;
   s.d = fp + 64;
   s.f = fp + 128;
   s.red = fp + 192;
-  s.perm = reinterpret_cast<int *>(fp + 208);
-  s.rpq = reinterpret_cast<int2 *>(s.perm + 64);
+  s.htau = fp + 208;
+  s.res = fp + 216;  // kMaxWarm * 64
+  s.perm = reinterpret_cast<int *>(s.res + kMaxWarm * 64);
+  s.mark = s.perm + 64;
+  s.rpq = reinterpret_cast<int2 *>(s.mark + 64);
   s.rot = reinterpret_cast<Rot *>(s.rpq + 32);
   s.kdet = reinterpret_cast<int *>(s.rot + 32);
   return s;
 }
 
 size_t eig_smem_bytes(int n, int B) {
-  return sizeof(float2) * (2 * n * n + 3 * B * n) + sizeof(float) * 208 + sizeof(int) * 64 +
-         sizeof(int2) * 32 + sizeof(Rot) * 32 + 16;
+  return sizeof(float2) * (2 * (n | 1) * n + n * n + 3 * B * n + 2 * kMaxWarm * n + 128) +
+         sizeof(float) * (216 + kMaxWarm * 64) + sizeof(int) * 128 + sizeof(int2) * 32 +
+         sizeof(Rot) * 32 + 16;
 }
 
-__global__ void __launch_bounds__(kEigThreads) eig_null_kernel(EigParams P) {
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Number of modes to remove: fixed K, or the detection rule of
+// detectSignalEigenmodeCountsKernel (spatial.cuh:900-950) on ascending ev.
+__device__ __forceinline__ int select_k(const float *ev, int n, int fixed, int detect, float delta, int *kdet) {
+  if (threadIdx.x == 0) {
+    int k = fixed;
+    if (detect) {
+      const float p20 = pct_linear(ev, n, 0.2f), p50 = pct_linear(ev, n, 0.5f),
+                  p80 = pct_linear(ev, n, 0.8f);
+      const float thr = p50 + delta * ((p80 - p20) / (2.0f * 0.8416f));
+      k = 0;
+      for (int i = 0; i < n; ++i) k += ev[i] > thr;
+    }
+    kdet[0] = max(0, min(n, k));
+  }
+  __syncthreads();
+  return kdet[0];
+}
+
+// Wave-level ordering of LDS traffic between the lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Z[k][i] = sum_j A(i, j) Y[k][j] for KW vectors (A column-major, stride ld).
+__device__ __forceinline__ void apply_cols(const float2 *A, int ld, const float2 *Y, float2 *Z, int n,
+                                           int KW) {
+  for (int e = threadIdx.x; e < KW * n; e += blockDim.x) {
+    const int k = e / n, i = e - k * n;
+    float2 z = make_float2(0.f, 0.f);
+    for (int j = 0; j < n; ++j) {
+      const float2 t = cmul(A[j * ld + i], Y[k * n + j]);
+      z.x += t.x;
+      z.y += t.y;
+    }
+    Z[e] = z;
+  }
+}
+
+// Gram-Schmidt with reorthogonalisation ("twice is enough") on KW vectors
+// Y[k][0..n) by wave 0 (lane = row, n <= 64).  A vector that vanishes is
+// replaced by a unit vector before orthogonalising.
+__device__ __forceinline__ void mgs_wave(float2 *Y, int n, int KW) {
+  const int i = threadIdx.x;
+  float2 y[kMaxWarm];
+#pragma unroll
+  for (int k = 0; k < kMaxWarm; ++k) y[k] = (k < KW && i < n) ? Y[k * n + i] : make_float2(0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < kMaxWarm; ++k) {
+    if (k >= KW) break;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      const float n0 = wave_sum(y[k].x * y[k].x + y[k].y * y[k].y);
+#pragma unroll
+      for (int pass = 0; pass < 2; ++pass) {
+#pragma unroll
+        for (int j = 0; j < kMaxWarm; ++j) {
+          if (j >= k) break;
+          const float gr = wave_sum(y[j].x * y[k].x + y[j].y * y[k].y);  // conj(y_j) . y_k
+          const float gi = wave_sum(y[j].x * y[k].y - y[j].y * y[k].x);
+          y[k].x -= gr * y[j].x - gi * y[j].y;
+          y[k].y -= gr * y[j].y + gi * y[j].x;
+        }
+      }
+      const float nrm2 = wave_sum(y[k].x * y[k].x + y[k].y * y[k].y);
+      if (nrm2 > 1e-12f * n0 && nrm2 > 1e-30f) {
+        const float inv = rsqrtf(nrm2);
+        y[k].x *= inv;
+        y[k].y *= inv;
+        break;
+      }
+      // (nearly) dependent: restart from a unit vector
+      y[k] = make_float2(i == (k * 5 + 3 + attempt * 7) % n ? 1.0f : 0.0f, 0.0f);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kMaxWarm; ++k)
+    if (k < KW && i < n) Y[k * n + i] = y[k];
+}
+
+// Cyclic Jacobi on the small Hermitian KW x KW matrix B (row-major, KW even)
+// by wave 0 alone: lane (a, b) owns B(a, b) and Vb(a, b); rotation angles in FP64.
+__device__ __forceinline__ void small_jacobi_wave(float2 *B, float2 *Vb, int KW, int max_sweeps,
+                                                  float tol, EigSmem &S) {
+  const int l = threadIdx.x, a = l >> 3, b = l & 7;
+  const bool act = a < KW && b < KW;
+  if (act) {
+    Vb[a * KW + b] = make_float2(a == b ? 1.0f : 0.0f, 0.0f);
+    if (a == b) B[a * KW + b].y = 0.0f;
+  }
+  wave_sync();
+  for (int sw = 0; sw < max_sweeps; ++sw) {
+    float2 x = act ? B[a * KW + b] : make_float2(0.f, 0.f);
+    const float m = x.x * x.x + x.y * x.y;
+    const float off = wave_sum(a != b ? m : 0.0f), tot = wave_sum(m);
+    if (off == 0.0f || off <= tol * tol * tot) break;
+    for (int r = 0; r < KW - 1; ++r) {
+      if (l < KW / 2) {
+        int p = rr_pos(l, r, KW), q = rr_pos(KW - 1 - l, r, KW);
+        if (p > q) {
+          const int t = p;
+          p = q;
+          q = t;
+        }
+        S.rot[l] = make_rotation(B[p * KW + q], B[p * KW + p].x, B[q * KW + q].x);
+        S.rpq[l] = make_int2(p, q);
+        S.mark[p] = 2 * l;
+        S.mark[q] = 2 * l + 1;
+      }
+      wave_sync();
+      float2 nb = make_float2(0.f, 0.f), nv = make_float2(0.f, 0.f);
+      if (act) {
+        const int sa = S.mark[a], sb = S.mark[b];
+        const int2 Pa = S.rpq[sa >> 1], Pb = S.rpq[sb >> 1];
+        const Rot Ra = S.rot[sa >> 1], Rb = S.rot[sb >> 1];
+        // B(i, j) = B[i * KW + j]; rotate_block expects X(r, c) with rows of pair a
+        float2 x00 = B[Pa.x * KW + Pb.x], x01 = B[Pa.x * KW + Pb.y];
+        float2 x10 = B[Pa.y * KW + Pb.x], x11 = B[Pa.y * KW + Pb.y];
+        rotate_block(Ra, Rb, x00, x01, x10, x11);
+        const int ra = sa & 1, rb = sb & 1;
+        nb = ra ? (rb ? x11 : x10) : (rb ? x01 : x00);
+        if ((sa >> 1) == (sb >> 1)) {  // the annihilated pair
+          if (ra != rb) nb = make_float2(0.f, 0.f);
+          else nb.y = 0.f;
+        }
+        const float2 vp = Vb[a * KW + Pb.x], vq = Vb[a * KW + Pb.y];
+        const float2 y = cmul(make_float2(Rb.er, -Rb.ei), vq);
+        nv = rb ? make_float2(Rb.s * vp.x + Rb.c * y.x, Rb.s * vp.y + Rb.c * y.y)
+                : make_float2(Rb.c * vp.x - Rb.s * y.x, Rb.c * vp.y - Rb.s * y.y);
+      }
+      wave_sync();
+      if (act) {
+        B[a * KW + b] = nb;
+        Vb[a * KW + b] = nv;
+      }
+      wave_sync();
+    }
+  }
+}
+
+// Warm-start eigen solve of the whitened matrix A (stride ld, not modified):
+//   Y <- proposal; `steps` times Y <- orth(A Y); Rayleigh-Ritz on Y^H A Y
+//   (small Jacobi); Ritz vectors U = Y Vb; an orthonormal complement of U from
+//   its Householder QR with Rayleigh quotients as the remaining eigenvalues.
+// Fills S.V (columns: complement, then Ritz vectors by ascending Ritz value),
+// S.ev / S.perm (ascending) and returns the accepted flag of the leakage test
+//   sum_{top K} |A u_k - theta_k u_k|^2 / theta_k * (theta_k - mu) / theta_k <= tol * mu,
+// mu = mean of the remaining eigenvalues: the interference a mode leaks through
+// the null is its angle error squared times its power above the noise floor.
+// A complement direction ranked among the K largest fails the test when its
+// Rayleigh quotient exceeds 1.5 mu (an interferer the warm subspace missed).
+__device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int KW, int K, int steps,
+                                           float tol, EigSmem &S) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  float2 *Y = S.Y, *Z = S.hv;
+  for (int st = 0; st < steps; ++st) {
+    apply_cols(A, ld, Y, Z, n, KW);
+    __syncthreads();
+    for (int e = tid; e < KW * n; e += nthr) Y[e] = Z[e];
+    __syncthreads();
+    if (tid < 64) mgs_wave(Y, n, KW);
+    __syncthreads();
+  }
+  if (steps == 0) {
+    if (tid < 64) mgs_wave(Y, n, KW);
+    __syncthreads();
+  }
+  // Rayleigh-Ritz: B = Y^H A Y
+  apply_cols(A, ld, Y, Z, n, KW);
+  __syncthreads();
+  float2 *Bm = S.t1, *Vb = S.t2;
+  for (int e = tid; e < KW * KW; e += nthr) {
+    const int ra = e / KW, cb = e - ra * KW;
+    float2 g = make_float2(0.f, 0.f);
+    for (int i = 0; i < n; ++i) {
+      const float2 t = cmulc(Z[cb * n + i], Y[ra * n + i]);  // conj(y_a) z_b
+      g.x += t.x;
+      g.y += t.y;
+    }
+    Bm[ra * KW + cb] = g;
+  }
+  __syncthreads();
+  if (tid < 64) small_jacobi_wave(Bm, Vb, KW, 12, 1e-7f, S);
+  __syncthreads();
+  // Ritz values ascending: rank of theta_b
+  int *rk = S.mark;  // reuse: rank -> Ritz index
+  if (tid < KW) {
+    const float tb = Bm[tid * KW + tid].x;
+    int rank = 0;
+    for (int c2 = 0; c2 < KW; ++c2) {
+      const float tc = Bm[c2 * KW + c2].x;
+      rank += (tc < tb) || (tc == tb && c2 < tid);
+    }
+    rk[rank] = tid;
+  }
+  __syncthreads();
+  // Ritz vectors into the last KW columns of V and residuals r = Z vb - theta u
+  const int m = n - KW;
+  for (int e = tid; e < KW * n; e += nthr) {
+    const int k = e / n, i = e - k * n;
+    const int bcol = rk[k];
+    const float th = Bm[bcol * KW + bcol].x;
+    float2 u = make_float2(0.f, 0.f), z = make_float2(0.f, 0.f);
+    for (int j = 0; j < KW; ++j) {
+      const float2 vb = Vb[j * KW + bcol];
+      const float2 t1 = cmul(Y[j * n + i], vb), t2 = cmul(Z[j * n + i], vb);
+      u.x += t1.x;
+      u.y += t1.y;
+      z.x += t2.x;
+      z.y += t2.y;
+    }
+    S.V[(m + k) * n + i] = u;
+    z.x -= th * u.x;
+    z.y -= th * u.y;
+    S.res[k * 64 + i] = z.x * z.x + z.y * z.y;
+  }
+  __syncthreads();
+  // complement: Householder QR of U (wave 0), then V[:, 0..m) = (H_0 ... H_{KW-1})[:, KW..n)
+  if (tid < 64) {
+    const int i = tid;
+    for (int k = 0; k < KW; ++k)
+      if (i < n) Y[k * n + i] = S.V[(m + k) * n + i];
+    wave_sync();
+    for (int k = 0; k < KW; ++k) {
+      const float2 x = (i < n && i >= k) ? Y[k * n + i] : make_float2(0.f, 0.f);
+      const float nrm2 = wave_sum(x.x * x.x + x.y * x.y);
+      const float xkr = __shfl(x.x, k, 64), xki = __shfl(x.y, k, 64);
+      const float axk = sqrtf(xkr * xkr + xki * xki), nrm = sqrtf(nrm2);
+      const float2 alpha = axk > 0.0f ? make_float2(-xkr / axk * nrm, -xki / axk * nrm)
+                                      : make_float2(-nrm, 0.0f);
+      float2 v = x;
+      if (i == k) {
+        v.x -= alpha.x;
+        v.y -= alpha.y;
+      }
+      const float vn2 = wave_sum(v.x * v.x + v.y * v.y);
+      const float tau = (nrm2 > 1e-30f && vn2 > 1e-30f) ? 2.0f / vn2 : 0.0f;
+      if (i < n) Z[k * n + i] = v;  // reflector k
+      if (i == 0) S.htau[k] = tau;
+      for (int j = k + 1; j < KW; ++j) {
+        float2 y = i < n ? Y[j * n + i] : make_float2(0.f, 0.f);
+        const float gr = wave_sum(v.x * y.x + v.y * y.y), gi = wave_sum(v.x * y.y - v.y * y.x);
+        y.x -= tau * (gr * v.x - gi * v.y);
+        y.y -= tau * (gr * v.y + gi * v.x);
+        if (i < n) Y[j * n + i] = y;
+      }
+    }
+  }
+  __syncthreads();
+  // Q columns KW..n-1 = H_0 ... H_{KW-1} e_c: apply the reflectors in reverse order
+  for (int e = tid; e < m * n; e += nthr) {
+    const int j = e / n, i = e - j * n;
+    S.A0[e] = make_float2(i == KW + j ? 1.0f : 0.0f, 0.0f);
+  }
+  __syncthreads();
+  for (int k = KW - 1; k >= 0; --k) {
+    const float tau = S.htau[k];
+    if (tau == 0.0f) continue;
+    const float2 *v = Z + k * n;
+    for (int j = tid; j < m; j += nthr) {  // g_j = v^H q_j
+      float2 g = make_float2(0.f, 0.f);
+      for (int i = 0; i < n; ++i) {
+        const float2 t = cmulc(S.A0[j * n + i], v[i]);
+        g.x += t.x;
+        g.y += t.y;
+      }
+      S.t2[j] = make_float2(tau * g.x, tau * g.y);  // Vb is no longer needed
+    }
+    __syncthreads();
+    for (int e = tid; e < m * n; e += nthr) {  // q_j -= tau (v^H q_j) v
+      const int j = e / n, i = e - j * n;
+      const float2 t = cmul(v[i], S.t2[j]);
+      S.A0[e].x -= t.x;
+      S.A0[e].y -= t.y;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < m * n; e += nthr) S.V[e] = S.A0[e];
+  __syncthreads();
+  // Rayleigh quotients of the complement: lambda_j = q_j^H A q_j
+  for (int e = tid; e < m * n; e += nthr) {  // W = A Q2 into A0
+    const int j = e / n, i = e - j * n;
+    float2 w = make_float2(0.f, 0.f);
+    for (int c2 = 0; c2 < n; ++c2) {
+      const float2 t = cmul(A[c2 * ld + i], S.V[j * n + c2]);
+      w.x += t.x;
+      w.y += t.y;
+    }
+    S.A0[e] = w;
+  }
+  __syncthreads();
+  float *evu = S.f;  // unsorted eigenvalues by column
+  for (int j = tid; j < n; j += nthr) {
+    if (j < m) {
+      float lam = 0.0f;
+      for (int i = 0; i < n; ++i) {
+        const float2 q = S.V[j * n + i], w = S.A0[j * n + i];
+        lam += q.x * w.x + q.y * w.y;
+      }
+      evu[j] = lam;
+    } else {
+      const int bcol = rk[j - m];
+      evu[j] = Bm[bcol * KW + bcol].x;
+    }
+  }
+  __syncthreads();
+  if (tid < n) {
+    const float lj = evu[tid];
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += (evu[k] < lj) || (evu[k] == lj && k < tid);
+    S.perm[rank] = tid;
+    S.ev[rank] = lj;
+  }
+  __syncthreads();
+  // leakage test on the K largest
+  if (tid < 64) {
+    float leak = 0.0f, rest = 0.0f;
+    bool ok = K <= KW;
+    for (int k = 0; k < n - K; ++k) rest += S.ev[k];
+    rest = (n > K) ? rest / (float)(n - K) : 0.0f;
+    for (int k = 0; k < K && ok; ++k) {
+      const int col = S.perm[n - 1 - k];
+      const float th = S.ev[n - 1 - k];
+      if (col < m) {
+        // a complement direction among the K largest: harmless at the noise
+        // floor, a missed interferer above it
+        if (th > 1.5f * rest) ok = false;
+        continue;
+      }
+      const float r2 = wave_sum(tid < n ? S.res[(col - m) * 64 + tid] : 0.0f);
+      // interference left by mode k ~ angle^2 x its excess over the noise floor
+      leak += th > 0.0f ? r2 / th * fmaxf(th - rest, 0.0f) / th : 1e30f;
+    }
+    if (tid == 0) {
+      S.red[1] = ok ? leak / fmaxf(rest, 1e-30f) : 1e30f;
+      S.kdet[1] = (ok && leak <= tol * fmaxf(rest, 1e-30f)) ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  return S.kdet[1] != 0;
+}
+
+template <int NTHR>
+__global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   extern __shared__ __align__(16) char smem_raw[];
   const int n = P.R, B = P.B;
   EigSmem S = carve(smem_raw, n, B);
   const int c = blockIdx.x >> 1, p = blockIdx.x & 1;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1;
+  // second launch after a deferred fallback: only the problems whose warm solve failed
+  if (P.only_failed && !P.fallback[(long)c * 2 + p]) return;
   const long nbl = (long)n * (n + 1) / 2;
 
   // 1. dense Hermitian covariance of pol p (A(i,j) = V[bl(i,j)][p][p], i <= j)
@@ -282,36 +684,74 @@ __global__ void __launch_bounds__(kEigThreads) eig_null_kernel(EigParams P) {
       im += v[1];
     }
     const float2 a = make_float2((float)re, (float)im);
-    S.A[j * n + i] = a;
-    S.A[i * n + j] = make_float2(a.x, -a.y);
+    S.A[j * ld + i] = a;
+    S.A[i * ld + j] = make_float2(a.x, -a.y);
   }
   __syncthreads();
   // 2. whitening (spatial.cuh:1065-1093)
-  if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * n + tid].x, 1e-10f));
+  if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f));
   __syncthreads();
   for (int i = tid; i < n * n; i += nthr) {
-    const float den = S.d[i % n] * S.d[i / n];
-    S.A[i].x /= den;
-    S.A[i].y /= den;
+    const int col = i / n, row = i - col * n;
+    const float den = S.d[row] * S.d[col];
+    S.A[col * ld + row].x /= den;
+    S.A[col * ld + row].y /= den;
   }
   __syncthreads();
-  // 3. eigen decomposition
-  const int sweeps = jacobi_lds(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
-  sort_eigen(S.A, n, S.perm, S.ev);
-  // 4. K
-  if (tid == 0) {
-    int k = P.fixed_k[c * 2 + p];
-    if (P.detect) {
-      const float p20 = pct_linear(S.ev, n, 0.2f), p50 = pct_linear(S.ev, n, 0.5f),
-                  p80 = pct_linear(S.ev, n, 0.8f);
-      const float thr = p50 + P.delta * ((p80 - p20) / (2.0f * 0.8416f));
-      k = 0;
-      for (int i = 0; i < n; ++i) k += S.ev[i] > thr;
+  // 3. eigen decomposition: plain cyclic Jacobi, or a warm start refined by a
+  //    few block sweeps and accepted only if the top-K residual is within tolerance
+  const long cp = (long)c * 2 + p;
+  const int fixed = P.fixed_k[c * 2 + p];
+  int sweeps = 0, K = 0, fell = 0;
+  const bool warm = P.proposal != nullptr && !P.only_failed && P.KW > 0 && n >= 2 * P.KW && !P.detect;
+  if (warm) {
+    const int KW = P.KW;
+    for (int e = tid; e < KW * n; e += nthr) S.Y[e] = P.proposal[cp * P.prop_stride + e];
+    __syncthreads();
+    K = max(0, min(n, fixed));
+    const bool ok = warm_solve(S.A, ld, n, KW, K, P.refine_sweeps, P.res_tol, S);
+    sweeps = P.refine_sweeps;
+    if (!ok) {  // fall back to the plain solver on the same matrix
+      if (P.defer) {  // leave it to the second launch
+        if (tid == 0) {
+          P.fallback[cp] = 1;
+          if (P.sweeps) P.sweeps[cp] = sweeps;
+        }
+        return;
+      }
+      sweeps += jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
+      sort_eigen(S.A, n, S.perm, S.ev);
+      fell = 1;
     }
-    S.kdet[0] = max(0, min(n, k));
+    if (tid == 0) S.kdet[0] = K;
+    __syncthreads();
+  } else {
+    sweeps = jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
+    sort_eigen(S.A, n, S.perm, S.ev);
+    K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
+    if (P.only_failed) {
+      sweeps += P.refine_sweeps;
+      fell = 1;
+    }
   }
-  __syncthreads();
-  const int K = S.kdet[0];
+  // modes clearly above the noise floor (> 4x the mean of the remaining eigenvalues):
+  // the label count of the warm-start training set
+  if (tid == 0 && P.nsig) {
+    float rest = 0.0f;
+    for (int i = 0; i < n - K; ++i) rest += S.ev[i];
+    rest = n > K ? rest / (float)(n - K) : 0.0f;
+    int ns = 0;
+    for (int k = 0; k < K; ++k) ns += S.ev[n - 1 - k] > 4.0f * rest;
+    P.nsig[cp] = ns;
+  }
+  // top-KW whitened eigenvectors (descending): the next block's warm start / training labels
+  if (P.basis_out) {
+    for (int e = tid; e < P.KW * n; e += nthr) {
+      const int k = e / n, i = e - k * n;
+      P.basis_out[cp * P.KW * n + e] = k < n ? S.V[S.perm[n - 1 - k] * n + i] : make_float2(0.f, 0.f);
+    }
+  }
+  if (tid == 0 && P.fallback) P.fallback[cp] = fell;
   // 5. output basis U (sorted columns) into S.A: unwhitened+normalised (null) or whitened (shrink)
   if (tid < n) {
     const int src = S.perm[tid];
@@ -353,7 +793,6 @@ __global__ void __launch_bounds__(kEigThreads) eig_null_kernel(EigParams P) {
   }
   __syncthreads();
   // 6. eigen outputs
-  const long cp = (long)c * 2 + p;
   if (tid < n) P.evals[cp * n + tid] = S.ev[tid];
   for (int i = tid; i < n * n; i += nthr)
     reinterpret_cast<float2 *>(P.evecs)[cp * n * n + i] = S.A[i];
@@ -413,15 +852,38 @@ __global__ void __launch_bounds__(kEigThreads) eig_null_kernel(EigParams P) {
   }
 }
 
-hipError_t launch_eig(const EigParams &p, hipStream_t s) {
+template <int NTHR>
+hipError_t launch_eig_t(const EigParams &p, hipStream_t s) {
   const size_t sm = eig_smem_bytes(p.R, p.B);
   if (sm > 65536) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eig_null_kernel),
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eig_null_kernel<NTHR>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(eig_null_kernel, dim3(p.C * 2), dim3(kEigThreads), sm, s, p);
+  hipLaunchKernelGGL(eig_null_kernel<NTHR>, dim3(p.C * 2), dim3(NTHR), sm, s, p);
   return hipGetLastError();
+}
+
+hipError_t launch_eig_threads(const EigParams &p, int nthr, hipStream_t s) {
+  switch (nthr) {
+    case 512: return launch_eig_t<512>(p, s);
+    case 1024: return launch_eig_t<1024>(p, s);
+    default: return launch_eig_t<256>(p, s);
+  }
+}
+
+// Plain: one launch.  Warm start: the warm launch defers failed problems to a
+// second launch with wider workgroups that runs the plain solver on them only.
+hipError_t launch_eig(const EigParams &p, hipStream_t s) {
+  if (!p.proposal) return launch_eig_threads(p, p.plain_threads, s);
+  EigParams w = p;
+  w.defer = 1;
+  w.only_failed = 0;
+  if (hipError_t e = launch_eig_t<256>(w, s)) return e;
+  EigParams f = p;
+  f.defer = 0;
+  f.only_failed = 1;
+  return launch_eig_threads(f, p.fallback_threads, s);
 }
 
 // ---- standalone batched eigensolver (unit-test entry point) ----
@@ -430,20 +892,21 @@ eigh_batched_kernel(const float2 *Ain, int n, int whiten, int max_sweeps, float 
                     float *evals, float2 *evecs, int *sweeps_out) {
   extern __shared__ __align__(16) char smem_raw[];
   EigSmem S = carve(smem_raw, n, 0);
-  const int m = blockIdx.x, tid = threadIdx.x;
-  for (int i = tid; i < n * n; i += blockDim.x) S.A[i] = Ain[(long)m * n * n + i];
+  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
+  for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * ld + i % n] = Ain[(long)m * n * n + i];
   __syncthreads();
   if (whiten) {
-    if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * n + tid].x, 1e-10f));
+    if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f));
     __syncthreads();
     for (int i = tid; i < n * n; i += blockDim.x) {
-      const float den = S.d[i % n] * S.d[i / n];
-      S.A[i].x /= den;
-      S.A[i].y /= den;
+      const int col = i / n, row = i - col * n;
+      const float den = S.d[row] * S.d[col];
+      S.A[col * ld + row].x /= den;
+      S.A[col * ld + row].y /= den;
     }
     __syncthreads();
   }
-  const int sw = jacobi_lds(S.A, S.V, n, max_sweeps, tol, S.rpq, S.rot, S.red);
+  const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.rpq, S.rot, S.red);
   sort_eigen(S.A, n, S.perm, S.ev);
   if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
   for (int i = tid; i < n * n; i += blockDim.x) {
@@ -451,6 +914,42 @@ eigh_batched_kernel(const float2 *Ain, int n, int whiten, int max_sweeps, float 
     evecs[(long)m * n * n + i] = S.V[S.perm[k] * n + r];
   }
   if (tid == 0 && sweeps_out) sweeps_out[m] = sw;
+}
+
+// Warm-start solve of whitened Hermitian matrices (test entry): proposal
+// [nmat][KW][n], outputs ascending eigenvalues / eigenvectors (columns) and
+// the leakage ratio of the K largest modes (accepted when <= tol).
+__global__ void __launch_bounds__(kEigThreads)
+eigh_warm_kernel(const float2 *Ain, const float2 *prop, int n, int KW, int K, int steps, float tol,
+                 float *evals, float2 *evecs, float *ratio) {
+  extern __shared__ __align__(16) char smem_raw[];
+  EigSmem S = carve(smem_raw, n, 0);
+  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
+  for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * ld + i % n] = Ain[(long)m * n * n + i];
+  for (int e = tid; e < KW * n; e += blockDim.x) S.Y[e] = prop[(long)m * KW * n + e];
+  __syncthreads();
+  warm_solve(S.A, ld, n, KW, K, steps, tol, S);
+  if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
+  for (int i = tid; i < n * n; i += blockDim.x) {
+    const int k = i / n, r = i % n;
+    evecs[(long)m * n * n + i] = S.V[S.perm[k] * n + r];
+  }
+  if (tid == 0) ratio[m] = S.red[1];
+}
+
+hipError_t launch_eigh_warm(const float *A, const float *prop, int n, int nmat, int KW, int K,
+                            int steps, float tol, float *evals, float *evecs, float *ratio,
+                            hipStream_t s) {
+  const size_t sm = eig_smem_bytes(n, 0);
+  if (sm > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eigh_warm_kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(eigh_warm_kernel, dim3(nmat), dim3(kEigThreads), sm, s,
+                     reinterpret_cast<const float2 *>(A), reinterpret_cast<const float2 *>(prop), n,
+                     KW, K, steps, tol, evals, reinterpret_cast<float2 *>(evecs), ratio);
+  return hipGetLastError();
 }
 
 hipError_t launch_eigh_batched(const float *A, int n, int nmat, int whiten, int max_sweeps,

@@ -15,7 +15,11 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 constexpr int kChunk = 128;
 constexpr int kMaxReceivers = 64;  // 4 antenna groups of 16
 constexpr int kMaxSplits = 16;     // time splits per channel (partial visibility buffers)
-constexpr int kEigThreads = 256;
+#ifndef LAM_EIG_THREADS
+#define LAM_EIG_THREADS 256
+#endif
+constexpr int kEigThreads = LAM_EIG_THREADS;
+constexpr int kMaxWarm = 8;        // warm-start vectors per eigenproblem
 
 // Error bits reported through the device flag word.
 constexpr int kErrRange = 1;
@@ -62,9 +66,65 @@ struct EigParams {
   float *evecs;            // out: [C][2][R][R][2]
   int *counts;             // out: [C][2]
   int *sweeps;             // out: [C][2] or nullptr
+  // warm start (learned or previous-block proposal); proposal == nullptr: plain Jacobi
+  const float2 *proposal;  // [C*2][prop_stride]: KW vectors of R entries
+  int prop_stride;         // float2 per problem
+  int KW;                  // warm vectors (even, <= kMaxWarm)
+  int refine_sweeps;
+  int power;               // 1: one Y <- A Y step before the Householder basis
+  float res_tol;           // accept if max_k |A u_k - l_k u_k| <= res_tol * l_max
+  float2 *basis_out;       // out: [C*2][KW][R] top-KW whitened eigenvectors, or nullptr
+  int *fallback;           // out: [C*2] 1 where the plain solver ran after the check failed
+  int *nsig;               // out: [C*2] modes above 4x the noise floor among the K removed, or nullptr
+  int plain_threads;       // workgroup width of the plain solver (256, 512, 1024)
+  int fallback_threads;    // workgroup width of the deferred fallback launch
+  int defer, only_failed;  // set by launch_eig for the two launches of the warm start
+};
+
+// Synthetic voltage generator (benchmark / test input).
+constexpr int kGenMaxSources = 8;
+struct GenParams {
+  int C, F, NT, R, RPP, NP;
+  int S;                   // tone slots per channel
+  unsigned long long seed;
+  int block_index;
+  float sigma;             // noise per real component
+  const float *tones;      // [C][S][6]: amplitude, cycles/sample, phase, pol-0 gain, pol-1 gain (re, im)
+  const float2 *steer;     // [C][S][R]
+  const float *gain;       // [C] requantisation gain
+  int8_t *samples;         // ring [C][NP+2][F][NT][RPP][2][2]
+  int16_t *scales;         // [C][NP+2][R][2]
+};
+
+// Features of the learned warm start, one row per (channel, pol).
+struct FeatParams {
+  const int *vis_part;     // [nsplit][C][NBL][2][2][2]
+  int nsplit, C, n, KW, Dp;
+  const float2 *basis;     // [C][2][KW][n] previous block's top-KW whitened eigenvectors or nullptr
+  uint16_t *X;             // out: [C*2][Dp] BF16
 };
 
 // Launchers (defined in the .hip files).
+hipError_t launch_features(const FeatParams &p, hipStream_t s);
+struct Skip {
+  const uint16_t *X = nullptr;  // BF16 rows holding the skip source
+  int ld = 0, off = 0, len = 0;
+  float scale = 0.0f;
+};
+hipError_t launch_dense_fwd(const uint16_t *X, int M, int K, const uint16_t *Wt, const float *b,
+                            int N, float *Z, uint16_t *Hbf, const Skip &sk, hipStream_t s);
+hipError_t launch_gemm_bwd(bool at, const float *A, int lda, const uint16_t *B, int ldb, float *C,
+                           int ldc, int I, int J, int K, const float *mask, int ldm, hipStream_t s);
+hipError_t launch_colsum(const float *D, int M, int N, float *out, hipStream_t s);
+hipError_t launch_loss(const float *Y, int Dyp, const float2 *U, const int *K, int M, int n,
+                       int KW, float *dY, float *loss_acc, hipStream_t s);
+hipError_t launch_adam(float *prm, const float *g, float *m1, float *m2, long P, float lr, float b1,
+                       float b2, float eps, int t, long w1_off, long w1_len, uint16_t *w1bf,
+                       long w2_off, long w2_len, uint16_t *w2bf, hipStream_t s);
+hipError_t launch_to_bf16(const float *src, long n, uint16_t *dst, hipStream_t s);
+hipError_t launch_gather(const int *idx, int nb, const uint16_t *DX, int Dp, const float2 *DU,
+                         int nu, const int *DK, uint16_t *X, float2 *U, int *K, hipStream_t s);
+hipError_t launch_synth(const GenParams &g, hipStream_t s);
 hipError_t launch_volt(const VoltParams &p, bool do_corr, bool do_beam, hipStream_t s);
 hipError_t launch_eig(const EigParams &p, hipStream_t s);
 hipError_t launch_prep_weights(const __half *w, const float2 *gains, const float2 *phase,
@@ -76,6 +136,9 @@ hipError_t launch_dump_vis(long long *acc, int C, int R, const float2 *gains,
                            const float2 *phase, float *out_f32, long long *out_i64,
                            hipStream_t s);
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
+hipError_t launch_eigh_warm(const float *A, const float *prop, int n, int nmat, int KW, int K,
+                            int steps, float tol, float *evals, float *evecs, float *ratio,
+                            hipStream_t s);
 hipError_t launch_eigh_batched(const float *A, int n, int nmat, int whiten, int max_sweeps,
                                float tol, float *evals, float *evecs, int *sweeps,
                                hipStream_t s);

@@ -1,0 +1,348 @@
+// Learned warm start of the eigen stage: feature extraction from the block's
+// visibilities, a 2-layer MLP (BF16 weights, FP32 accumulation) on MFMA for
+// inference, and the FP32 backward / Adam kernels that train it on the GPU.
+//
+// Features of one (channel, pol) problem (n receivers, KW warm vectors):
+//   [0, n(n-1))            whitened covariance A(i,j)/(d_i d_j), i < j, ordered
+//                          idx = j(j-1)/2 + i, (re, im) interleaved
+//   [n(n-1), +2 KW n)      previous block's top-KW whitened eigenvectors x sqrt(n)
+//   zero padding to Dp (multiple of 32)
+// Output: KW complex vectors of n entries (re, im interleaved), padded to Dyp.
+// The covariance and whitening follow spatial.cuh:672-713 and 1065-1093.
+#include "lam_internal.hpp"
+
+namespace lam {
+
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---- features: one workgroup per (channel, pol) ----
+__global__ void __launch_bounds__(256) feature_kernel(FeatParams P) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const int n = P.n, cp = blockIdx.x, c = cp >> 1, p = cp & 1, tid = threadIdx.x;
+  float *d = reinterpret_cast<float *>(smem_raw);
+  const long nbl = (long)n * (n + 1) / 2;
+  if (tid < n) {
+    const long idx = (long)tid * (tid + 1) / 2 + tid;  // diagonal baseline (tid, tid)
+    long re = 0;
+    for (int k = 0; k < P.nsplit; ++k)
+      re += P.vis_part[((((long)k * P.C + c) * nbl + idx) * 4 + p * 2 + p) * 2];
+    d[tid] = sqrtf(fmaxf((float)re, 1e-10f));
+  }
+  __syncthreads();
+  uint16_t *x = P.X + (long)cp * P.Dp;
+  const int noff = n * (n - 1) / 2;
+  for (int e = tid; e < noff; e += blockDim.x) {
+    int j = (int)((1.0f + sqrtf(1.0f + 8.0f * e)) * 0.5f);
+    while (j * (j - 1) / 2 > e) --j;
+    while ((j + 1) * j / 2 <= e) ++j;
+    const int i = e - j * (j - 1) / 2;
+    const long bl = (long)j * (j + 1) / 2 + i;
+    long re = 0, im = 0;
+    for (int k = 0; k < P.nsplit; ++k) {
+      const int *v = P.vis_part + ((((long)k * P.C + c) * nbl + bl) * 4 + p * 2 + p) * 2;
+      re += v[0];
+      im += v[1];
+    }
+    const float den = d[i] * d[j];
+    x[2 * e] = f2bf((float)re / den);
+    x[2 * e + 1] = f2bf((float)im / den);
+  }
+  const int boff = 2 * noff, nb = P.KW * n;
+  const float sq = sqrtf((float)n);
+  for (int e = tid; e < nb; e += blockDim.x) {
+    float2 b = make_float2(0.f, 0.f);
+    if (P.basis) b = P.basis[(long)cp * nb + e];
+    x[boff + 2 * e] = f2bf(sq * b.x);
+    x[boff + 2 * e + 1] = f2bf(sq * b.y);
+  }
+  for (int e = boff + 2 * nb + tid; e < P.Dp; e += blockDim.x) x[e] = 0;
+}
+
+hipError_t launch_features(const FeatParams &p, hipStream_t s) {
+  hipLaunchKernelGGL(feature_kernel, dim3(p.C * 2), dim3(256), 64 * sizeof(float), s, p);
+  return hipGetLastError();
+}
+
+// ---- dense layer on MFMA: Z[M][N] = X[M][K] Wt[N][K]^T + b[N] (+ skip) ----
+// The output layer adds a skip connection from the previous block's vectors
+// held in the features: Z[m][j] += scale * X'[m][off + j] for j < len.
+// One workgroup per 32x32 output tile; its 4 waves split K in 16-wide steps
+// and reduce through LDS.  Output: FP32 Z and/or BF16 relu(Z).
+__global__ void __launch_bounds__(256)
+dense_fwd_kernel(const uint16_t *__restrict__ X, int M, int K, const uint16_t *__restrict__ Wt,
+                 const float *__restrict__ b, int N, float *__restrict__ Z,
+                 uint16_t *__restrict__ Hbf, Skip sk) {
+  __shared__ float red[4][16][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+  const int row = lane & 31, kh = lane >> 5;
+  const int m = m0 + row;
+  const bool mv = m < M;
+  const uint16_t *xa = X + (long)(mv ? m : 0) * K + kh * 8;
+  const uint16_t *wb = Wt + (long)(n0 + row) * K + kh * 8;
+  v16f acc = {};
+  const int nks = K / 16;
+  int ks = w;
+  for (; ks + 12 < nks; ks += 16) {
+    v8bf a[4], bb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      a[u] = *reinterpret_cast<const v8bf *>(xa + (ks + 4 * u) * 16);
+      bb[u] = *reinterpret_cast<const v8bf *>(wb + (ks + 4 * u) * 16);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!mv) a[u] = v8bf{};
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], bb[u], acc, 0, 0, 0);
+    }
+  }
+  for (; ks < nks; ks += 4) {
+    v8bf a = *reinterpret_cast<const v8bf *>(xa + ks * 16);
+    const v8bf bb = *reinterpret_cast<const v8bf *>(wb + ks * 16);
+    if (!mv) a = v8bf{};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int v = 0; v < 16; ++v) red[w][v][lane] = acc[v];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int o = threadIdx.x + 256 * j;
+    const int l = o & 63, v = o >> 6;
+    const float s = red[0][v][l] + red[1][v][l] + red[2][v][l] + red[3][v][l];
+    const int i = 8 * (v >> 2) + 4 * (l >> 5) + (v & 3), col = l & 31;
+    const int mm = m0 + i, nn = n0 + col;
+    if (mm < M) {
+      float z = s + b[nn];
+      if (sk.X && nn < sk.len) z += sk.scale * bf2f(sk.X[(long)mm * sk.ld + sk.off + nn]);
+      if (Z) Z[(long)mm * N + nn] = z;
+      if (Hbf) Hbf[(long)mm * N + nn] = f2bf(fmaxf(z, 0.0f));
+    }
+  }
+}
+
+hipError_t launch_dense_fwd(const uint16_t *X, int M, int K, const uint16_t *Wt, const float *b,
+                            int N, float *Z, uint16_t *Hbf, const Skip &sk, hipStream_t s) {
+  if (K % 16 || N % 32) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(dense_fwd_kernel, dim3((M + 31) / 32, N / 32), dim3(256), 0, s, X, M, K, Wt,
+                     b, N, Z, Hbf, sk);
+  return hipGetLastError();
+}
+
+// ---- FP32 GEMM for the backward pass ----
+// C[i][j] = sum_k A(i,k) B(k,j), A(i,k) = AT ? A[k*lda+i] : A[i*lda+k] (FP32),
+// B(k,j) = Bbf[k*ldb+j] (BF16); optional ReLU mask: C *= (mask[i*ldm+j] > 0).
+template <bool AT>
+__global__ void __launch_bounds__(256)
+gemm_bwd_kernel(const float *__restrict__ A, int lda, const uint16_t *__restrict__ Bbf, int ldb,
+                float *__restrict__ C, int ldc, int I, int J, int K, const float *__restrict__ mask,
+                int ldm) {
+  __shared__ float As[16][64 + 4];
+  __shared__ float Bs[16][64 + 4];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  float acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = tid + 256 * u;  // 0..1023
+      int ii, kk;
+      if (AT) {
+        ii = e & 63;
+        kk = e >> 6;
+      } else {
+        kk = e & 15;
+        ii = e >> 4;
+      }
+      const int gi = i0 + ii, gk = k0 + kk;
+      float v = 0.0f;
+      if (gi < I && gk < K) v = AT ? A[(long)gk * lda + gi] : A[(long)gi * lda + gk];
+      As[kk][ii] = v;
+      const int jj = e & 63, kb = e >> 6;
+      const int gj = j0 + jj, gkb = k0 + kb;
+      Bs[kb][jj] = (gj < J && gkb < K) ? bf2f(Bbf[(long)gkb * ldb + gj]) : 0.0f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      float a[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = As[kk][ty * 4 + u];
+        bv[u] = Bs[kk][tx * 4 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = fmaf(a[u], bv[v], acc[u][v]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int gi = i0 + ty * 4 + u;
+    if (gi >= I) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int gj = j0 + tx * 4 + v;
+      if (gj >= J) continue;
+      float r = acc[u][v];
+      if (mask && !(mask[(long)gi * ldm + gj] > 0.0f)) r = 0.0f;
+      C[(long)gi * ldc + gj] = r;
+    }
+  }
+}
+
+hipError_t launch_gemm_bwd(bool at, const float *A, int lda, const uint16_t *B, int ldb, float *C,
+                           int ldc, int I, int J, int K, const float *mask, int ldm, hipStream_t s) {
+  const dim3 grid((J + 63) / 64, (I + 63) / 64);
+  if (at)
+    hipLaunchKernelGGL(gemm_bwd_kernel<true>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, I, J,
+                       K, mask, ldm);
+  else
+    hipLaunchKernelGGL(gemm_bwd_kernel<false>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, I, J,
+                       K, mask, ldm);
+  return hipGetLastError();
+}
+
+// db[j] = sum_m D[m][j]
+__global__ void colsum_kernel(const float *__restrict__ D, int M, int N, float *__restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= N) return;
+  float s = 0.0f;
+  for (int m = 0; m < M; ++m) s += D[(long)m * N + j];
+  out[j] = s;
+}
+
+hipError_t launch_colsum(const float *D, int M, int N, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(colsum_kernel, dim3((N + 255) / 256), dim3(256), 0, s, D, M, N, out);
+  return hipGetLastError();
+}
+
+// ---- phase-invariant subspace loss, one wave per sample ----
+// L = (1/M) sum_m sum_{k < K_m} (1 - |u_k^H y_k|^2 / (|y_k|^2 + eps))
+// dL/dRe(y) + i dL/dIm(y) = -(2/M) (s u - f y) / (|y|^2 + eps),  s = u^H y, f = |s|^2 / (|y|^2 + eps)
+__global__ void __launch_bounds__(256)
+loss_kernel(const float *__restrict__ Y, int Dyp, const float2 *__restrict__ U,
+            const int *__restrict__ Kl, int M, int n, int KW, float *__restrict__ dY,
+            float *__restrict__ loss_acc) {
+  const int lane = threadIdx.x & 63;
+  const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (m >= M) return;
+  const float eps = 1e-12f;
+  const float invM = 1.0f / (float)M;
+  const int K = min(Kl[m], KW);
+  const float2 *y = reinterpret_cast<const float2 *>(Y + (long)m * Dyp);
+  float2 *dy = reinterpret_cast<float2 *>(dY + (long)m * Dyp);
+  float total = 0.0f;
+  for (int k = 0; k < KW; ++k) {
+    float sr = 0.f, si = 0.f, nn = 0.f;
+    for (int i = lane; i < n; i += 64) {
+      const float2 yv = y[k * n + i], uv = U[((long)m * KW + k) * n + i];
+      sr += uv.x * yv.x + uv.y * yv.y;  // conj(u) y
+      si += uv.x * yv.y - uv.y * yv.x;
+      nn += yv.x * yv.x + yv.y * yv.y;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      sr += __shfl_xor(sr, o, 64);
+      si += __shfl_xor(si, o, 64);
+      nn += __shfl_xor(nn, o, 64);
+    }
+    const float den = nn + eps;
+    const float f = (sr * sr + si * si) / den;
+    const bool on = k < K;
+    if (on) total += 1.0f - f;
+    for (int i = lane; i < n; i += 64) {
+      float2 g = make_float2(0.f, 0.f);
+      if (on) {
+        const float2 yv = y[k * n + i], uv = U[((long)m * KW + k) * n + i];
+        // s u - f y with s = sr + i si
+        const float gr = (sr * uv.x - si * uv.y) - f * yv.x;
+        const float gi = (sr * uv.y + si * uv.x) - f * yv.y;
+        const float sc = -2.0f * invM / den;
+        g = make_float2(sc * gr, sc * gi);
+      }
+      dy[k * n + i] = g;
+    }
+  }
+  // padded outputs carry no gradient
+  for (int e = 2 * KW * n + lane; e < Dyp; e += 64) dY[(long)m * Dyp + e] = 0.0f;
+  if (lane == 0 && loss_acc) atomicAdd(loss_acc, total * invM);
+}
+
+hipError_t launch_loss(const float *Y, int Dyp, const float2 *U, const int *K, int M, int n,
+                       int KW, float *dY, float *loss_acc, hipStream_t s) {
+  hipLaunchKernelGGL(loss_kernel, dim3((M + 3) / 4), dim3(256), 0, s, Y, Dyp, U, K, M, n, KW, dY,
+                     loss_acc);
+  return hipGetLastError();
+}
+
+// ---- Adam on the FP32 master parameters, refreshing the BF16 weight copies ----
+__global__ void adam_kernel(float *__restrict__ prm, const float *__restrict__ g,
+                            float *__restrict__ m1, float *__restrict__ m2, long P, float lr,
+                            float b1, float b2, float eps, float c1, float c2, long w1_off,
+                            long w1_len, uint16_t *__restrict__ w1bf, long w2_off, long w2_len,
+                            uint16_t *__restrict__ w2bf) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.0f - b1) * gi;
+  const float v = b2 * m2[i] + (1.0f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = v;
+  const float mh = a / c1, vh = v / c2;
+  const float w = prm[i] - lr * mh / (sqrtf(vh) + eps);
+  prm[i] = w;
+  if (i >= w1_off && i < w1_off + w1_len) w1bf[i - w1_off] = f2bf(w);
+  if (i >= w2_off && i < w2_off + w2_len) w2bf[i - w2_off] = f2bf(w);
+}
+
+hipError_t launch_adam(float *prm, const float *g, float *m1, float *m2, long P, float lr, float b1,
+                       float b2, float eps, int t, long w1_off, long w1_len, uint16_t *w1bf,
+                       long w2_off, long w2_len, uint16_t *w2bf, hipStream_t s) {
+  const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s, prm, g, m1,
+                     m2, P, lr, b1, b2, eps, c1, c2, w1_off, w1_len, w1bf, w2_off, w2_len, w2bf);
+  return hipGetLastError();
+}
+
+// BF16 copies of the weight blocks of the master parameters.
+__global__ void to_bf16_kernel(const float *__restrict__ src, long n, uint16_t *__restrict__ dst) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = f2bf(src[i]);
+}
+
+hipError_t launch_to_bf16(const float *src, long n, uint16_t *dst, hipStream_t s) {
+  hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, n,
+                     dst);
+  return hipGetLastError();
+}
+
+// Gather training rows: X[b] = DX[idx[b]], U[b] = DU[idx[b]], K[b] = DK[idx[b]].
+__global__ void gather_kernel(const int *__restrict__ idx, int nb, const uint16_t *__restrict__ DX,
+                              int Dp, const float2 *__restrict__ DU, int nu,
+                              const int *__restrict__ DK, uint16_t *__restrict__ X,
+                              float2 *__restrict__ U, int *__restrict__ K) {
+  const int b = blockIdx.x;
+  if (b >= nb) return;
+  const long src = idx[b];
+  for (int e = threadIdx.x; e < Dp; e += blockDim.x) X[(long)b * Dp + e] = DX[src * Dp + e];
+  for (int e = threadIdx.x; e < nu; e += blockDim.x) U[(long)b * nu + e] = DU[src * nu + e];
+  if (threadIdx.x == 0) K[b] = DK[src];
+}
+
+hipError_t launch_gather(const int *idx, int nb, const uint16_t *DX, int Dp, const float2 *DU,
+                         int nu, const int *DK, uint16_t *X, float2 *U, int *K, hipStream_t s) {
+  hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, s, idx, nb, DX, Dp, DU, nu, DK, X, U, K);
+  return hipGetLastError();
+}
+
+}  // namespace lam

@@ -18,13 +18,14 @@ import numpy as np
 from lam_synth import BlockShape
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liblam.so")
+LIB_PATH = os.environ.get("LAM_LIBRARY", os.path.join(_HERE, "liblam.so"))
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "lam.h")
 
 LAM_OK, LAM_ERR_ARG, LAM_ERR_HIP, LAM_ERR_RANGE, LAM_ERR_STATE, LAM_ERR_UNSUPPORTED, LAM_ERR_NO_DEVICE = range(7)
 MODE_CORR_BEAM, MODE_ADAPTIVE = 0, 1
 K_CORRBEAM, K_CORR, K_EIG, K_BEAM2B = range(4)
-KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b"]
+KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp"]
+EIG_PLAIN, EIG_LEARNED, EIG_PREVIOUS = 0, 1, 2
 
 _lib = None
 
@@ -104,6 +105,20 @@ def load_library(path: str = LIB_PATH):
     L.lam_host_free.argtypes = [vp]
     L.lam_eigh_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_float, vp, vp, vp, vp]
+    L.lam_set_eig_method.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+    L.lam_eig_stats.argtypes = [vp, vp, vp]
+    L.lam_set_eig_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.lam_eigh_warm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
+    L.lam_learn_init.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
+    L.lam_learn_dims.argtypes = [vp, vp, vp]
+    L.lam_learn_capture.argtypes = [vp, ctypes.c_int]
+    L.lam_learn_captured.argtypes = [vp]
+    L.lam_learn_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, vp]
+    L.lam_learn_params.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
+    L.lam_learn_grad.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, vp]
+    L.lam_synth_block.argtypes = [ctypes.POINTER(LamShape), ctypes.c_uint64, i32, ctypes.c_float, i32,
+                                  vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -150,6 +165,13 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def memcpy(dst, src, nbytes):
+    """Synchronous copy between any host/device buffers (DeviceArray or address)."""
+    L = load_library()
+    if L.lam_memcpy(_ptr(dst), _ptr(src), nbytes) != LAM_OK:
+        raise LamError(LAM_ERR_HIP, "lam_memcpy failed")
 
 
 def device_count() -> int:
@@ -295,10 +317,69 @@ class Pipeline:
         return self.L.lam_next_stream(self.h)
 
     def kernel_times(self):
-        ms = np.zeros(4, np.float64)
-        n = np.zeros(4, np.int32)
+        k = len(KERNEL_NAMES)
+        ms = np.zeros(k, np.float64)
+        n = np.zeros(k, np.int32)
         self._chk(self.L.lam_kernel_times(self.h, ms.ctypes.data, n.ctypes.data))
-        return {KERNEL_NAMES[i]: (float(ms[i]), int(n[i])) for i in range(4) if n[i]}
+        return {KERNEL_NAMES[i]: (float(ms[i]), int(n[i])) for i in range(k) if n[i]}
+
+    # -- eigen-stage method and the learned warm start
+    def set_eig_method(self, method, refine_sweeps=2, power=False, residual_tol=0.1):
+        self._chk(self.L.lam_set_eig_method(self.h, method, refine_sweeps, int(power), residual_tol))
+
+    def set_eig_tuning(self, plain_threads=256, fallback_threads=1024):
+        self._chk(self.L.lam_set_eig_tuning(self.h, plain_threads, fallback_threads))
+
+    def eig_stats(self):
+        s = self.shape
+        sw = np.zeros((s.C, 2), np.int32)
+        fb = np.zeros((s.C, 2), np.int32)
+        self._chk(self.L.lam_eig_stats(self.h, sw.ctypes.data, fb.ctypes.data))
+        return sw, fb
+
+    def learn_init(self, warm_vectors=4, hidden=256, seed=1):
+        self._chk(self.L.lam_learn_init(self.h, warm_vectors, hidden, seed))
+
+    def learn_dims(self):
+        d = np.zeros(7, np.int32)
+        P = ctypes.c_int64()
+        self._chk(self.L.lam_learn_dims(self.h, d.ctypes.data, ctypes.byref(P)))
+        keys = ["KW", "n", "Dx", "Dp", "H", "Dy", "Dyp"]
+        out = {k: int(v) for k, v in zip(keys, d)}
+        out["P"] = int(P.value)
+        return out
+
+    def learn_capture(self, max_blocks):
+        self._chk(self.L.lam_learn_capture(self.h, max_blocks))
+
+    def learn_captured(self):
+        return int(self.L.lam_learn_captured(self.h))
+
+    def learn_train(self, epochs, batch=768, lr=1e-3, seed=1):
+        loss = np.zeros(max(1, epochs), np.float32)
+        self._chk(self.L.lam_learn_train(self.h, epochs, batch, lr, seed, loss.ctypes.data))
+        return loss[:epochs]
+
+    def learn_params(self, new=None):
+        P = self.learn_dims()["P"]
+        if new is None:
+            out = np.zeros(P, np.float32)
+            self._chk(self.L.lam_learn_params(self.h, out.ctypes.data, P, 0))
+            return out
+        arr = np.ascontiguousarray(new, np.float32)
+        assert arr.size == P
+        self._chk(self.L.lam_learn_params(self.h, arr.ctypes.data, P, 1))
+
+    def learn_grad(self, x_bf16, labels, k):
+        x = np.ascontiguousarray(x_bf16, np.uint16)
+        u = np.ascontiguousarray(labels, np.float32)
+        kk = np.ascontiguousarray(k, np.int32)
+        P = self.learn_dims()["P"]
+        g = np.zeros(P, np.float32)
+        loss = ctypes.c_float()
+        self._chk(self.L.lam_learn_grad(self.h, x.ctypes.data, u.ctypes.data, kk.ctypes.data, len(kk),
+                                        ctypes.byref(loss), g.ctypes.data))
+        return float(loss.value), g
 
     # -- output helpers
     def beams_array(self):
@@ -320,6 +401,27 @@ class Pipeline:
         out = dict(beams=beams[..., 0].astype(np.float32) + 1j * beams[..., 1].astype(np.float32))
         out.update(extra)
         return out
+
+
+def synth_block(shape, seed, block_index, sigma, tones_dev, steer_dev, gain_dev, n_sources,
+                samples_dev, scales_dev, stream=None):
+    """Seeded synthetic ring on the device (lam_synth_block)."""
+    L = load_library()
+    sh = LamShape(shape.C, shape.F, shape.NT, shape.R, shape.P, shape.RPP, shape.NP, shape.B,
+                  shape.R_pad, shape.blocks_to_accumulate)
+    rc = L.lam_synth_block(ctypes.byref(sh), seed, block_index, sigma, n_sources, _ptr(tones_dev),
+                           _ptr(steer_dev), _ptr(gain_dev), _ptr(samples_dev), _ptr(scales_dev), stream)
+    if rc != LAM_OK:
+        raise LamError(rc, "lam_synth_block failed")
+
+
+def eigh_warm(A_dev, prop_dev, n, nmat, warm_vectors, k, steps, tol, evals_dev, evecs_dev,
+              ratio_dev, stream=None):
+    L = load_library()
+    rc = L.lam_eigh_warm(_ptr(A_dev), _ptr(prop_dev), n, nmat, warm_vectors, k, steps, tol,
+                         _ptr(evals_dev), _ptr(evecs_dev), _ptr(ratio_dev), stream)
+    if rc != LAM_OK:
+        raise LamError(rc, "lam_eigh_warm failed")
 
 
 def eigh_batched(A_dev, n, nmat, evals_dev, evecs_dev, sweeps_dev=None, whiten=False,

@@ -187,3 +187,145 @@ def ring_from_fn(shape: BlockShape, sample_fn, scale_fn=None):
                             ring[c, slot, f, t, n, p, 0] = re
                             ring[c, slot, f, t, n, p, 1] = im
     return ring, scales
+
+
+# ---- device generator tables (lam_synth_block) -----------------------------
+
+def tone_tables(steer, amps, nus, phases, pol0, pol1, active):
+    """Pack per-channel tone slots for lam_synth_block.
+
+    steer [C][S][R] complex, amps/nus/phases/pol0 [C][S] real, pol1 [C][S]
+    complex, active [C][S] bool.  Returns (tones float32 [C][S][6],
+    steer float32 [C][S][R][2])."""
+    C, S = amps.shape
+    tones = np.zeros((C, S, 6), np.float32)
+    tones[..., 0] = np.where(active, amps, 0.0)
+    tones[..., 1] = nus
+    tones[..., 2] = phases
+    tones[..., 3] = pol0
+    tones[..., 4] = np.real(pol1)
+    tones[..., 5] = np.imag(pol1)
+    st = np.stack([steer.real, steer.imag], -1).astype(np.float32)
+    return tones, np.ascontiguousarray(st)
+
+
+class DriftScenario:
+    """Interferers whose directions drift and which switch on and off.
+
+    Per channel 1..4 CW sources (n_src), INR drawn in inr_db with
+    A = 16 sqrt(2) 10^(INR/20) (noise sigma 16 per component), azimuth
+    drifting by `drift` rad per block at fixed distance from zenith on the
+    (l, m) disk, and each source toggling on/off with probability p_switch
+    per block.  A per-channel requantisation gain keeps noise plus tones
+    inside +-127 (the INR is unchanged by it)."""
+
+    def __init__(self, C, R, seed, n_blocks, n_src=(1, 4), inr_db=(5.0, 30.0), drift=0.01,
+                 p_switch=0.1, first_channel=64, sigma=16.0, max_sources=4):
+        rng = np.random.default_rng(seed)
+        self.C, self.R, self.S, self.sigma = C, R, max_sources, sigma
+        self.east, self.north = random_layout(rng, R)
+        self.freqs = (first_channel + np.arange(C)) * CHANNEL_BW_HZ
+        self.nsrc = rng.integers(n_src[0], n_src[1] + 1, C)
+        S = max_sources
+        self.exists = np.arange(S)[None, :] < self.nsrc[:, None]
+        self.inr = rng.uniform(inr_db[0], inr_db[1], (C, S))
+        self.amps = 16.0 * np.sqrt(2.0) * 10 ** (self.inr / 20.0)
+        self.nus = rng.uniform(-0.5, 0.5, (C, S))
+        self.phases = rng.uniform(0, 2 * np.pi, (C, S))
+        psi = rng.uniform(np.pi / 8, 3 * np.pi / 8, (C, S))
+        chi = rng.uniform(0, 2 * np.pi, (C, S))
+        self.pol0 = np.cos(psi)
+        self.pol1 = np.sin(psi) * np.exp(1j * chi)
+        self.rho = np.sqrt(rng.uniform(0.0, 0.95, (C, S)))
+        self.az0 = rng.uniform(0, 2 * np.pi, (C, S))
+        self.drift = drift
+        on = np.ones((C, S), bool)
+        self.on = np.zeros((n_blocks, C, S), bool)
+        for b in range(n_blocks):
+            if b > 0:
+                on = on ^ (rng.random((C, S)) < p_switch)
+            self.on[b] = on & self.exists
+
+    def steering(self, b):
+        az = self.az0 + self.drift * b
+        l, m = self.rho * np.cos(az), self.rho * np.sin(az)
+        tau = (l[..., None] * self.east + m[..., None] * self.north) / C_LIGHT
+        return np.exp(-2j * np.pi * self.freqs[:, None, None] * tau)  # [C][S][R]
+
+    def block(self, b):
+        """(tones, steer, gain, steer_complex, active) of block b."""
+        st = self.steering(b)
+        active = self.on[b]
+        tones, st32 = tone_tables(st, self.amps, self.nus, self.phases, self.pol0, self.pol1, active)
+        peak = 3.0 * self.sigma + np.sum(np.where(active, self.amps, 0.0), axis=1)
+        gain = np.minimum(1.0, 120.0 / peak).astype(np.float32)
+        return tones, st32, gain, st, active
+
+
+def _mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xbf58476d1ce4e5b9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94d049bb133111eb)
+    return z ^ (z >> np.uint64(31))
+
+
+def _gauss2(h):
+    u1 = ((h >> np.uint64(40)).astype(np.float32) + np.float32(1)) * np.float32(1 / 16777216)
+    u2 = ((h >> np.uint64(8)) & np.uint64(0xffffff)).astype(np.float32) * np.float32(1 / 16777216)
+    r = np.sqrt(np.float32(-2) * np.log(u1))
+    a = np.float32(2 * np.pi) * u2
+    return r * np.cos(a), r * np.sin(a)
+
+
+def host_synth(shape: BlockShape, seed, block_index, sigma, tones, steer, gain):
+    """numpy restatement of lam_synth_block (test reference; small shapes)."""
+    C, F, NT, RPP, NP = shape.C, shape.F, shape.NT, shape.RPP, shape.NP
+    NS = NP + 2
+    S = tones.shape[1]
+    with np.errstate(over="ignore"):
+        gid = np.arange(C * NS * F * NT, dtype=np.uint64).reshape(C, NS, F, NT)
+        base = (np.uint64(seed) * np.uint64(0x9e3779b97f4a7c15)) ^ (np.uint64(block_index) << np.uint64(44))
+        rr = np.arange(RPP, dtype=np.uint64)
+        ctr = (gid[..., None] * np.uint64(RPP) + rr) * np.uint64(2)  # [C][NS][F][NT][RPP]
+        n0 = _gauss2(_mix64(base + ctr))
+        n1 = _gauss2(_mix64(base + ctr + np.uint64(1)))
+    slot = np.arange(NS)[:, None, None]
+    t = np.arange(NT)[None, None, :]
+    tt = ((slot - 1) * NT + t).astype(np.float64)  # [NS][1][NT]
+    x0 = (sigma * n0[0] + 1j * sigma * n0[1]).astype(np.complex128)
+    x1 = (sigma * n1[0] + 1j * sigma * n1[1]).astype(np.complex128)
+    st = steer[..., 0] + 1j * steer[..., 1]  # [C][S][R]
+    for c in range(C):
+        for s in range(S):
+            amp, nu, ph, g0, g1r, g1i = (float(v) for v in tones[c, s])
+            if amp == 0.0:
+                continue
+            cyc = nu * tt
+            fr = (cyc - np.floor(cyc)).astype(np.float32)
+            ang = (np.float32(6.283185307179586) * fr + np.float32(ph)).astype(np.float32)
+            tone = amp * (np.cos(ang) + 1j * np.sin(ang))  # [NS][1][NT]
+            a = st[c, s].reshape(F, RPP)  # receiver r = f*RPP + rr
+            v = tone[:, :, :, None] * a[None, :, None, :]  # [NS][F][NT][RPP]
+            x0[c] += g0 * v
+            x1[c] += (g1r + 1j * g1i) * v
+    out = np.empty(shape.ring_shape(), np.int8)
+    gb = gain.reshape(C, 1, 1, 1, 1).astype(np.float32)
+    for p, x in enumerate((x0, x1)):
+        xr = x.real.astype(np.float32) * gb
+        xi = x.imag.astype(np.float32) * gb
+        out[..., p, 0] = np.clip(np.rint(xr), -127, 127)
+        out[..., p, 1] = np.clip(np.rint(xi), -127, 127)
+    return out, np.ones(shape.scale_shape(), np.int16)
+
+
+def null_depth_db(W, U, K, steer_active):
+    """10 log10(|W P a|^2 / |W a|^2) summed over beams and active sources,
+    P = I - U_K U_K^H with U the unwhitened unit-norm eigenvectors in
+    ascending order (columns), W [B][R] complex, steer_active [S'][R]."""
+    if K == 0 or len(steer_active) == 0:
+        return 0.0
+    Uk = U[:, -K:]
+    P = np.eye(U.shape[0]) - Uk @ Uk.conj().T
+    a = np.asarray(steer_active).T  # [R][S']
+    num = np.sum(np.abs(W @ P @ a) ** 2)
+    den = np.sum(np.abs(W @ a) ** 2)
+    return 10 * np.log10(max(num, 1e-300) / den)

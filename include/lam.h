@@ -163,13 +163,19 @@ int lam_set_graphs(lam_ctx *ctx, int enable);
 /* Per-kernel timing: when enabled, HIP events bracket every kernel of each
  * block on its stream; lam_kernel_times returns the accumulated milliseconds
  * and launch counts per kernel id (see LAM_K_* below) and resets them. */
-enum { LAM_K_CORRBEAM = 0, LAM_K_CORR = 1, LAM_K_EIG = 2, LAM_K_BEAM2B = 3, LAM_K_COUNT = 4 };
+enum {
+  LAM_K_CORRBEAM = 0, LAM_K_CORR = 1, LAM_K_EIG = 2, LAM_K_BEAM2B = 3,
+  LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_COUNT = 6
+};
 int lam_set_profiling(lam_ctx *ctx, int enable);
 int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);
 
 /* Tunables picked by the autotuner: time samples per workgroup of the
  * voltage kernels (multiple of 64; 0 = automatic). */
 int lam_set_tuning(lam_ctx *ctx, int samples_per_wg);
+/* Workgroup width (256, 512 or 1024 threads) of the plain Jacobi launch and
+ * of the second launch that re-solves the problems a warm start rejected. */
+int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads, int fallback_threads);
 
 /* Pinned host memory for block buffers and output landing areas
  * (cudaHostAlloc in LambdaFinalPacketData, packet_formats.hpp:131-142, and
@@ -187,6 +193,75 @@ int lam_memcpy(void *dst, const void *src, size_t bytes);
 /* Number of visible gfx950 devices (0 when none, e.g. on a build host). */
 int lam_device_count(void);
 
+/* ---- eigen-stage method and learned warm start (adaptive mode) ----
+ * No counterpart in the reference, which always calls cusolverDnXsyevBatched
+ * (lambda_adaptive_beamformed_spectra_pipeline.hpp:679-686).
+ *
+ * LAM_EIG_PLAIN     cyclic Jacobi from the identity (the default).
+ * LAM_EIG_LEARNED   a 2-layer MLP (lam_learn_*) reads the whitened covariance
+ *                   upper triangle and the previous block's top eigenvectors
+ *                   and proposes a warm-start subspace of `warm_vectors`
+ *                   vectors; `refine_sweeps` subspace-iteration steps
+ *                   (Y <- orth(A Y)) and a Rayleigh-Ritz Jacobi solve of the
+ *                   projected warm_vectors x warm_vectors matrix refine it;
+ *                   the result is accepted only if the interference the K
+ *                   removed modes would leak, sum_k |A u_k - l_k u_k|^2 / l_k
+ *                   x (l_k - mu) / l_k, is at most residual_tol x mu (mu: mean
+ *                   of the remaining eigenvalues); otherwise the plain solver
+ *                   runs on that matrix.  Needs fixed K (detection needs the
+ *                   whole spectrum and always runs the plain solver).
+ * LAM_EIG_PREVIOUS  the same refinement with the previous block's vectors as
+ *                   the proposal (no network).
+ * The eigenpairs outside the warm subspace are an orthonormal complement with
+ * its Rayleigh quotients.  `power` is reserved (pass 0).
+ * With LEARNED / PREVIOUS, consecutive blocks are ordered (each block's warm
+ * start depends on its predecessor). */
+enum { LAM_EIG_PLAIN = 0, LAM_EIG_LEARNED = 1, LAM_EIG_PREVIOUS = 2 };
+int lam_set_eig_method(lam_ctx *ctx, int method, int refine_sweeps, int power, float residual_tol);
+/* Per (channel, pol) of the most recent block: Jacobi sweeps run (refinement
+ * plus fallback) and whether the plain fallback ran.  Synchronous. */
+int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback);
+
+/* Allocate and randomly initialise the MLP: warm_vectors (even, 2..8) and
+ * hidden width (multiple of 32).  Called implicitly with (4, 256, 1) by the
+ * other lam_learn_* calls when needed. */
+int lam_learn_init(lam_ctx *ctx, int warm_vectors, int hidden, uint64_t seed);
+/* dims[7] = {warm_vectors, receivers, features, padded features, hidden,
+ * outputs, padded outputs}; params = FP32 master parameter count laid out as
+ * W1[hidden][padded features], b1[hidden], W2[padded outputs][hidden], b2. */
+int lam_learn_dims(lam_ctx *ctx, int32_t *dims, int64_t *params);
+/* Record the features and the plain-Jacobi labels (top warm_vectors whitened
+ * eigenvectors and the mode count K) of the next max_blocks blocks run with
+ * LAM_EIG_PLAIN; lam_learn_captured returns how many blocks are held. */
+int lam_learn_capture(lam_ctx *ctx, int max_blocks);
+int lam_learn_captured(lam_ctx *ctx);
+/* Adam on the captured set with the project's own forward / backward
+ * kernels: loss = mean over samples of sum_{k<K} (1 - |u_k^H y_k|^2/|y_k|^2).
+ * loss_per_epoch (may be NULL) receives `epochs` values.  Synchronous. */
+int lam_learn_train(lam_ctx *ctx, int epochs, int batch, float lr, uint64_t seed,
+                    float *loss_per_epoch);
+/* Read (set = 0) or replace (set = 1) the FP32 master parameters (host array
+ * of the size reported by lam_learn_dims). */
+int lam_learn_params(lam_ctx *ctx, float *params, int64_t count, int set);
+/* Loss and gradient at the current parameters for m host samples:
+ * x [m][padded features] BF16 bits, labels [m][warm_vectors][receivers][2]
+ * FP32, k [m]; grad receives the parameter-shaped gradient.  Test entry. */
+int lam_learn_grad(lam_ctx *ctx, const uint16_t *x, const float *labels, const int32_t *k, int m,
+                   float *loss, float *grad);
+
+/* ---- seeded synthetic voltages (benchmarks and tests) ----
+ * Fills a whole device ring [C][NP+2][F][NT][RPP][2][2] and unit scales:
+ * x = round(gain_c (sigma n + sum_s tone_s(t) steer[c][s][r] pol_s)), clipped
+ * to +-127, with n ~ N(0,1) per real component from a counter hash of
+ * (seed, block_index, sample) and t = (slot - 1) NT + sample-in-packet.
+ * d_tones float[C][n_sources][6] = {amplitude, cycles/sample, phase (rad),
+ * pol-0 gain, pol-1 gain re, im}; d_steer float[C][n_sources][R][2];
+ * d_gain float[C].  n_sources <= 8.  Ordered on `stream`; with stream NULL the
+ * call returns once the block is written. */
+int lam_synth_block(const lam_shape *shape, uint64_t seed, int32_t block_index, float sigma,
+                    int32_t n_sources, const float *d_tones, const float *d_steer,
+                    const float *d_gain, int8_t *d_samples, int16_t *d_scales, void *stream);
+
 /* ---- kernel-level entry points (device pointers, stream = NULL for the
  * default stream); used by unit tests the way the reference tests its
  * kernels directly (tests/test_adaptive_pipeline.cu:743-1129). ---- */
@@ -198,6 +273,15 @@ int lam_device_count(void);
 int lam_eigh_batched(const float *d_A, int n, int nmat, int whiten, int max_sweeps,
                      float tol, float *d_evals, float *d_evecs, int32_t *d_sweeps,
                      void *stream);
+
+/* Warm-start solve (the LAM_EIG_LEARNED / PREVIOUS refinement) of nmat
+ * whitened Hermitian matrices [nmat][n][n] (column-major complex float32)
+ * from proposals [nmat][warm_vectors][n]: `steps` subspace-iteration steps,
+ * Rayleigh-Ritz, complement.  Writes ascending evals [nmat][n], evecs
+ * [nmat][n][n][2] (columns) and ratio [nmat] = leaked interference of the k
+ * largest modes over the mean remaining eigenvalue (accepted when <= tol). */
+int lam_eigh_warm(const float *d_A, const float *d_prop, int n, int nmat, int warm_vectors, int k,
+                  int steps, float tol, float *d_evals, float *d_evecs, float *d_ratio, void *stream);
 
 #ifdef __cplusplus
 }
