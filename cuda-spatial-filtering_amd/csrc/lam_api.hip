@@ -296,8 +296,17 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
         e.prop_stride = L.KW * ctx->R;
       }
     }
-    Prof pr(ctx, s, LAM_K_EIG);
-    HIPC(ctx, launch_eig(e, s));
+    if (eig_is_warm(e)) {
+      {
+        Prof pr(ctx, s, LAM_K_EIG);
+        HIPC(ctx, launch_eig(e, 1, s));
+      }
+      Prof pr(ctx, s, LAM_K_EIGFB);
+      HIPC(ctx, launch_eig(e, 2, s));
+    } else {
+      Prof pr(ctx, s, LAM_K_EIG);
+      HIPC(ctx, launch_eig(e, 0, s));
+    }
   }
   if (capture) {  // labels: this block's plain-Jacobi vectors and mode counts
     const size_t nu = (size_t)L.KW * ctx->R;

@@ -35,9 +35,9 @@ __device__ __forceinline__ int rr_pos(int j, int r, int ne) {
   return j == 0 ? 0 : 1 + v;
 }
 
+__device__ __forceinline__ float wave_sum(float v);
 __device__ __forceinline__ float block_sum(float v, float *red) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v = wave_sum(v);
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[w] = v;
@@ -279,26 +279,39 @@ struct EigSmem {
   int *kdet, *mark;
 };
 
-__device__ __forceinline__ EigSmem carve(char *base, int n, int B) {
+// LDS layout.  The warm-start buffers exist only in the warm kernel; its copy
+// of A shares the space of W and G (loaded after the eigen solve) when that
+// is large enough.
+__device__ __forceinline__ EigSmem carve(char *base, int n, int B, bool warm) {
   EigSmem s;
-  s.A = reinterpret_cast<float2 *>(base);
   const int ld = n | 1;  // odd column stride of A: conflict-free strided LDS access
+  s.A = reinterpret_cast<float2 *>(base);
   s.V = s.A + ld * n;
   s.W = s.V + n * n;
   s.G = s.W + 2 * B * n;
-  s.A0 = s.G + B * n;
-  s.Y = s.A0 + ld * n;
-  s.hv = s.Y + kMaxWarm * n;
-  s.t1 = s.hv + kMaxWarm * n;
-  s.t2 = s.t1 + 64;
-  float *fp = reinterpret_cast<float *>(s.t2 + 64);
+  float2 *q = s.G + B * n;
+  s.A0 = s.Y = s.hv = s.t1 = s.t2 = nullptr;
+  if (warm) {
+    if (3 * B * n >= ld * n) {
+      s.A0 = s.W;
+    } else {
+      s.A0 = q;
+      q += ld * n;
+    }
+    s.Y = q;
+    s.hv = s.Y + kMaxWarm * n;
+    s.t1 = s.hv + kMaxWarm * n;
+    s.t2 = s.t1 + 64;
+    q = s.t2 + 64;
+  }
+  float *fp = reinterpret_cast<float *>(q);
   s.ev = fp;
   s.d = fp + 64;
   s.f = fp + 128;
   s.red = fp + 192;
   s.htau = fp + 208;
-  s.res = fp + 216;  // kMaxWarm * 64
-  s.perm = reinterpret_cast<int *>(s.res + kMaxWarm * 64);
+  s.res = fp + 216;  // kMaxWarm * 64 (warm only)
+  s.perm = reinterpret_cast<int *>(s.res + (warm ? kMaxWarm * 64 : 0));
   s.mark = s.perm + 64;
   s.rpq = reinterpret_cast<int2 *>(s.mark + 64);
   s.rot = reinterpret_cast<Rot *>(s.rpq + 32);
@@ -306,16 +319,30 @@ __device__ __forceinline__ EigSmem carve(char *base, int n, int B) {
   return s;
 }
 
-size_t eig_smem_bytes(int n, int B) {
-  return sizeof(float2) * (2 * (n | 1) * n + n * n + 3 * B * n + 2 * kMaxWarm * n + 128) +
-         sizeof(float) * (216 + kMaxWarm * 64) + sizeof(int) * 128 + sizeof(int2) * 32 +
-         sizeof(Rot) * 32 + 16;
+size_t eig_smem_bytes(int n, int B, bool warm) {
+  const int ld = n | 1;
+  size_t f2 = (size_t)ld * n + (size_t)n * n + 3 * (size_t)B * n;
+  if (warm) f2 += (3 * B * n >= ld * n ? 0 : (size_t)ld * n) + 2 * kMaxWarm * n + 128;
+  return sizeof(float2) * f2 + sizeof(float) * (216 + (warm ? kMaxWarm * 64 : 0)) + sizeof(int) * 128 +
+         sizeof(int2) * 32 + sizeof(Rot) * 32 + 16;
 }
 
+// Sum over the 64 lanes of a wavefront with DPP lane moves (row_shr 1/2/4/8
+// leaves each 16-lane row's total in its lane 15; row_bcast 15 / 31 carry the
+// row totals up to lane 63), broadcast back with readlane.  All lanes active.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                                               ROW_MASK, 0xf, false));
+}
 __device__ __forceinline__ float wave_sum(float v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  v += dpp_move<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_move<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_move<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
 
 // Number of modes to remove: fixed K, or the detection rule of
@@ -660,11 +687,13 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
   return S.kdet[1] != 0;
 }
 
-template <int NTHR>
+// WARM = false: plain Jacobi (and the second launch that re-solves deferred
+// problems); WARM = true: warm-start solve, rejected problems are deferred.
+template <int NTHR, bool WARM>
 __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   extern __shared__ __align__(16) char smem_raw[];
   const int n = P.R, B = P.B;
-  EigSmem S = carve(smem_raw, n, B);
+  EigSmem S = carve(smem_raw, n, B, WARM);
   const int c = blockIdx.x >> 1, p = blockIdx.x & 1;
   const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1;
   // second launch after a deferred fallback: only the problems whose warm solve failed
@@ -703,25 +732,19 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   const long cp = (long)c * 2 + p;
   const int fixed = P.fixed_k[c * 2 + p];
   int sweeps = 0, K = 0, fell = 0;
-  const bool warm = P.proposal != nullptr && !P.only_failed && P.KW > 0 && n >= 2 * P.KW && !P.detect;
-  if (warm) {
+  if constexpr (WARM) {
     const int KW = P.KW;
     for (int e = tid; e < KW * n; e += nthr) S.Y[e] = P.proposal[cp * P.prop_stride + e];
     __syncthreads();
     K = max(0, min(n, fixed));
     const bool ok = warm_solve(S.A, ld, n, KW, K, P.refine_sweeps, P.res_tol, S);
     sweeps = P.refine_sweeps;
-    if (!ok) {  // fall back to the plain solver on the same matrix
-      if (P.defer) {  // leave it to the second launch
-        if (tid == 0) {
-          P.fallback[cp] = 1;
-          if (P.sweeps) P.sweeps[cp] = sweeps;
-        }
-        return;
+    if (!ok) {  // leave it to the second launch (plain solver)
+      if (tid == 0) {
+        P.fallback[cp] = 1;
+        if (P.sweeps) P.sweeps[cp] = sweeps;
       }
-      sweeps += jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
-      sort_eigen(S.A, n, S.perm, S.ev);
-      fell = 1;
+      return;
     }
     if (tid == 0) S.kdet[0] = K;
     __syncthreads();
@@ -852,38 +875,49 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   }
 }
 
-template <int NTHR>
+template <int NTHR, bool WARM>
 hipError_t launch_eig_t(const EigParams &p, hipStream_t s) {
-  const size_t sm = eig_smem_bytes(p.R, p.B);
+  const size_t sm = eig_smem_bytes(p.R, p.B, WARM);
   if (sm > 65536) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eig_null_kernel<NTHR>),
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eig_null_kernel<NTHR, WARM>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(eig_null_kernel<NTHR>, dim3(p.C * 2), dim3(NTHR), sm, s, p);
+  hipLaunchKernelGGL((eig_null_kernel<NTHR, WARM>), dim3(p.C * 2), dim3(NTHR), sm, s, p);
   return hipGetLastError();
 }
 
-hipError_t launch_eig_threads(const EigParams &p, int nthr, hipStream_t s) {
+hipError_t launch_eig_plain(const EigParams &p, int nthr, hipStream_t s) {
   switch (nthr) {
-    case 512: return launch_eig_t<512>(p, s);
-    case 1024: return launch_eig_t<1024>(p, s);
-    default: return launch_eig_t<256>(p, s);
+    case 512: return launch_eig_t<512, false>(p, s);
+    case 1024: return launch_eig_t<1024, false>(p, s);
+    default: return launch_eig_t<256, false>(p, s);
   }
 }
 
-// Plain: one launch.  Warm start: the warm launch defers failed problems to a
-// second launch with wider workgroups that runs the plain solver on them only.
-hipError_t launch_eig(const EigParams &p, hipStream_t s) {
-  if (!p.proposal) return launch_eig_threads(p, p.plain_threads, s);
-  EigParams w = p;
-  w.defer = 1;
-  w.only_failed = 0;
-  if (hipError_t e = launch_eig_t<256>(w, s)) return e;
+// Plain: one launch.  Warm start (fixed K only): the warm launch defers the
+// problems it rejects to a second launch that runs the plain solver on them.
+// part: 0 = everything, 1 = the first launch only, 2 = the second only.
+bool eig_is_warm(const EigParams &p) {
+  return p.proposal && !p.detect && p.KW >= 2 && p.R >= 2 * p.KW;
+}
+
+hipError_t launch_eig(const EigParams &p, int part, hipStream_t s) {
+  if (!eig_is_warm(p)) {
+    EigParams q = p;
+    q.proposal = nullptr;
+    q.only_failed = 0;
+    return part == 2 ? hipSuccess : launch_eig_plain(q, p.plain_threads, s);
+  }
+  if (part != 2) {
+    EigParams w = p;
+    w.only_failed = 0;
+    if (hipError_t e = launch_eig_t<256, true>(w, s)) return e;
+  }
+  if (part == 1) return hipSuccess;
   EigParams f = p;
-  f.defer = 0;
   f.only_failed = 1;
-  return launch_eig_threads(f, p.fallback_threads, s);
+  return launch_eig_plain(f, p.fallback_threads, s);
 }
 
 // ---- standalone batched eigensolver (unit-test entry point) ----
@@ -891,7 +925,7 @@ __global__ void __launch_bounds__(kEigThreads)
 eigh_batched_kernel(const float2 *Ain, int n, int whiten, int max_sweeps, float tol,
                     float *evals, float2 *evecs, int *sweeps_out) {
   extern __shared__ __align__(16) char smem_raw[];
-  EigSmem S = carve(smem_raw, n, 0);
+  EigSmem S = carve(smem_raw, n, 0, false);
   const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
   for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * ld + i % n] = Ain[(long)m * n * n + i];
   __syncthreads();
@@ -923,7 +957,7 @@ __global__ void __launch_bounds__(kEigThreads)
 eigh_warm_kernel(const float2 *Ain, const float2 *prop, int n, int KW, int K, int steps, float tol,
                  float *evals, float2 *evecs, float *ratio) {
   extern __shared__ __align__(16) char smem_raw[];
-  EigSmem S = carve(smem_raw, n, 0);
+  EigSmem S = carve(smem_raw, n, 0, true);
   const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
   for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * ld + i % n] = Ain[(long)m * n * n + i];
   for (int e = tid; e < KW * n; e += blockDim.x) S.Y[e] = prop[(long)m * KW * n + e];
@@ -940,7 +974,7 @@ eigh_warm_kernel(const float2 *Ain, const float2 *prop, int n, int KW, int K, in
 hipError_t launch_eigh_warm(const float *A, const float *prop, int n, int nmat, int KW, int K,
                             int steps, float tol, float *evals, float *evecs, float *ratio,
                             hipStream_t s) {
-  const size_t sm = eig_smem_bytes(n, 0);
+  const size_t sm = eig_smem_bytes(n, 0, true);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eigh_warm_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -955,7 +989,7 @@ hipError_t launch_eigh_warm(const float *A, const float *prop, int n, int nmat, 
 hipError_t launch_eigh_batched(const float *A, int n, int nmat, int whiten, int max_sweeps,
                                float tol, float *evals, float *evecs, int *sweeps,
                                hipStream_t s) {
-  const size_t sm = eig_smem_bytes(n, 0);
+  const size_t sm = eig_smem_bytes(n, 0, false);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eigh_batched_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);

@@ -24,7 +24,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "lam.h")
 LAM_OK, LAM_ERR_ARG, LAM_ERR_HIP, LAM_ERR_RANGE, LAM_ERR_STATE, LAM_ERR_UNSUPPORTED, LAM_ERR_NO_DEVICE = range(7)
 MODE_CORR_BEAM, MODE_ADAPTIVE = 0, 1
 K_CORRBEAM, K_CORR, K_EIG, K_BEAM2B = range(4)
-KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp"]
+KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp", "eig_fallback"]
 EIG_PLAIN, EIG_LEARNED, EIG_PREVIOUS = 0, 1, 2
 
 _lib = None

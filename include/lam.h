@@ -165,7 +165,7 @@ int lam_set_graphs(lam_ctx *ctx, int enable);
  * and launch counts per kernel id (see LAM_K_* below) and resets them. */
 enum {
   LAM_K_CORRBEAM = 0, LAM_K_CORR = 1, LAM_K_EIG = 2, LAM_K_BEAM2B = 3,
-  LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_COUNT = 6
+  LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_EIGFB = 6, LAM_K_COUNT = 7
 };
 int lam_set_profiling(lam_ctx *ctx, int enable);
 int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);
