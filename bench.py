@@ -76,6 +76,108 @@ def cpu_baseline(shape, ring, scales, w, channels):
                 seconds=dt)
 
 
+def run_learned(args):
+    """Learned warm start on 384 ch x 2 pol x 36 ant with drifting, switching
+    interferers: blocks [0, train) run the plain solver and are captured as the
+    training set; the MLP is trained on the GPU; blocks [train, total) run with
+    the plain solver, the previous block's vectors as warm start, and the MLP's
+    proposal, on the same seeded voltages.  Reports the eigen-stage time
+    (features + MLP + warm solve + deferred fallback, or the plain solve), the
+    fallback rate and the null depth against the known steering vectors."""
+    C, KW, K = args.channels, 8, 4
+    shape = syn.BlockShape(C=C, F=1, NT=64, R=36, P=2, RPP=36, NP=128, B=16, blocks_to_accumulate=4)
+    nb, ntr = args.learned_blocks, args.train_blocks
+    sc = syn.DriftScenario(C, 36, seed=20261203 + 4, n_blocks=nb)
+    w16, W = syn.random_weights(shape, seed=20261203 + 5)
+    dr = lam.DeviceArray(shape.ring_shape(), np.int8)
+    ds = lam.DeviceArray(shape.scale_shape(), np.int16)
+    dt = lam.DeviceArray((C, 4, 6), np.float32)
+    dst = lam.DeviceArray((C, 4, 36, 2), np.float32)
+    dg = lam.DeviceArray((C,), np.float32)
+
+    def gen(b):
+        tones, st, gain, stc, act = sc.block(b)
+        dt.upload(tones)
+        dst.upload(st)
+        dg.upload(gain)
+        lam.synth_block(shape, 20261203 + 4, b, 16.0, dt, dst, dg, 4, dr, ds)
+        return stc, act
+
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE, num_buffers=1)
+    p.set_weights(w16)
+    p.set_fixed_k(np.full((C, 2), K, np.int32))
+    p.set_eig_params(args.sweeps, 1e-6)
+    p.set_graphs(False)
+    p.learn_init(KW, 256, seed=1)
+    p.learn_capture(ntr)
+    log(f"learned: capturing {ntr} plain blocks ...")
+    for b in range(ntr):
+        gen(b)
+        p.execute(dr, ds, on_device=True)
+        if (b + 1) % 4 == 0:
+            p.dump_visibilities_i64()
+    p.sync()
+    log(f"learned: training {args.epochs} epochs on {p.learn_captured()} blocks ...")
+    t0 = time.perf_counter()
+    loss = p.learn_train(args.epochs, batch=512, lr=1e-3, seed=7)
+    train_s = time.perf_counter() - t0
+    evecs = np.zeros((C, 2, 36, 36, 2), np.float32)
+    counts = np.zeros((C, 2), np.int32)
+    stage_keys = ("eig_null", "eig_fallback", "features", "mlp")
+    results = {}
+    for name, method in (("plain", lam.EIG_PLAIN), ("previous", lam.EIG_PREVIOUS),
+                         ("learned", lam.EIG_LEARNED)):
+        p.set_eig_method(lam.EIG_PLAIN)
+        gen(ntr - 1)
+        p.execute(dr, ds, on_device=True)  # seeds the warm-start chain
+        p.sync()
+        p.set_eig_method(method, refine_sweeps=args.refine, residual_tol=0.1)
+        p.set_profiling(True)
+        p.kernel_times()
+        depth, fb = [], []
+        for b in range(ntr, nb):
+            stc, act = gen(b)
+            p.execute(dr, ds, evecs_out=evecs, counts_out=counts, on_device=True)
+            p.sync()
+            if method != lam.EIG_PLAIN:
+                fb.append(p.eig_stats()[1].mean())
+            U = evecs[..., 0] + 1j * evecs[..., 1]
+            for c in range(C):
+                a = stc[c][act[c]]
+                for pp in range(2):
+                    depth.append(syn.null_depth_db(W[c, pp], U[c, pp].T, counts[c, pp], a)
+                                 if len(a) else np.nan)
+        kt = p.kernel_times()
+        p.set_profiling(False)
+        n_eval = nb - ntr
+        per = {k: v[0] / v[1] * 1e3 for k, v in kt.items()}
+        stage_us = sum(kt[k][0] for k in stage_keys if k in kt) / n_eval * 1e3
+        gpu_ms = sum(v[0] for v in kt.values()) / n_eval
+        results[name] = dict(eig_stage_us=round(stage_us, 1), gpu_ms_per_block=round(gpu_ms, 4),
+                             kernels_us={k: round(v, 1) for k, v in per.items()},
+                             fallback_rate=float(np.mean(fb)) if fb else 0.0,
+                             depth=np.array(depth))
+        log(f"learned: {name}: eig stage {stage_us:.1f} us")
+    p.close()
+    plain_depth = results["plain"]["depth"]
+    ok = ~np.isnan(plain_depth)
+    out = dict(workload=f"{C} ch x 2 pol x 36 ant, {nb} blocks of {shape.T} samples, 1-4 drifting "
+                        f"CW interferers (INR 5-30 dB, 0.01 rad/block, on/off p=0.1), rank-{K} null; "
+                        f"train blocks 0-{ntr - 1}, evaluate {ntr}-{nb - 1}",
+               warm_vectors=KW, refine_steps=args.refine, epochs=args.epochs,
+               train_seconds=round(train_s, 2), loss_first_last=[float(loss[0]), float(loss[-1])])
+    for name, r in results.items():
+        d = r.pop("depth")
+        r["null_depth_db_mean"] = round(float(np.mean(d[ok])), 3)
+        if name != "plain":
+            diff = d[ok] - plain_depth[ok]
+            r["null_depth_diff_db_mean"] = round(float(np.mean(diff)), 4)
+            r["null_depth_diff_db_abs_p99"] = round(float(np.quantile(np.abs(diff), 0.99)), 4)
+            r["speedup_vs_plain"] = round(results["plain"]["eig_stage_us"] / r["eig_stage_us"], 3)
+        out[name] = r
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +190,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graphs", type=int, default=0,
                     help="time with hipGraph replay instead of per-kernel events")
+    ap.add_argument("--learned-blocks", type=int, default=200)
+    ap.add_argument("--train-blocks", type=int, default=150)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--refine", type=int, default=2, help="warm-start refinement steps")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -163,6 +270,19 @@ def main():
                     "kernel_ms": round(ms, 4), "algorithmic_bytes": per_kernel_bytes[dominant]}
     for k, v in kstats.items():
         v["GBps_algorithmic"] = round(per_kernel_bytes[k] / (v["ms_avg"] * 1e-3) / 1e9, 1)
+    # HBM bytes per launch measured with rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE scaled by a copy probe)
+    tfile = os.path.join(REPO, "profiles", "adaptive_384ch_hbm_traffic.json")
+    if roofline and mode == lam.MODE_ADAPTIVE and shape.C == 384 and os.path.exists(tfile):
+        tr = json.load(open(tfile))["bytes_per_launch"]
+        roofline["traffic"] = tr.get(dominant)
+
+    learned = None
+    if not args.no_learned:
+        learned = run_learned(args)
+        lp = learned["learned"]
+        lp["real_time_factor"] = round((shape.T / CHANNEL_RATE_HZ) / (lp["gpu_ms_per_block"] * 1e-3), 3)
+        learned["plain"]["real_time_factor"] = round(
+            (shape.T / CHANNEL_RATE_HZ) / (learned["plain"]["gpu_ms_per_block"] * 1e-3), 3)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -192,6 +312,7 @@ def main():
         "real_time_factor_ref_convention": (shape.T / REF_RATE_HZ) / sec,
         "roofline": roofline,
         "kernels": kstats,
+        "learned_vs_plain": learned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
