@@ -195,6 +195,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--refine", type=int, default=2, help="warm-start refinement steps")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--spw", type=int, default=0, help="samples per voltage workgroup (0: automatic)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -226,6 +227,7 @@ def main():
         p.set_fixed_k(np.ones((shape.C, 2), np.int32))
         p.set_eig_params(args.sweeps, 1e-6)
     p.set_graphs(bool(args.graphs))
+    p.set_tuning(args.spw)
     p.set_profiling(not args.graphs)
 
     log("warmup ...")

@@ -87,10 +87,13 @@ __device__ __forceinline__ void wait_vmcnt() {
 }
 
 // ------------------------------------------------------------- geometry
-template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT, bool LOADER_WAVE>
+template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT, bool LOADER_WAVE, bool TIGHT = false>
 struct Geo {
   static constexpr int NA = 16 * NG;              // padded antennas
-  static constexpr int RSD = NA + 4;              // LDS row stride (dwords), == 4 mod 16
+  // LDS row stride (dwords), == 4 mod 16.  TIGHT: R = NA - 12 antennas, rows
+  // packed exactly as in the ring (reads of the padding columns run into the
+  // next row: finite samples that only meet discarded outputs or zero weights)
+  static constexpr int RSD = TIGHT ? NA - 12 : NA + 4;
   static constexpr int NK = 2 * NG - KODD;        // beam K steps of 8 antennas
   static constexpr int NPAIRS = NG * (NG + 1) / 2;
   static constexpr int NWC = DO_CORR ? NPAIRS : 0;        // correlator waves
@@ -250,14 +253,21 @@ __device__ __forceinline__ void wg_coords(const VoltParams &P, int &c, int &spli
 }
 
 // ================================================================ fast path
-template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT>
-__global__ void __launch_bounds__((Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, true>::NT))
+#ifndef LAM_VOLT_STAGES
+#define LAM_VOLT_STAGES 3
+#endif
+template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT, bool TIGHT>
+__global__ void __launch_bounds__((Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, true, TIGHT>::NT))
 volt_fast_kernel(VoltParams P) {
-  using G = Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, true>;
-  constexpr int RSD = G::RSD, NK = G::NK, STAGE = G::STAGE, NS = 3;
+  using G = Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, true, TIGHT>;
+  constexpr int RSD = G::RSD, NK = G::NK, STAGE = G::STAGE;
   constexpr int NI = STAGE / 256;  // dwordx4 wave loads per chunk
+  // LDS ring depth: up to LAM_VOLT_STAGES, as long as the wait count stays encodable
+  constexpr int NS = (LAM_VOLT_STAGES >= 4 && 2 * NI < 64) ? 4 : 3;
   static_assert(NI < 64, "loads of one chunk must fit the vmcnt counter");
-  __shared__ __align__(16) uint32_t lds[NS * STAGE];
+  static_assert(NI * (NS - 2) < 64, "the wait count for the oldest chunk must fit vmcnt");
+  // slack past the last stage for the padding-column reads of a TIGHT layout
+  __shared__ __align__(16) uint32_t lds[NS * STAGE + (TIGHT ? 16 : 0)];
   __shared__ int s_flags;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -316,15 +326,22 @@ volt_fast_kernel(VoltParams P) {
   }
 
   if (is_loader) {
-    issue(0);
-    if (nchunks > 1) issue(1);
+#pragma unroll
+    for (int k = 0; k < NS - 1; ++k)
+      if (k < nchunks) issue(k);
   }
   for (int ch = 0; ch < nchunks; ++ch) {
     if (is_loader) {
-      if (ch + 1 < nchunks)
-        wait_vmcnt<NI>();
-      else
-        wait_vmcnt<0>();
+      // chunk ch is complete when at most min(NS - 2, chunks issued after it) chunk loads remain
+      const int after = min(NS - 2, nchunks - 1 - ch);
+      if constexpr (NS >= 4) {
+        if (after >= 2) wait_vmcnt<2 * NI>();
+        else if (after == 1) wait_vmcnt<NI>();
+        else wait_vmcnt<0>();
+      } else {
+        if (after >= 1) wait_vmcnt<NI>();
+        else wait_vmcnt<0>();
+      }
     }
     uint32_t *L = lds + (ch % NS) * STAGE;
     const int s0 = s_begin + ch * kChunk;
@@ -344,7 +361,7 @@ volt_fast_kernel(VoltParams P) {
       }
     }
     __syncthreads();
-    if (is_loader && ch + 2 < nchunks) issue(ch + 2);
+    if (is_loader && ch + NS - 1 < nchunks) issue(ch + NS - 1);
     if constexpr (DO_CORR) {
       if (is_corr) corr_chunk<RSD>(L, gA, gB, lane, acc);
     }
@@ -512,9 +529,12 @@ template <int NG, int KODD, bool C_, bool B_, int NMT>
 static hipError_t launch_t(const VoltParams &p, hipStream_t s) {
   dim3 grid(p.C * p.nsplit);
   const bool fast = p.F == 1 && (p.R % 4) == 0 && p.recv_perm == nullptr;
-  if (fast) {
+  if (fast && p.R == 16 * NG - 12) {
+    using G = Geo<NG, KODD, C_, B_, NMT, true, true>;
+    hipLaunchKernelGGL((volt_fast_kernel<NG, KODD, C_, B_, NMT, true>), grid, dim3(G::NT), 0, s, p);
+  } else if (fast) {
     using G = Geo<NG, KODD, C_, B_, NMT, true>;
-    hipLaunchKernelGGL((volt_fast_kernel<NG, KODD, C_, B_, NMT>), grid, dim3(G::NT), 0, s, p);
+    hipLaunchKernelGGL((volt_fast_kernel<NG, KODD, C_, B_, NMT, false>), grid, dim3(G::NT), 0, s, p);
   } else {
     using G = Geo<NG, KODD, C_, B_, NMT, false>;
     if (p.recv_perm)
