@@ -124,14 +124,30 @@ def run_learned(args):
     evecs = np.zeros((C, 2, 36, 36, 2), np.float32)
     counts = np.zeros((C, 2), np.int32)
     stage_keys = ("eig_null", "eig_fallback", "features", "mlp")
+    # autotuner inputs: the last training-period blocks (never the evaluated ones)
+    tune = []
+    if not args.no_autotune:
+        for b in range(max(0, ntr - 8), ntr):
+            gen(b)
+            r2 = lam.DeviceArray(shape.ring_shape(), np.int8)
+            s2 = lam.DeviceArray(shape.scale_shape(), np.int16)
+            lam.memcpy(r2, dr, dr.nbytes)
+            lam.memcpy(s2, ds, ds.nbytes)
+            tune.append((r2, s2))
     results = {}
     for name, method in (("plain", lam.EIG_PLAIN), ("previous", lam.EIG_PREVIOUS),
                          ("learned", lam.EIG_LEARNED)):
+        steps, tuned = args.refine, None
+        if tune:
+            p.set_eig_method(method, refine_sweeps=args.refine, residual_tol=0.1)
+            tuned = p.autotune([t[0] for t in tune], [t[1] for t in tune], iters=2)
+            steps = tuned["refine_steps"]
+            log(f"learned: {name}: autotune {tuned}")
         p.set_eig_method(lam.EIG_PLAIN)
         gen(ntr - 1)
         p.execute(dr, ds, on_device=True)  # seeds the warm-start chain
         p.sync()
-        p.set_eig_method(method, refine_sweeps=args.refine, residual_tol=0.1)
+        p.set_eig_method(method, refine_sweeps=steps, residual_tol=0.1)
         p.set_profiling(True)
         p.kernel_times()
         depth, fb = [], []
@@ -154,17 +170,21 @@ def run_learned(args):
         stage_us = sum(kt[k][0] for k in stage_keys if k in kt) / n_eval * 1e3
         gpu_ms = sum(v[0] for v in kt.values()) / n_eval
         results[name] = dict(eig_stage_us=round(stage_us, 1), gpu_ms_per_block=round(gpu_ms, 4),
+                             autotune=tuned,
                              kernels_us={k: round(v, 1) for k, v in per.items()},
                              fallback_rate=float(np.mean(fb)) if fb else 0.0,
                              depth=np.array(depth))
         log(f"learned: {name}: eig stage {stage_us:.1f} us")
     p.close()
+    for r2, s2 in tune:
+        r2.free()
+        s2.free()
     plain_depth = results["plain"]["depth"]
     ok = ~np.isnan(plain_depth)
     out = dict(workload=f"{C} ch x 2 pol x 36 ant, {nb} blocks of {shape.T} samples, 1-4 drifting "
                         f"CW interferers (INR 5-30 dB, 0.01 rad/block, on/off p=0.1), rank-{K} null; "
                         f"train blocks 0-{ntr - 1}, evaluate {ntr}-{nb - 1}",
-               warm_vectors=KW, refine_steps=args.refine, epochs=args.epochs,
+               warm_vectors=KW, epochs=args.epochs,
                train_seconds=round(train_s, 2), loss_first_last=[float(loss[0]), float(loss[-1])])
     for name, r in results.items():
         d = r.pop("depth")
@@ -196,6 +216,7 @@ def main():
     ap.add_argument("--refine", type=int, default=2, help="warm-start refinement steps")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--spw", type=int, default=0, help="samples per voltage workgroup (0: automatic)")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run device-to-device copies of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -228,6 +249,11 @@ def main():
         p.set_eig_params(args.sweeps, 1e-6)
     p.set_graphs(bool(args.graphs))
     p.set_tuning(args.spw)
+    tuned = None
+    if not args.no_autotune and not args.spw:
+        log("autotune ...")
+        tuned = p.autotune([d_ring], [d_scales], iters=3)
+        log(f"autotune: {tuned}")
     p.set_profiling(not args.graphs)
 
     log("warmup ...")
@@ -314,6 +340,7 @@ def main():
         "real_time_factor_ref_convention": (shape.T / REF_RATE_HZ) / sec,
         "roofline": roofline,
         "kernels": kstats,
+        "autotune": tuned,
         "learned_vs_plain": learned,
         "cpu_baseline": cpu,
     }

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <numeric>
 #include <random>
 #include <string>
@@ -506,6 +507,46 @@ int forward_backward(lam_ctx *ctx, int nb, hipStream_t s) {
   return LAM_OK;
 }
 
+
+// Drain the profiling events into per-kernel sums (ms) and launch counts.
+int drain_prof(lam_ctx *ctx, double *ms, int *n) {
+  if (int r = sync_all(ctx)) return r;
+  for (auto &e : ctx->prof) {
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, e.start, e.stop) == hipSuccess) {
+      ms[e.kernel] += t;
+      n[e.kernel] += 1;
+    }
+    hipEventDestroy(e.start);
+    hipEventDestroy(e.stop);
+  }
+  ctx->prof.clear();
+  return LAM_OK;
+}
+
+// Average milliseconds per block of the kernels in `ids` over `iters` passes of
+// the block sequence (after one untimed pass).
+int time_sequence(lam_ctx *ctx, const void *const *smp, const void *const *scl, int nblocks, int iters,
+                  std::initializer_list<int> ids, double &out) {
+  double ms[LAM_K_COUNT] = {0};
+  int n[LAM_K_COUNT] = {0};
+  Buf &b = ctx->bufs[0];
+  for (int pass = 0; pass <= iters; ++pass) {
+    for (int i = 0; i < nblocks; ++i)
+      if (int r = enqueue_kernels(ctx, b, smp[i], scl[i])) return r;
+    if (pass == 0) {  // discard the untimed pass
+      double d0[LAM_K_COUNT] = {0};
+      int d1[LAM_K_COUNT] = {0};
+      if (int r = drain_prof(ctx, d0, d1)) return r;
+    }
+  }
+  if (int r = drain_prof(ctx, ms, n)) return r;
+  double t = 0.0;
+  for (int k : ids) t += ms[k];
+  out = t / ((double)iters * nblocks);
+  return LAM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -934,6 +975,95 @@ void *lam_host_alloc(size_t bytes) {
 
 void lam_host_free(void *p) {
   if (p) (void)hipHostFree(p);
+}
+
+int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *scales, int nblocks, int iters,
+                 int32_t *result) {
+  if (!ctx || !samples || !scales || nblocks < 1 || iters < 1) return LAM_ERR_ARG;
+  for (int i = 0; i < nblocks; ++i)
+    if (!samples[i] || !scales[i]) return LAM_ERR_ARG;
+  if (int r = sync_all(ctx)) return r;
+  const bool prof_was = ctx->profiling;
+  double d0[LAM_K_COUNT] = {0};
+  int d1[LAM_K_COUNT] = {0};
+  if (int r = drain_prof(ctx, d0, d1)) return r;  // keep the caller's pending timings out
+  for (int k = 0; k < LAM_K_COUNT; ++k) {
+    ctx->prof_ms[k] += d0[k];
+    ctx->prof_n[k] += d1[k];
+  }
+  ctx->profiling = true;
+  int rc = LAM_OK;
+  // 1. time samples per voltage workgroup
+  {
+    const int Tr = (ctx->T + kChunk - 1) / kChunk * kChunk;
+    int best = ctx->samples_per_wg;
+    double best_t = 1e30;
+    for (int cand : {0, 1024, 2048, 3072, 4096, 8192}) {
+      if (cand > Tr) continue;
+      ctx->samples_per_wg = cand;
+      double t = 0.0;
+      if ((rc = time_sequence(ctx, samples, scales, nblocks, iters,
+                              {LAM_K_CORRBEAM, LAM_K_CORR, LAM_K_BEAM2B}, t)))
+        break;
+      if (t < best_t) {
+        best_t = t;
+        best = cand;
+      }
+    }
+    ctx->samples_per_wg = best;
+  }
+  // 2. plain eigen solve: workgroup width
+  if (rc == LAM_OK && ctx->mode == LAM_MODE_ADAPTIVE) {
+    const int method = ctx->eig_method;
+    ctx->eig_method = LAM_EIG_PLAIN;
+    int best = ctx->eig_plain_threads;
+    double best_t = 1e30;
+    for (int th : {256, 512, 1024}) {
+      ctx->eig_plain_threads = th;
+      double t = 0.0;
+      if ((rc = time_sequence(ctx, samples, scales, nblocks, iters, {LAM_K_EIG}, t))) break;
+      if (t < best_t) {
+        best_t = t;
+        best = th;
+      }
+    }
+    ctx->eig_plain_threads = best;
+    ctx->eig_method = method;
+    // 3. warm start: refinement steps (rejections fall back to the plain solver and
+    //    show up in the time of the second launch)
+    if (rc == LAM_OK && method != LAM_EIG_PLAIN && ctx->lr.ready) {
+      int bst = ctx->refine_sweeps;
+      double bt = 1e30;
+      for (int st : {1, 2, 3}) {
+        ctx->refine_sweeps = st;
+        double t = 0.0;
+        if ((rc = time_sequence(ctx, samples, scales, nblocks, iters,
+                                {LAM_K_EIG, LAM_K_EIGFB, LAM_K_FEAT, LAM_K_MLP}, t)))
+          break;
+        if (t < bt) {
+          bt = t;
+          bst = st;
+        }
+      }
+      ctx->refine_sweeps = bst;
+    }
+  }
+  ctx->profiling = prof_was;
+  ctx->graphs_dirty = true;
+  // the tuning blocks must not reach the visibility accumulator
+  if (rc == LAM_OK) {
+    hipStream_t st = ctx->bufs[0].stream;
+    if (launch_zero(ctx->d_vis_acc, (size_t)ctx->C * ctx->NBL * 8 * sizeof(long long), st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = fail(ctx, LAM_ERR_HIP, "autotune: clearing the accumulator failed");
+  }
+  if (result) {
+    result[0] = ctx->samples_per_wg;
+    result[1] = ctx->eig_plain_threads;
+    result[2] = ctx->refine_sweeps;
+    result[3] = ctx->eig_fallback_threads;
+  }
+  return rc;
 }
 
 int lam_set_eig_method(lam_ctx *ctx, int method, int refine_sweeps, int power, float residual_tol) {

@@ -108,6 +108,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_set_eig_method.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float]
     L.lam_eig_stats.argtypes = [vp, vp, vp]
     L.lam_set_eig_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.lam_autotune.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     L.lam_eigh_warm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
     L.lam_learn_init.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]
@@ -326,6 +327,16 @@ class Pipeline:
     # -- eigen-stage method and the learned warm start
     def set_eig_method(self, method, refine_sweeps=2, power=False, residual_tol=0.1):
         self._chk(self.L.lam_set_eig_method(self.h, method, refine_sweeps, int(power), residual_tol))
+
+    def autotune(self, samples_list, scales_list, iters=3):
+        """Tune on device-resident blocks; returns the chosen settings."""
+        n = len(samples_list)
+        sp = (ctypes.c_void_p * n)(*[_ptr(x) for x in samples_list])
+        sc = (ctypes.c_void_p * n)(*[_ptr(x) for x in scales_list])
+        res = np.zeros(4, np.int32)
+        self._chk(self.L.lam_autotune(self.h, sp, sc, n, iters, res.ctypes.data))
+        return dict(samples_per_wg=int(res[0]), plain_threads=int(res[1]), refine_steps=int(res[2]),
+                    fallback_threads=int(res[3]))
 
     def set_eig_tuning(self, plain_threads=256, fallback_threads=1024):
         self._chk(self.L.lam_set_eig_tuning(self.h, plain_threads, fallback_threads))

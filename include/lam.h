@@ -177,6 +177,19 @@ int lam_set_tuning(lam_ctx *ctx, int samples_per_wg);
  * of the second launch that re-solves the problems a warm start rejected. */
 int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads, int fallback_threads);
 
+/* Autotuner (no counterpart in the reference): times this context's kernels
+ * on a sequence of nblocks device-resident blocks (samples[i], scales[i]),
+ * `iters` timed passes after one untimed pass, and keeps the fastest
+ *   - time samples per voltage workgroup: automatic, 1024, 2048, 3072, 4096, 8192;
+ *   - plain Jacobi workgroup width: 256, 512, 1024 (adaptive mode);
+ *   - warm-start refinement steps 1..3 when LAM_EIG_LEARNED / PREVIOUS is set
+ *     (rejected problems fall back to the plain solver inside the timing, so a
+ *     too-short refinement loses on time).
+ * The blocks do not reach the visibility accumulator.  result[4] (may be NULL)
+ * = {samples_per_wg, plain_threads, refine_steps, fallback_threads}. */
+int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *scales, int nblocks,
+                 int iters, int32_t *result);
+
 /* Pinned host memory for block buffers and output landing areas
  * (cudaHostAlloc in LambdaFinalPacketData, packet_formats.hpp:131-142, and
  * SingleHostMemoryOutput, output.hpp:146-154).  Returns NULL on failure. */

@@ -379,3 +379,50 @@ def test_adaptive_delays_and_permutation():
     ref = orc.beamform(s, y, w)
     assert rel_err(out["beams"][:, :, :2], ref) < BEAM_TOL
     assert rel_err(out["beams"][:, :, 2:], ref) < BEAM_TOL
+
+
+def test_autotune_leaves_results_exact():
+    """The autotuner times launch shapes on the caller's blocks; afterwards the
+    pipeline's visibilities hold only the blocks executed after it (bit-exact)
+    and the beams stay within tolerance.  36 receivers with a partial last
+    chunk and scales 0/1/2 exercise the packed-row loader."""
+    shape = syn.BlockShape(C=6, F=1, NT=64, R=36, P=2, RPP=36, NP=5, B=4)
+    rng = np.random.default_rng(11)
+    ring = rng.integers(-60, 61, shape.ring_shape(), dtype=np.int16).astype(np.int8)
+    scales = rng.choice(np.array([0, 1, 1, 2], np.int16), shape.scale_shape())
+    w16, w = syn.random_weights(shape, seed=4)
+    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM)
+    p.set_weights(w16)
+    d_ring = lam.DeviceArray.from_numpy(ring)
+    d_scales = lam.DeviceArray.from_numpy(scales)
+    res = p.autotune([d_ring, d_ring], [d_scales, d_scales], iters=1)
+    _log(f"autotune {res}")
+    assert res["samples_per_wg"] in (0, 1024, 2048, 3072, 4096, 8192)
+    out = p.run_host(ring, scales)
+    vis64 = p.dump_visibilities_i64()
+    p.close()
+    s = _o(shape)
+    y = orc.ingest(s, ring, scales, orc.WINDOW_CORR_BEAM)
+    assert np.array_equal(vis64, orc.correlate(s, y))
+    assert rel_err(out["beams"], orc.beamform(s, y, w)) < BEAM_TOL
+
+
+def test_autotune_adaptive_warm_start():
+    shape = syn.BlockShape(C=4, F=1, NT=64, R=36, P=2, RPP=36, NP=8, B=4)
+    scene = syn.make_scene(shape, seed=21, n_tones=2, amp=30.0)
+    blocks = [syn.make_block(shape, scene, seed=22 + i) for i in range(2)]
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+    p.set_weights(syn.random_weights(shape, seed=5)[0])
+    p.set_fixed_k(np.full((shape.C, 2), 2, np.int32))
+    p.learn_init(8, 256, seed=2)
+    p.set_eig_method(lam.EIG_PREVIOUS, refine_sweeps=2)
+    dev = [(lam.DeviceArray.from_numpy(r), lam.DeviceArray.from_numpy(s)) for r, s in blocks]
+    res = p.autotune([d[0] for d in dev], [d[1] for d in dev], iters=1)
+    _log(f"autotune {res}")
+    assert res["plain_threads"] in (256, 512, 1024) and res["refine_steps"] in (1, 2, 3)
+    ring, scales = blocks[0]
+    p.run_host(ring, scales)
+    vis64 = p.dump_visibilities_i64()
+    p.close()
+    s = _o(shape)
+    assert np.array_equal(vis64, orc.correlate(s, orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)))
