@@ -385,9 +385,10 @@ __device__ __forceinline__ void apply_cols(const float2 *A, int ld, const float2
   }
 }
 
-// Gram-Schmidt with reorthogonalisation ("twice is enough") on KW vectors
-// Y[k][0..n) by wave 0 (lane = row, n <= 64).  A vector that vanishes is
-// replaced by a unit vector before orthogonalising.
+// Classical Gram-Schmidt applied twice ("twice is enough") on KW vectors
+// Y[k][0..n) by wave 0 (lane = row, n <= 64): the projections of vector k on
+// all earlier vectors are independent wave sums, issued together.  A vector
+// that vanishes is replaced by a unit vector before orthogonalising.
 __device__ __forceinline__ void mgs_wave(float2 *Y, int n, int KW) {
   const int i = threadIdx.x;
   float2 y[kMaxWarm];
@@ -395,29 +396,33 @@ __device__ __forceinline__ void mgs_wave(float2 *Y, int n, int KW) {
   for (int k = 0; k < kMaxWarm; ++k) y[k] = (k < KW && i < n) ? Y[k * n + i] : make_float2(0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < kMaxWarm; ++k) {
-    if (k >= KW) break;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      const float n0 = wave_sum(y[k].x * y[k].x + y[k].y * y[k].y);
+    if (k < KW) {
+      for (int attempt = 0; attempt < 2; ++attempt) {
+        const float n0 = wave_sum(y[k].x * y[k].x + y[k].y * y[k].y);
 #pragma unroll
-      for (int pass = 0; pass < 2; ++pass) {
+        for (int pass = 0; pass < 2; ++pass) {
+          float gr[kMaxWarm], gi[kMaxWarm];
 #pragma unroll
-        for (int j = 0; j < kMaxWarm; ++j) {
-          if (j >= k) break;
-          const float gr = wave_sum(y[j].x * y[k].x + y[j].y * y[k].y);  // conj(y_j) . y_k
-          const float gi = wave_sum(y[j].x * y[k].y - y[j].y * y[k].x);
-          y[k].x -= gr * y[j].x - gi * y[j].y;
-          y[k].y -= gr * y[j].y + gi * y[j].x;
+          for (int j = 0; j < k; ++j) {  // conj(y_j) . y_k
+            gr[j] = wave_sum(y[j].x * y[k].x + y[j].y * y[k].y);
+            gi[j] = wave_sum(y[j].x * y[k].y - y[j].y * y[k].x);
+          }
+#pragma unroll
+          for (int j = 0; j < k; ++j) {
+            y[k].x -= gr[j] * y[j].x - gi[j] * y[j].y;
+            y[k].y -= gr[j] * y[j].y + gi[j] * y[j].x;
+          }
         }
+        const float nrm2 = wave_sum(y[k].x * y[k].x + y[k].y * y[k].y);
+        if (nrm2 > 1e-12f * n0 && nrm2 > 1e-30f) {
+          const float inv = rsqrtf(nrm2);
+          y[k].x *= inv;
+          y[k].y *= inv;
+          break;
+        }
+        // (nearly) dependent: restart from a unit vector
+        y[k] = make_float2(i == (k * 5 + 3 + attempt * 7) % n ? 1.0f : 0.0f, 0.0f);
       }
-      const float nrm2 = wave_sum(y[k].x * y[k].x + y[k].y * y[k].y);
-      if (nrm2 > 1e-12f * n0 && nrm2 > 1e-30f) {
-        const float inv = rsqrtf(nrm2);
-        y[k].x *= inv;
-        y[k].y *= inv;
-        break;
-      }
-      // (nearly) dependent: restart from a unit vector
-      y[k] = make_float2(i == (k * 5 + 3 + attempt * 7) % n ? 1.0f : 0.0f, 0.0f);
     }
   }
 #pragma unroll
@@ -562,13 +567,20 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
     S.res[k * 64 + i] = z.x * z.x + z.y * z.y;
   }
   __syncthreads();
-  // complement: Householder QR of U (wave 0), then V[:, 0..m) = (H_0 ... H_{KW-1})[:, KW..n)
+  // complement: Householder QR of U (wave 0) and the compact form
+  // H_0 ... H_{KW-1} = I - Vh T Vh^H (T upper triangular, built in the same pass);
+  // V[:, 0..m) = (I - Vh T Vh^H)[:, KW..n)
+  float2 *Tm = S.t2;  // KW x KW, row-major (Vb is no longer needed)
   if (tid < 64) {
     const int i = tid;
     for (int k = 0; k < KW; ++k)
       if (i < n) Y[k * n + i] = S.V[(m + k) * n + i];
+    for (int e = i; e < KW * KW; e += 64) Tm[e] = make_float2(0.f, 0.f);
     wave_sync();
-    for (int k = 0; k < KW; ++k) {
+    float2 vk[kMaxWarm];
+#pragma unroll
+    for (int k = 0; k < kMaxWarm; ++k) {
+      if (k >= KW) break;
       const float2 x = (i < n && i >= k) ? Y[k * n + i] : make_float2(0.f, 0.f);
       const float nrm2 = wave_sum(x.x * x.x + x.y * x.y);
       const float xkr = __shfl(x.x, k, 64), xki = __shfl(x.y, k, 64);
@@ -582,46 +594,65 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
       }
       const float vn2 = wave_sum(v.x * v.x + v.y * v.y);
       const float tau = (nrm2 > 1e-30f && vn2 > 1e-30f) ? 2.0f / vn2 : 0.0f;
+      vk[k] = v;
       if (i < n) Z[k * n + i] = v;  // reflector k
-      if (i == 0) S.htau[k] = tau;
-      for (int j = k + 1; j < KW; ++j) {
+      // T(0:k, k) = -tau T(0:k, 0:k) (Vh(:, 0:k)^H v_k), T(k, k) = tau
+      float zr[kMaxWarm], zi[kMaxWarm];
+#pragma unroll
+      for (int j = 0; j < k; ++j) {
+        zr[j] = wave_sum(vk[j].x * v.x + vk[j].y * v.y);
+        zi[j] = wave_sum(vk[j].x * v.y - vk[j].y * v.x);
+      }
+      if (i < k) {
+        float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < k; ++j) {
+          const float2 t = cmul(Tm[i * KW + j], make_float2(zr[j], zi[j]));
+          acc.x += t.x;
+          acc.y += t.y;
+        }
+        Tm[i * KW + k] = make_float2(-tau * acc.x, -tau * acc.y);
+      }
+      if (i == k) Tm[k * KW + k] = make_float2(tau, 0.f);
+      wave_sync();
+      // apply H_k to the remaining columns of U
+#pragma unroll
+      for (int j = k + 1; j < kMaxWarm; ++j) {
+        if (j >= KW) break;
         float2 y = i < n ? Y[j * n + i] : make_float2(0.f, 0.f);
         const float gr = wave_sum(v.x * y.x + v.y * y.y), gi = wave_sum(v.x * y.y - v.y * y.x);
         y.x -= tau * (gr * v.x - gi * v.y);
         y.y -= tau * (gr * v.y + gi * v.x);
         if (i < n) Y[j * n + i] = y;
       }
+      wave_sync();
     }
   }
   __syncthreads();
-  // Q columns KW..n-1 = H_0 ... H_{KW-1} e_c: apply the reflectors in reverse order
+  // Sm(a, c) = sum_b T(a, b) conj(Vh(KW + c, b)), then Q2(i, c) = delta - sum_a Vh(i, a) Sm(a, c)
+  float2 *Sm = Y;  // KW x m, column c at Sm + c * KW
+  for (int e = tid; e < KW * m; e += nthr) {
+    const int c2 = e / KW, a2 = e - c2 * KW;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int b2 = a2; b2 < KW; ++b2) {
+      const float2 t = cmulc(Tm[a2 * KW + b2], Z[b2 * n + KW + c2]);
+      acc.x += t.x;
+      acc.y += t.y;
+    }
+    Sm[c2 * KW + a2] = acc;
+  }
+  __syncthreads();
   for (int e = tid; e < m * n; e += nthr) {
-    const int j = e / n, i = e - j * n;
-    S.A0[e] = make_float2(i == KW + j ? 1.0f : 0.0f, 0.0f);
+    const int c2 = e / n, i = e - c2 * n;
+    float2 q = make_float2(i == KW + c2 ? 1.0f : 0.0f, 0.0f);
+    for (int a2 = 0; a2 < KW; ++a2) {
+      const float2 t = cmul(Z[a2 * n + i], Sm[c2 * KW + a2]);
+      q.x -= t.x;
+      q.y -= t.y;
+    }
+    S.A0[e] = q;
   }
   __syncthreads();
-  for (int k = KW - 1; k >= 0; --k) {
-    const float tau = S.htau[k];
-    if (tau == 0.0f) continue;
-    const float2 *v = Z + k * n;
-    for (int j = tid; j < m; j += nthr) {  // g_j = v^H q_j
-      float2 g = make_float2(0.f, 0.f);
-      for (int i = 0; i < n; ++i) {
-        const float2 t = cmulc(S.A0[j * n + i], v[i]);
-        g.x += t.x;
-        g.y += t.y;
-      }
-      S.t2[j] = make_float2(tau * g.x, tau * g.y);  // Vb is no longer needed
-    }
-    __syncthreads();
-    for (int e = tid; e < m * n; e += nthr) {  // q_j -= tau (v^H q_j) v
-      const int j = e / n, i = e - j * n;
-      const float2 t = cmul(v[i], S.t2[j]);
-      S.A0[e].x -= t.x;
-      S.A0[e].y -= t.y;
-    }
-    __syncthreads();
-  }
   for (int e = tid; e < m * n; e += nthr) S.V[e] = S.A0[e];
   __syncthreads();
   // Rayleigh quotients of the complement: lambda_j = q_j^H A q_j

@@ -73,13 +73,14 @@ hipError_t launch_features(const FeatParams &p, hipStream_t s) {
 // ---- dense layer on MFMA: Z[M][N] = X[M][K] Wt[N][K]^T + b[N] (+ skip) ----
 // The output layer adds a skip connection from the previous block's vectors
 // held in the features: Z[m][j] += scale * X'[m][off + j] for j < len.
-// One workgroup per 32x32 output tile; its 4 waves split K in 16-wide steps
+// One workgroup per 32x32 output tile; its NW waves split K in 16-wide steps
 // and reduce through LDS.  Output: FP32 Z and/or BF16 relu(Z).
-__global__ void __launch_bounds__(256)
+template <int NW>
+__global__ void __launch_bounds__(64 * NW)
 dense_fwd_kernel(const uint16_t *__restrict__ X, int M, int K, const uint16_t *__restrict__ Wt,
                  const float *__restrict__ b, int N, float *__restrict__ Z,
                  uint16_t *__restrict__ Hbf, Skip sk) {
-  __shared__ float red[4][16][64];
+  __shared__ float red[NW][16][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
   const int row = lane & 31, kh = lane >> 5;
@@ -90,12 +91,12 @@ dense_fwd_kernel(const uint16_t *__restrict__ X, int M, int K, const uint16_t *_
   v16f acc = {};
   const int nks = K / 16;
   int ks = w;
-  for (; ks + 12 < nks; ks += 16) {
+  for (; ks + 3 * NW < nks; ks += 4 * NW) {
     v8bf a[4], bb[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      a[u] = *reinterpret_cast<const v8bf *>(xa + (ks + 4 * u) * 16);
-      bb[u] = *reinterpret_cast<const v8bf *>(wb + (ks + 4 * u) * 16);
+      a[u] = *reinterpret_cast<const v8bf *>(xa + (ks + NW * u) * 16);
+      bb[u] = *reinterpret_cast<const v8bf *>(wb + (ks + NW * u) * 16);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -103,7 +104,7 @@ dense_fwd_kernel(const uint16_t *__restrict__ X, int M, int K, const uint16_t *_
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], bb[u], acc, 0, 0, 0);
     }
   }
-  for (; ks < nks; ks += 4) {
+  for (; ks < nks; ks += NW) {
     v8bf a = *reinterpret_cast<const v8bf *>(xa + ks * 16);
     const v8bf bb = *reinterpret_cast<const v8bf *>(wb + ks * 16);
     if (!mv) a = v8bf{};
@@ -113,10 +114,12 @@ dense_fwd_kernel(const uint16_t *__restrict__ X, int M, int K, const uint16_t *_
   for (int v = 0; v < 16; ++v) red[w][v][lane] = acc[v];
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int o = threadIdx.x + 256 * j;
+  for (int j = 0; j < 16 / NW; ++j) {
+    const int o = threadIdx.x + 64 * NW * j;
     const int l = o & 63, v = o >> 6;
-    const float s = red[0][v][l] + red[1][v][l] + red[2][v][l] + red[3][v][l];
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) s += red[q][v][l];
     const int i = 8 * (v >> 2) + 4 * (l >> 5) + (v & 3), col = l & 31;
     const int mm = m0 + i, nn = n0 + col;
     if (mm < M) {
@@ -131,8 +134,11 @@ dense_fwd_kernel(const uint16_t *__restrict__ X, int M, int K, const uint16_t *_
 hipError_t launch_dense_fwd(const uint16_t *X, int M, int K, const uint16_t *Wt, const float *b,
                             int N, float *Z, uint16_t *Hbf, const Skip &sk, hipStream_t s) {
   if (K % 16 || N % 32) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(dense_fwd_kernel, dim3((M + 31) / 32, N / 32), dim3(256), 0, s, X, M, K, Wt,
-                     b, N, Z, Hbf, sk);
+  const dim3 grid((M + 31) / 32, N / 32);
+  if (K >= 1024)  // long reductions: more waves per tile
+    hipLaunchKernelGGL(dense_fwd_kernel<8>, grid, dim3(512), 0, s, X, M, K, Wt, b, N, Z, Hbf, sk);
+  else
+    hipLaunchKernelGGL(dense_fwd_kernel<4>, grid, dim3(256), 0, s, X, M, K, Wt, b, N, Z, Hbf, sk);
   return hipGetLastError();
 }
 
