@@ -198,6 +198,77 @@ def run_learned(args):
     return out
 
 
+def run_precision(args):
+    """Beam-weight precision sweep on the cfg2 shape: 3 CW interferers per
+    channel at INR 0-30 dB, rank-3 null.  Per precision: the 2B-beam kernel
+    time, the end-to-end null depth measured on the output beams (each known
+    tone's amplitude in the nulled beams against the original beams), the
+    beams' difference from the hi+lo run, and the top-3 eigenvalues against
+    LAPACK on the block's own exact visibilities."""
+    C, K = args.channels, 3
+    shape = syn.BlockShape(C=C, F=1, NT=64, R=36, P=2, RPP=36, NP=128, B=16, blocks_to_accumulate=1)
+    seed = 20261203 + 3
+    sc = syn.DriftScenario(C, 36, seed=seed, n_blocks=1, n_src=(3, 3), inr_db=(0.0, 30.0), drift=0.0,
+                           p_switch=0.0, max_sources=3)
+    tones, st, gain, _, _ = sc.block(0)
+    dr = lam.DeviceArray(shape.ring_shape(), np.int8)
+    ds = lam.DeviceArray(shape.scale_shape(), np.int16)
+    dt, dst, dg = (lam.DeviceArray.from_numpy(a) for a in (tones, st, gain))
+    lam.synth_block(shape, seed, 0, 16.0, dt, dst, dg, 3, dr, ds)
+    w16, _ = syn.random_weights(shape, seed=seed + 1)
+    T, B = shape.T, shape.B
+    t = np.arange(T)
+    # tone phasors e^{-2 pi i nu t} per (channel, source)
+    ph = np.exp(-2j * np.pi * tones[..., 1].astype(np.float64)[..., None] * t)  # [C][S][T]
+    out, ref_beams = {}, None
+    for name, mode in (("f16_hilo", lam.WPREC_F16_HILO), ("f16", lam.WPREC_F16), ("bf16", lam.WPREC_BF16)):
+        p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+        p.set_weights(w16)
+        p.set_fixed_k(np.full((C, 2), K, np.int32))
+        p.set_weight_precision(mode)
+        beams = p.beams_array()
+        evals = np.zeros((C, 2, 36), np.float32)
+        p.execute(dr, ds, beams_out=beams, evals_out=evals, on_device=True)
+        p.sync()
+        vis = p.dump_visibilities_i64()
+        p.set_profiling(True)
+        p.kernel_times()
+        for _ in range(5):
+            p.execute(dr, ds, on_device=True)
+        kt = p.kernel_times()
+        p.close()
+        y = beams[..., 0].astype(np.float64) + 1j * beams[..., 1].astype(np.float64)  # [C][2][2B][T]
+        amp = np.einsum("cpbt,cst->cpbs", y, ph) / T
+        orig = np.sum(np.abs(amp[:, :, :B]) ** 2, axis=(2, 3))
+        mitig = np.sum(np.abs(amp[:, :, B:]) ** 2, axis=(2, 3))
+        depth = 10 * np.log10(np.maximum(mitig, 1e-30) / orig)
+        # LAPACK on the whitened XX / YY covariance of the block's exact visibilities
+        Vc = vis[..., 0] + 1j * vis[..., 1]  # [C][nbl][2][2]
+        ii, jj = np.triu_indices(36)
+        bl = jj * (jj + 1) // 2 + ii
+        A = np.zeros((C, 2, 36, 36), np.complex128)
+        for pp in range(2):
+            A[:, pp, ii, jj] = Vc[:, bl, pp, pp]
+            A[:, pp, jj, ii] = np.conj(Vc[:, bl, pp, pp])
+        dd = np.sqrt(np.maximum(np.einsum("cpii->cpi", A).real, 1e-10))
+        lam_ref = np.linalg.eigvalsh(A / (dd[..., :, None] * dd[..., None, :]))
+        ev_err = np.abs(evals[..., -K:] - lam_ref[..., -K:]) / np.abs(lam_ref[..., -K:])
+        r = dict(beam_2b_us=round(kt["beam_2b"][0] / kt["beam_2b"][1] * 1e3, 1),
+                 null_depth_db_median=round(float(np.median(depth)), 2),
+                 null_depth_db_p90=round(float(np.quantile(depth, 0.9)), 2),
+                 top3_eigenvalue_rel_err_max=float(ev_err.max()))
+        if ref_beams is None:
+            ref_beams = y
+        else:
+            r["beam_rel_diff_vs_f16_hilo"] = float(np.max(np.abs(y - ref_beams)) / np.max(np.abs(ref_beams)))
+        out[name] = r
+        log(f"precision {name}: {r}")
+    out["workload"] = (f"{C} ch x 2 pol x 36 ant, {T} samples, 3 CW interferers per channel at INR 0-30 dB, "
+                       "rank-3 null, 16 -> 32 beams; beam error against the FP64 oracle is in "
+                       "tests/test_gpu_precision.py")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +286,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--refine", type=int, default=2, help="warm-start refinement steps")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--spw", type=int, default=0, help="samples per voltage workgroup (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -312,6 +384,10 @@ def main():
         learned["plain"]["real_time_factor"] = round(
             (shape.T / CHANNEL_RATE_HZ) / (learned["plain"]["gpu_ms_per_block"] * 1e-3), 3)
 
+    precision = None
+    if not args.no_precision:
+        precision = run_precision(args)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -342,6 +418,7 @@ def main():
         "kernels": kstats,
         "autotune": tuned,
         "learned_vs_plain": learned,
+        "precision_sweep": precision,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

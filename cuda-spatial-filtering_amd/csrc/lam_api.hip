@@ -103,6 +103,7 @@ struct lam_ctx {
   int eig_method = LAM_EIG_PLAIN, refine_sweeps = 2, power = 0;
   float res_tol = 0.1f;
   int eig_plain_threads = 256, eig_fallback_threads = 1024;
+  int wprec = kWprecF16HiLo;
   Learn lr;
   int last_buf = -1;
   double prof_ms[LAM_K_COUNT] = {0};
@@ -169,6 +170,7 @@ VoltParams base_volt(lam_ctx *ctx, const void *samples, const void *scales) {
   p.xcd_map = (ctx->C % 8 == 0) ? 1 : 0;
   p.err = ctx->d_err;
   p.NK = ctx->NK;
+  p.wprec = ctx->wprec;
   return p;
 }
 
@@ -281,6 +283,7 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
     e.sweeps = b.d_sweeps;
     e.fallback = b.d_fallback;
     e.nsig = b.d_nsig;
+    e.wprec = ctx->wprec;
     e.plain_threads = ctx->eig_plain_threads;
     e.fallback_threads = ctx->eig_fallback_threads;
     if (L.ready) {
@@ -387,7 +390,7 @@ int update_wop(lam_ctx *ctx) {
   if (ctx->mode != LAM_MODE_CORR_BEAM) return LAM_OK;
   hipStream_t s = ctx->bufs[0].stream;
   HIPC(ctx, launch_prep_weights(ctx->d_weights, ctx->d_gains, ctx->d_phase, ctx->C, ctx->B,
-                                ctx->R, ctx->NMT1, ctx->NK, ctx->d_wop, s));
+                                ctx->R, ctx->NMT1, ctx->NK, ctx->wprec, ctx->d_wop, s));
   HIPC(ctx, hipStreamSynchronize(s));
   return LAM_OK;
 }
@@ -807,6 +810,14 @@ int lam_set_tuning(lam_ctx *ctx, int samples_per_wg) {
   ctx->samples_per_wg = samples_per_wg;
   ctx->graphs_dirty = true;
   return LAM_OK;
+}
+
+int lam_set_weight_precision(lam_ctx *ctx, int precision) {
+  if (!ctx || precision < LAM_WPREC_F16_HILO || precision > LAM_WPREC_BF16) return LAM_ERR_ARG;
+  if (int r = sync_all(ctx)) return r;
+  ctx->wprec = precision;
+  ctx->graphs_dirty = true;
+  return update_wop(ctx);
 }
 
 int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads, int fallback_threads) {

@@ -880,7 +880,7 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
     S.W[(B + b) * n + i] = acc;
   }
   __syncthreads();
-  // 8. MFMA operands of the 2B beam rows [W; W'] (split hi + lo FP16)
+  // 8. MFMA operands of the 2B beam rows [W; W'] (split hi + lo FP16, FP16 or BF16)
   const int nrow = 2 * B;
   for (int idx = tid; idx < P.NMT * P.NK * 64; idx += nthr) {
     const int lane = idx & 63, j = (idx >> 6) % P.NK, mt = (idx >> 6) / P.NK;
@@ -894,11 +894,13 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
       if (e < nrow && a < n) w = S.W[e * n + a];
       const float v0 = u == 0 ? w.x : w.y;
       const float v1 = u == 0 ? -w.y : w.x;
-      const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
+      _Float16 h0, l0, h1, l1;
+      weight_operands(v0, P.wprec, h0, l0);
+      weight_operands(v1, P.wprec, h1, l1);
       hi[2 * i] = h0;
+      lo[2 * i] = l0;
       hi[2 * i + 1] = h1;
-      lo[2 * i] = (_Float16)(v0 - (float)h0);
-      lo[2 * i + 1] = (_Float16)(v1 - (float)h1);
+      lo[2 * i + 1] = l1;
     }
     v8h *dst = P.wop + ((cp * P.NMT + mt) * P.NK + j) * 2 * 64;
     dst[lane] = hi;

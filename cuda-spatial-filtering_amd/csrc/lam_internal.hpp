@@ -10,6 +10,26 @@ namespace lam {
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// Beam-weight operand precision (the (hi, lo) operand slots of a k-step):
+//   F16_HILO: hi = fp16(w), lo = fp16(w - hi), two MFMAs (about fp32 weights)
+//   F16:      hi = fp16(w), one MFMA
+//   BF16:     hi = bf16(w) bits, one BF16 MFMA with BF16 samples
+constexpr int kWprecF16HiLo = 0, kWprecF16 = 1, kWprecBF16 = 2;
+
+// Weight operand pair for one real weight value.
+__device__ __forceinline__ void weight_operands(float v, int wprec, _Float16 &hi, _Float16 &lo) {
+  if (wprec == kWprecBF16) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    hi = __builtin_bit_cast(_Float16, (uint16_t)(u >> 16));
+    lo = (_Float16)0.0f;
+  } else {
+    hi = (_Float16)v;
+    lo = wprec == kWprecF16HiLo ? (_Float16)(v - (float)hi) : (_Float16)0.0f;
+  }
+}
 
 // Voltage kernels stage 128-sample chunks through LDS.
 constexpr int kChunk = 128;
@@ -45,6 +65,7 @@ struct VoltParams {
   int NMT;                 // 32-row operand tiles (16 beams each)
   int NK;                  // K steps of 8 antennas
   __half *beams;           // [C][2][nbeams_out][T][2]
+  int wprec;               // beam-weight operand precision (kWprec*)
   int *err;                // error flag word
 };
 
@@ -79,6 +100,7 @@ struct EigParams {
   int plain_threads;       // workgroup width of the plain solver (256, 512, 1024)
   int fallback_threads;    // workgroup width of the deferred fallback launch
   int only_failed;         // set by launch_eig for the second launch of the warm start
+  int wprec;               // precision of the [W; W'] beam operands (kWprec*)
 };
 
 // Synthetic voltage generator (benchmark / test input).
@@ -129,7 +151,7 @@ hipError_t launch_volt(const VoltParams &p, bool do_corr, bool do_beam, hipStrea
 hipError_t launch_eig(const EigParams &p, int part, hipStream_t s);
 bool eig_is_warm(const EigParams &p);
 hipError_t launch_prep_weights(const __half *w, const float2 *gains, const float2 *phase,
-                               int C, int B, int R, int NMT, int NK, v8h *wop,
+                               int C, int B, int R, int NMT, int NK, int wprec, v8h *wop,
                                hipStream_t s);
 hipError_t launch_accumulate_vis(const int *vis_part, int nsplit, long long count, long long *acc,
                                  bool atomic, hipStream_t s);

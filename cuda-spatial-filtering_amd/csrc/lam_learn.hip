@@ -13,8 +13,6 @@
 
 namespace lam {
 
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
-
 __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
   uint32_t u = __float_as_uint(f);
   u += 0x7fffu + ((u >> 16) & 1u);

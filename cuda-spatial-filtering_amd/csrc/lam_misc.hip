@@ -11,7 +11,7 @@ namespace lam {
 // w_eff = w * gain * exp(i phi) (phase of the fine delay at channel centre),
 // split into hi + lo FP16.
 __global__ void prep_weights_kernel(const __half *w, const float2 *gains, const float2 *phase,
-                                    int C, int B, int R, int NMT, int NK, v8h *wop) {
+                                    int C, int B, int R, int NMT, int NK, int wprec, v8h *wop) {
   const long total = (long)C * 2 * NMT * NK * 64;
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total;
        idx += (long)gridDim.x * blockDim.x) {
@@ -40,11 +40,13 @@ __global__ void prep_weights_kernel(const __half *w, const float2 *gains, const 
       }
       const float v0 = u == 0 ? ww.x : ww.y;
       const float v1 = u == 0 ? -ww.y : ww.x;
-      const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
+      _Float16 h0, l0, h1, l1;
+      weight_operands(v0, wprec, h0, l0);
+      weight_operands(v1, wprec, h1, l1);
       hi[2 * i] = h0;
+      lo[2 * i] = l0;
       hi[2 * i + 1] = h1;
-      lo[2 * i] = (_Float16)(v0 - (float)h0);
-      lo[2 * i + 1] = (_Float16)(v1 - (float)h1);
+      lo[2 * i + 1] = l1;
     }
     v8h *dst = wop + ((((long)c * 2 + p) * NMT + mt) * NK + j) * 2 * 64;
     dst[lane] = hi;
@@ -53,12 +55,12 @@ __global__ void prep_weights_kernel(const __half *w, const float2 *gains, const 
 }
 
 hipError_t launch_prep_weights(const __half *w, const float2 *gains, const float2 *phase,
-                               int C, int B, int R, int NMT, int NK, v8h *wop,
+                               int C, int B, int R, int NMT, int NK, int wprec, v8h *wop,
                                hipStream_t s) {
   const long total = (long)C * 2 * NMT * NK * 64;
   const int blocks = (int)((total + 255) / 256);
   hipLaunchKernelGGL(prep_weights_kernel, dim3(blocks), dim3(256), 0, s, w, gains, phase, C, B,
-                     R, NMT, NK, wop);
+                     R, NMT, NK, wprec, wop);
   return hipGetLastError();
 }
 

@@ -206,15 +206,28 @@ __device__ __forceinline__ void beam_chunk(const VoltParams &P, const uint32_t *
       // 4 antennas x both pols: 16-byte aligned since RSD and 8j + 4kg are multiples of 4
       const uint4 d = *reinterpret_cast<const uint4 *>(L + t * RSD + 8 * j + 4 * kg);
       const uint32_t w4[4] = {d.x, d.y, d.z, d.w};
-      v8h xop;
+      if (P.wprec == kWprecBF16) {
+        // int8 samples are exact in BF16 too; the weight operand holds BF16 bits
+        v8bf xb;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t pr = w4[i] >> (16 * pol);
-        xop[2 * i] = (_Float16)(int)(int8_t)(pr & 0xffu);
-        xop[2 * i + 1] = (_Float16)(int)(int8_t)((pr >> 8) & 0xffu);
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t pr = w4[i] >> (16 * pol);
+          xb[2 * i] = (__bf16)(float)(int)(int8_t)(pr & 0xffu);
+          xb[2 * i + 1] = (__bf16)(float)(int)(int8_t)((pr >> 8) & 0xffu);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, wop[2 * j]), xb, acc, 0, 0, 0);
+      } else {
+        v8h xop;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t pr = w4[i] >> (16 * pol);
+          xop[2 * i] = (_Float16)(int)(int8_t)(pr & 0xffu);
+          xop[2 * i + 1] = (_Float16)(int)(int8_t)((pr >> 8) & 0xffu);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[2 * j], xop, acc, 0, 0, 0);
+        if (P.wprec == kWprecF16HiLo)
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[2 * j + 1], xop, acc, 0, 0, 0);
       }
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[2 * j], xop, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[2 * j + 1], xop, acc, 0, 0, 0);
     }
     const int s = s_chunk + t;
     if (s < s_end) {

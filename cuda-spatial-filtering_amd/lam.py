@@ -26,6 +26,7 @@ MODE_CORR_BEAM, MODE_ADAPTIVE = 0, 1
 K_CORRBEAM, K_CORR, K_EIG, K_BEAM2B = range(4)
 KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp", "eig_fallback"]
 EIG_PLAIN, EIG_LEARNED, EIG_PREVIOUS = 0, 1, 2
+WPREC_F16_HILO, WPREC_F16, WPREC_BF16 = 0, 1, 2
 
 _lib = None
 
@@ -108,6 +109,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_set_eig_method.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float]
     L.lam_eig_stats.argtypes = [vp, vp, vp]
     L.lam_set_eig_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.lam_set_weight_precision.argtypes = [vp, ctypes.c_int]
     L.lam_autotune.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     L.lam_eigh_warm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
@@ -337,6 +339,9 @@ class Pipeline:
         self._chk(self.L.lam_autotune(self.h, sp, sc, n, iters, res.ctypes.data))
         return dict(samples_per_wg=int(res[0]), plain_threads=int(res[1]), refine_steps=int(res[2]),
                     fallback_threads=int(res[3]))
+
+    def set_weight_precision(self, precision):
+        self._chk(self.L.lam_set_weight_precision(self.h, precision))
 
     def set_eig_tuning(self, plain_threads=256, fallback_threads=1024):
         self._chk(self.L.lam_set_eig_tuning(self.h, plain_threads, fallback_threads))

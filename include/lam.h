@@ -173,6 +173,16 @@ int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);
 /* Tunables picked by the autotuner: time samples per workgroup of the
  * voltage kernels (multiple of 64; 0 = automatic). */
 int lam_set_tuning(lam_ctx *ctx, int samples_per_wg);
+/* Precision of the beamformer's weight operands (no counterpart in the
+ * reference, whose ccglib GEMM takes FP16 weights):
+ *   LAM_WPREC_F16_HILO  weights split into FP16 hi + lo, two FP16 MFMAs (default;
+ *                       about FP32 weights, incl. the projected W' = W P)
+ *   LAM_WPREC_F16       FP16 weights, one FP16 MFMA
+ *   LAM_WPREC_BF16      BF16 weights and samples, one BF16 MFMA
+ * The int8 samples are exact in every case; accumulation is FP32. */
+enum { LAM_WPREC_F16_HILO = 0, LAM_WPREC_F16 = 1, LAM_WPREC_BF16 = 2 };
+int lam_set_weight_precision(lam_ctx *ctx, int precision);
+
 /* Workgroup width (256, 512 or 1024 threads) of the plain Jacobi launch and
  * of the second launch that re-solves the problems a warm start rejected. */
 int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads, int fallback_threads);
