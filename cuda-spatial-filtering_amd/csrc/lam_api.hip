@@ -104,6 +104,13 @@ struct lam_ctx {
   float res_tol = 0.1f;
   int eig_plain_threads = 256, eig_fallback_threads = 1024;
   int wprec = kWprecF16HiLo;
+  int num_cus = 256;
+  // subarrays of uneven size (persistent eigen launch)
+  int sub_G = 0;
+  uint8_t *d_sub_mask = nullptr;
+  int *d_sub_k = nullptr, *d_sub_order = nullptr, *d_sub_counter = nullptr;
+  float *d_sub_evals = nullptr, *d_sub_evecs = nullptr, *d_sub_proj = nullptr;
+  int *d_sub_counts = nullptr, *d_sub_sweeps = nullptr, *d_sub_sizes = nullptr;
   Learn lr;
   int last_buf = -1;
   double prof_ms[LAM_K_COUNT] = {0};
@@ -312,6 +319,29 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
       HIPC(ctx, launch_eig(e, 0, s));
     }
   }
+  if (ctx->sub_G > 0) {  // subarrays: every (channel, pol, group) problem in one launch
+    SubParams q{};
+    q.C = ctx->C;
+    q.R = ctx->R;
+    q.G = ctx->sub_G;
+    q.vis_part = b.d_vis_part;
+    q.nsplit = p.nsplit;
+    q.mask = ctx->d_sub_mask;
+    q.k = ctx->d_sub_k;
+    q.order = ctx->d_sub_order;
+    q.nprob = ctx->C * 2 * ctx->sub_G;
+    q.max_sweeps = ctx->max_sweeps;
+    q.tol = ctx->tol;
+    q.counter = ctx->d_sub_counter;
+    q.evals = ctx->d_sub_evals;
+    q.evecs = ctx->d_sub_evecs;
+    q.proj = ctx->d_sub_proj;
+    q.counts = ctx->d_sub_counts;
+    q.sweeps = ctx->d_sub_sweeps;
+    q.sizes = ctx->d_sub_sizes;
+    Prof pr(ctx, s, LAM_K_SUBARRAY);
+    HIPC(ctx, launch_subarray_null(q, ctx->num_cus * 2, s));
+  }
   if (capture) {  // labels: this block's plain-Jacobi vectors and mode counts
     const size_t nu = (size_t)L.KW * ctx->R;
     HIPC(ctx, hipMemcpyAsync(L.d_dsU + (size_t)L.cap_count * M * nu, L.d_basis, M * nu * sizeof(float2),
@@ -395,6 +425,19 @@ int update_wop(lam_ctx *ctx) {
   return LAM_OK;
 }
 
+
+void free_subarrays(lam_ctx *ctx) {
+  for (void *q : {(void *)ctx->d_sub_mask, (void *)ctx->d_sub_k, (void *)ctx->d_sub_order,
+                  (void *)ctx->d_sub_counter, (void *)ctx->d_sub_evals, (void *)ctx->d_sub_evecs,
+                  (void *)ctx->d_sub_proj, (void *)ctx->d_sub_counts, (void *)ctx->d_sub_sweeps,
+                  (void *)ctx->d_sub_sizes})
+    if (q) hipFree(q);
+  ctx->d_sub_mask = nullptr;
+  ctx->d_sub_k = ctx->d_sub_order = ctx->d_sub_counter = nullptr;
+  ctx->d_sub_evals = ctx->d_sub_evecs = ctx->d_sub_proj = nullptr;
+  ctx->d_sub_counts = ctx->d_sub_sweeps = ctx->d_sub_sizes = nullptr;
+  ctx->sub_G = 0;
+}
 
 void free_learn(Learn &L) {
   for (void *q : {(void *)L.d_prm, (void *)L.d_grad, (void *)L.d_m1, (void *)L.d_m2,
@@ -583,6 +626,7 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return LAM_ERR_NO_DEVICE;
 
   lam_ctx *ctx = new lam_ctx();
+  ctx->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   ctx->shape = *sh;
   ctx->mode = mode;
   ctx->nbuf = num_buffers;
@@ -699,6 +743,7 @@ int lam_destroy(lam_ctx *ctx) {
   hipFree(ctx->d_dump_f32);
   hipFree(ctx->d_dump_i64);
   free_learn(ctx->lr);
+  free_subarrays(ctx);
   if (ctx->reset_done) hipEventDestroy(ctx->reset_done);
   delete ctx;
   return LAM_OK;
@@ -1075,6 +1120,65 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
     result[3] = ctx->eig_fallback_threads;
   }
   return rc;
+}
+
+int lam_set_subarrays(lam_ctx *ctx, int ngroups, const uint8_t *mask, const int32_t *k) {
+  if (!ctx || ngroups < 0 || (ngroups > 0 && (!mask || !k))) return LAM_ERR_ARG;
+  if (ctx->mode != LAM_MODE_ADAPTIVE) return fail(ctx, LAM_ERR_STATE, "subarrays need the adaptive mode");
+  if (int r = sync_all(ctx)) return r;
+  free_subarrays(ctx);
+  ctx->graphs_dirty = true;
+  if (ngroups == 0) return LAM_OK;
+  const int C = ctx->C, R = ctx->R, G = ngroups, np = C * 2 * G;
+  // problem order: largest subarray first (the persistent workgroups take them in turn)
+  std::vector<int> size(np), order(np);
+  for (int c = 0; c < C; ++c)
+    for (int p = 0; p < 2; ++p)
+      for (int g = 0; g < G; ++g) {
+        int n = 0;
+        for (int r = 0; r < R; ++r) n += mask[((size_t)c * G + g) * R + r] != 0;
+        size[(c * 2 + p) * G + g] = n;
+      }
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return size[a] > size[b]; });
+  int rc = LAM_OK;
+  auto chk = [&](int r) {
+    if (r != LAM_OK && rc == LAM_OK) rc = r;
+  };
+  chk(dalloc(ctx, &ctx->d_sub_mask, (size_t)C * G * R));
+  chk(dalloc(ctx, &ctx->d_sub_k, (size_t)np * sizeof(int)));
+  chk(dalloc(ctx, &ctx->d_sub_order, (size_t)np * sizeof(int)));
+  chk(dalloc(ctx, &ctx->d_sub_counter, sizeof(int)));
+  chk(dalloc(ctx, &ctx->d_sub_evals, (size_t)np * R * sizeof(float)));
+  chk(dalloc(ctx, &ctx->d_sub_evecs, (size_t)np * R * R * 2 * sizeof(float)));
+  chk(dalloc(ctx, &ctx->d_sub_proj, (size_t)np * R * R * 2 * sizeof(float)));
+  chk(dalloc(ctx, &ctx->d_sub_counts, (size_t)np * sizeof(int)));
+  chk(dalloc(ctx, &ctx->d_sub_sweeps, (size_t)np * sizeof(int)));
+  chk(dalloc(ctx, &ctx->d_sub_sizes, (size_t)np * sizeof(int)));
+  if (rc != LAM_OK) {
+    free_subarrays(ctx);
+    return rc;
+  }
+  HIPC(ctx, hipMemcpy(ctx->d_sub_mask, mask, (size_t)C * G * R, hipMemcpyHostToDevice));
+  HIPC(ctx, hipMemcpy(ctx->d_sub_k, k, (size_t)np * sizeof(int), hipMemcpyHostToDevice));
+  HIPC(ctx, hipMemcpy(ctx->d_sub_order, order.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
+  ctx->sub_G = G;
+  return LAM_OK;
+}
+
+int lam_subarray_results(lam_ctx *ctx, float *evals, float *evecs, float *proj, int32_t *counts,
+                         int32_t *sizes, int32_t *sweeps) {
+  if (!ctx) return LAM_ERR_ARG;
+  if (ctx->sub_G == 0) return fail(ctx, LAM_ERR_STATE, "no subarrays set");
+  if (int r = sync_all(ctx)) return r;
+  const size_t np = (size_t)ctx->C * 2 * ctx->sub_G, R = ctx->R;
+  if (evals) HIPC(ctx, hipMemcpy(evals, ctx->d_sub_evals, np * R * sizeof(float), hipMemcpyDeviceToHost));
+  if (evecs) HIPC(ctx, hipMemcpy(evecs, ctx->d_sub_evecs, np * R * R * 2 * sizeof(float), hipMemcpyDeviceToHost));
+  if (proj) HIPC(ctx, hipMemcpy(proj, ctx->d_sub_proj, np * R * R * 2 * sizeof(float), hipMemcpyDeviceToHost));
+  if (counts) HIPC(ctx, hipMemcpy(counts, ctx->d_sub_counts, np * sizeof(int), hipMemcpyDeviceToHost));
+  if (sizes) HIPC(ctx, hipMemcpy(sizes, ctx->d_sub_sizes, np * sizeof(int), hipMemcpyDeviceToHost));
+  if (sweeps) HIPC(ctx, hipMemcpy(sweeps, ctx->d_sub_sweeps, np * sizeof(int), hipMemcpyDeviceToHost));
+  return LAM_OK;
 }
 
 int lam_set_eig_method(lam_ctx *ctx, int method, int refine_sweeps, int power, float residual_tol) {

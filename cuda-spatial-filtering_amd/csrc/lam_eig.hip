@@ -12,6 +12,8 @@
 // with a single launch.  The eigensolver is a parallel-order (round-robin)
 // cyclic Jacobi method: each round applies n/2 disjoint complex rotations,
 // rotation angles are computed in FP64 from the FP32 matrix.
+#include <algorithm>
+
 #include "lam_internal.hpp"
 
 namespace lam {
@@ -951,6 +953,129 @@ hipError_t launch_eig(const EigParams &p, int part, hipStream_t s) {
   EigParams f = p;
   f.only_failed = 1;
   return launch_eig_plain(f, p.fallback_threads, s);
+}
+
+// ---- subarrays of uneven size: one persistent launch ----
+// Problem q = order[q'] (largest first) is (channel c, pol p, group g): the
+// principal submatrix of the (c, p) covariance on the group's antennas (mask
+// bits, ascending antenna order), whitened, solved by cyclic Jacobi, K_g
+// largest modes removed: outputs eigenvalues (ascending), unwhitened
+// normalised eigenvectors and P = I - U_K U_K^H, all in the group's n_g x n_g
+// antenna space.  Workgroups take problems from an atomic counter until the
+// list is exhausted, so sizes 12 .. 64 share the machine without tail waste.
+__global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P) {
+  extern __shared__ __align__(16) char smem_raw[];
+  __shared__ int s_next, s_n, s_ant[64];
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const long nbl_full = (long)P.R * (P.R + 1) / 2;
+  for (;;) {
+    if (tid == 0) s_next = atomicAdd(P.counter, 1);
+    __syncthreads();
+    const int qi = s_next;
+    if (qi >= P.nprob) break;  // every wave of the workgroup leaves together
+    const int q = P.order[qi];
+    const int g = q % P.G, cp = q / P.G, c = cp >> 1, pol = cp & 1;
+    // antenna list of the group: ballot prefix over the mask
+    if (tid < 64) {
+      const bool on = tid < P.R && P.mask[((long)c * P.G + g) * P.R + tid] != 0;
+      const unsigned long long bits = __ballot(on);
+      const int pos = __popcll(bits & ((1ull << tid) - 1ull));
+      if (on) s_ant[pos] = tid;
+      if (tid == 0) s_n = __popcll(bits);
+    }
+    __syncthreads();
+    const int n = s_n;
+    const long obase = ((long)q) * P.R * P.R;  // output slot of this problem
+    if (n == 0) {
+      __syncthreads();
+      continue;
+    }
+    EigSmem S = carve(smem_raw, n, 0, false);
+    const int ld = n | 1;
+    const int nb = n * (n + 1) / 2;
+    for (int idx = tid; idx < nb; idx += nthr) {  // packed (i <= j) of the subarray
+      int j = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
+      while (j * (j + 1) / 2 > idx) --j;
+      while ((j + 1) * (j + 2) / 2 <= idx) ++j;
+      const int i = idx - j * (j + 1) / 2;
+      const int ai = s_ant[i], aj = s_ant[j];
+      const long bl = (long)aj * (aj + 1) / 2 + ai;
+      long re = 0, im = 0;
+      for (int k = 0; k < P.nsplit; ++k) {
+        const int *v = P.vis_part + ((((long)k * P.C + c) * nbl_full + bl) * 4 + pol * 2 + pol) * 2;
+        re += v[0];
+        im += v[1];
+      }
+      const float2 a = make_float2((float)re, (float)im);
+      S.A[j * ld + i] = a;
+      S.A[i * ld + j] = make_float2(a.x, -a.y);
+    }
+    __syncthreads();
+    if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f));
+    __syncthreads();
+    for (int i = tid; i < n * n; i += nthr) {
+      const int col = i / n, row = i - col * n;
+      const float den = S.d[row] * S.d[col];
+      S.A[col * ld + row].x /= den;
+      S.A[col * ld + row].y /= den;
+    }
+    __syncthreads();
+    const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
+    sort_eigen(S.A, n, S.perm, S.ev);
+    const int K = max(0, min(n, P.k[q]));
+    // unwhitened, normalised eigenvectors (ascending) into A (stride n)
+    if (tid < n) {
+      const int src = S.perm[tid];
+      float nrm = 0.0f;
+      for (int i = 0; i < n; ++i) {
+        float2 v = S.V[src * n + i];
+        v.x *= S.d[i];
+        v.y *= S.d[i];
+        S.A[tid * n + i] = v;
+        nrm += v.x * v.x + v.y * v.y;
+      }
+      const float inv = nrm > 0.0f ? rsqrtf(nrm) : 0.0f;
+      for (int i = 0; i < n; ++i) {
+        S.A[tid * n + i].x *= inv;
+        S.A[tid * n + i].y *= inv;
+      }
+      P.evals[(long)q * P.R + tid] = S.ev[tid];
+    }
+    __syncthreads();
+    float2 *ev_out = reinterpret_cast<float2 *>(P.evecs) + obase;
+    float2 *pr_out = reinterpret_cast<float2 *>(P.proj) + obase;
+    for (int e = tid; e < n * n; e += nthr) {
+      const int col = e / n, row = e - col * n;
+      ev_out[e] = S.A[e];
+      // P(row, col) = delta - sum_{k >= n-K} u_k(row) conj(u_k(col))
+      float2 acc = make_float2(row == col ? 1.0f : 0.0f, 0.0f);
+      for (int k = n - K; k < n; ++k) {
+        const float2 t = cmulc(S.A[k * n + row], S.A[k * n + col]);
+        acc.x -= t.x;
+        acc.y -= t.y;
+      }
+      pr_out[e] = acc;
+    }
+    if (tid == 0) {
+      P.counts[q] = K;
+      if (P.sweeps) P.sweeps[q] = sw;
+      P.sizes[q] = n;
+    }
+    __syncthreads();  // LDS is reused by the next problem
+  }
+}
+
+hipError_t launch_subarray_null(const SubParams &p, int max_wgs, hipStream_t s) {
+  const size_t sm = eig_smem_bytes(64, 0, false);
+  if (sm > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(subarray_null_kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+  }
+  if (hipError_t e = launch_zero(p.counter, sizeof(int), s)) return e;
+  const int grid = std::max(1, std::min(p.nprob, max_wgs));
+  hipLaunchKernelGGL(subarray_null_kernel, dim3(grid), dim3(kEigThreads), sm, s, p);
+  return hipGetLastError();
 }
 
 // ---- standalone batched eigensolver (unit-test entry point) ----

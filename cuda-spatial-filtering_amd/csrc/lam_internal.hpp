@@ -103,6 +103,24 @@ struct EigParams {
   int wprec;               // precision of the [W; W'] beam operands (kWprec*)
 };
 
+// Eigen stage of subarrays (uneven sizes), one persistent launch.
+struct SubParams {
+  int C, R, G;             // channels, receivers, groups per channel
+  const int *vis_part;     // [nsplit][C][NBL][2][2][2]
+  int nsplit;
+  const uint8_t *mask;     // [C][G][R] antenna membership
+  const int *k;            // [C*2*G] modes to remove
+  const int *order;        // [nprob] problem ids, largest first
+  int nprob;               // C * 2 * G
+  int max_sweeps;
+  float tol;
+  int *counter;            // work-queue counter (cleared by the launcher)
+  float *evals;            // out: [nprob][R] (first n_g valid)
+  float *evecs;            // out: [nprob][R*R][2] (n_g x n_g, columns)
+  float *proj;             // out: [nprob][R*R][2] P = I - U_K U_K^H (n_g x n_g)
+  int *counts, *sweeps, *sizes;  // out: [nprob]
+};
+
 // Synthetic voltage generator (benchmark / test input).
 constexpr int kGenMaxSources = 8;
 struct GenParams {
@@ -149,6 +167,7 @@ hipError_t launch_gather(const int *idx, int nb, const uint16_t *DX, int Dp, con
 hipError_t launch_synth(const GenParams &g, hipStream_t s);
 hipError_t launch_volt(const VoltParams &p, bool do_corr, bool do_beam, hipStream_t s);
 hipError_t launch_eig(const EigParams &p, int part, hipStream_t s);
+hipError_t launch_subarray_null(const SubParams &p, int max_wgs, hipStream_t s);
 bool eig_is_warm(const EigParams &p);
 hipError_t launch_prep_weights(const __half *w, const float2 *gains, const float2 *phase,
                                int C, int B, int R, int NMT, int NK, int wprec, v8h *wop,

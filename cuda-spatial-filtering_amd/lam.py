@@ -24,7 +24,8 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "lam.h")
 LAM_OK, LAM_ERR_ARG, LAM_ERR_HIP, LAM_ERR_RANGE, LAM_ERR_STATE, LAM_ERR_UNSUPPORTED, LAM_ERR_NO_DEVICE = range(7)
 MODE_CORR_BEAM, MODE_ADAPTIVE = 0, 1
 K_CORRBEAM, K_CORR, K_EIG, K_BEAM2B = range(4)
-KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp", "eig_fallback"]
+KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp", "eig_fallback",
+                "subarray_eig"]
 EIG_PLAIN, EIG_LEARNED, EIG_PREVIOUS = 0, 1, 2
 WPREC_F16_HILO, WPREC_F16, WPREC_BF16 = 0, 1, 2
 
@@ -110,6 +111,8 @@ def load_library(path: str = LIB_PATH):
     L.lam_eig_stats.argtypes = [vp, vp, vp]
     L.lam_set_eig_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.lam_set_weight_precision.argtypes = [vp, ctypes.c_int]
+    L.lam_set_subarrays.argtypes = [vp, ctypes.c_int, vp, vp]
+    L.lam_subarray_results.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.lam_autotune.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     L.lam_eigh_warm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_int, ctypes.c_float, vp, vp, vp, vp]
@@ -339,6 +342,27 @@ class Pipeline:
         self._chk(self.L.lam_autotune(self.h, sp, sc, n, iters, res.ctypes.data))
         return dict(samples_per_wg=int(res[0]), plain_threads=int(res[1]), refine_steps=int(res[2]),
                     fallback_threads=int(res[3]))
+
+    def set_subarrays(self, mask, k):
+        """mask [C][G][R] bool/uint8, k [C][2][G] int."""
+        m = np.ascontiguousarray(mask, np.uint8)
+        kk = np.ascontiguousarray(k, np.int32)
+        self._chk(self.L.lam_set_subarrays(self.h, m.shape[1], m.ctypes.data, kk.ctypes.data))
+        self._sub_G = m.shape[1]
+
+    def subarray_results(self):
+        s = self.shape
+        npb = s.C * 2 * self._sub_G
+        ev = np.zeros((npb, s.R), np.float32)
+        vec = np.zeros((npb, s.R * s.R, 2), np.float32)
+        pr = np.zeros((npb, s.R * s.R, 2), np.float32)
+        cnt = np.zeros(npb, np.int32)
+        sz = np.zeros(npb, np.int32)
+        sw = np.zeros(npb, np.int32)
+        self._chk(self.L.lam_subarray_results(self.h, ev.ctypes.data, vec.ctypes.data, pr.ctypes.data,
+                                              cnt.ctypes.data, sz.ctypes.data, sw.ctypes.data))
+        return dict(evals=ev, evecs=vec[..., 0] + 1j * vec[..., 1], proj=pr[..., 0] + 1j * pr[..., 1],
+                    counts=cnt, sizes=sz, sweeps=sw)
 
     def set_weight_precision(self, precision):
         self._chk(self.L.lam_set_weight_precision(self.h, precision))

@@ -165,7 +165,7 @@ int lam_set_graphs(lam_ctx *ctx, int enable);
  * and launch counts per kernel id (see LAM_K_* below) and resets them. */
 enum {
   LAM_K_CORRBEAM = 0, LAM_K_CORR = 1, LAM_K_EIG = 2, LAM_K_BEAM2B = 3,
-  LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_EIGFB = 6, LAM_K_COUNT = 7
+  LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_EIGFB = 6, LAM_K_SUBARRAY = 7, LAM_K_COUNT = 8
 };
 int lam_set_profiling(lam_ctx *ctx, int enable);
 int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);
@@ -244,6 +244,21 @@ int lam_set_eig_method(lam_ctx *ctx, int method, int refine_sweeps, int power, f
 /* Per (channel, pol) of the most recent block: Jacobi sweeps run (refinement
  * plus fallback) and whether the plain fallback ran.  Synchronous. */
 int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback);
+
+/* Subarrays (no counterpart in the reference): ngroups antenna groups per
+ * channel, mask[C][ngroups][R] (non-zero = member), k[C][2][ngroups] modes to
+ * remove.  Every block then also solves, in one persistent launch after the
+ * eigen stage, all C x 2 x ngroups eigenproblems of uneven size (the principal
+ * submatrices of each pol's covariance on the group's antennas, whitened,
+ * plain Jacobi, largest first across the machine) and forms each group's null
+ * projector P = I - U_K U_K^H in its own antenna space.  ngroups = 0 clears. */
+int lam_set_subarrays(lam_ctx *ctx, int ngroups, const uint8_t *mask, const int32_t *k);
+/* Results of the last block, problem q = (c * 2 + pol) * ngroups + g, synchronous
+ * host copies (any pointer may be NULL): evals [q][R] (first n_q valid,
+ * ascending), evecs and proj [q][R*R][2] (n_q x n_q column-major at the start of
+ * each slot), counts / sizes / sweeps [q]. */
+int lam_subarray_results(lam_ctx *ctx, float *evals, float *evecs, float *proj, int32_t *counts,
+                         int32_t *sizes, int32_t *sweeps);
 
 /* Allocate and randomly initialise the MLP: warm_vectors (even, 2..8) and
  * hidden width (multiple of 32).  Called implicitly with (4, 256, 1) by the

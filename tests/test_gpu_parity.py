@@ -426,3 +426,52 @@ def test_autotune_adaptive_warm_start():
     p.close()
     s = _o(shape)
     assert np.array_equal(vis64, orc.correlate(s, orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)))
+
+
+def test_subarrays_uneven_sizes_persistent():
+    """Subarray groups of 12/24/36 antennas: every (channel, pol, group)
+    eigenproblem in one persistent launch, against LAPACK on the oracle's
+    principal submatrices; null projectors against I - U_K U_K^H."""
+    shape = syn.BlockShape(C=4, F=1, NT=64, R=36, P=2, RPP=36, NP=8, B=4)
+    scene = syn.make_scene(shape, seed=51, n_tones=3, amp=30.0)
+    ring, scales = syn.make_block(shape, scene, seed=52)
+    rng = np.random.default_rng(53)
+    G = 2
+    mask = np.zeros((shape.C, G, shape.R), np.uint8)
+    for c in range(shape.C):
+        for g in range(G):
+            n = rng.choice([12, 24, 36])
+            mask[c, g, rng.choice(shape.R, n, replace=False)] = 1
+    k = rng.integers(1, 4, (shape.C, 2, G)).astype(np.int32)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+    p.set_weights(syn.random_weights(shape, seed=54)[0])
+    p.set_fixed_k(np.ones((shape.C, 2), np.int32))
+    p.set_subarrays(mask, k)
+    p.run_host(ring, scales)
+    res = p.subarray_results()
+    p.close()
+    s = _o(shape)
+    vis = orc.correlate(s, orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE))
+    for c in range(shape.C):
+        for pol in range(2):
+            A = orc.covariance(shape.R, 2, vis[c], pol)
+            for g in range(G):
+                q = (c * 2 + pol) * G + g
+                ant = np.flatnonzero(mask[c, g])
+                n = len(ant)
+                assert res["sizes"][q] == n and res["counts"][q] == k[c, pol, g]
+                sub = A[np.ix_(ant, ant)]
+                dd = np.sqrt(np.maximum(sub.diagonal().real, 1e-10))
+                w, V = np.linalg.eigh(sub / np.outer(dd, dd))
+                assert np.max(np.abs(res["evals"][q, :n] - w)) < 1e-4 * np.max(np.abs(w))
+                K = k[c, pol, g]
+                U = dd[:, None] * V[:, n - K:]
+                U /= np.linalg.norm(U, axis=0)
+                Pref = np.eye(n) - U @ U.conj().T
+                Pg = res["proj"][q, :n * n].reshape(n, n).T  # column-major
+                assert np.max(np.abs(Pg - Pref)) < 1e-3, (c, pol, g, n)
+                # I - sum u_k u_k^H with unwhitened unit vectors (buildIdentityMinusProjection,
+                # spatial.cuh:993-1034): Hermitian, a projector only for K = 1
+                assert np.max(np.abs(Pg - Pg.conj().T)) < 1e-5
+                if K == 1:
+                    assert np.max(np.abs(Pg @ Pg - Pg)) < 1e-4
