@@ -269,6 +269,70 @@ def run_precision(args):
     return out
 
 
+def run_subarrays(args):
+    """Uneven batching: 384 ch x 2 pol x 2 subarray groups of 12, 24 or 36
+    antennas (1536 eigenproblems per block), 1-3 CW interferers per channel at
+    INR 10-30 dB, 8192 samples.  The per-block sequence (correlator, eigen /
+    null stage, persistent subarray launch, 2B beams) runs from its hipGraph;
+    the autotuner picks the launch shapes first.  Reports the graph-replayed
+    block time and the persistent launch's time and problem rate."""
+    C, G = args.channels, 2
+    shape = syn.BlockShape(C=C, F=1, NT=64, R=36, P=2, RPP=36, NP=128, B=16, blocks_to_accumulate=4)
+    seed = 20261203 + 4 + 100
+    sc = syn.DriftScenario(C, 36, seed=seed, n_blocks=1, n_src=(1, 3), inr_db=(10.0, 30.0), drift=0.0,
+                           p_switch=0.0, max_sources=3)
+    tones, st, gain, _, act = sc.block(0)
+    dr = lam.DeviceArray(shape.ring_shape(), np.int8)
+    ds = lam.DeviceArray(shape.scale_shape(), np.int16)
+    dt, dst, dg = (lam.DeviceArray.from_numpy(a) for a in (tones, st, gain))
+    lam.synth_block(shape, seed, 0, 16.0, dt, dst, dg, 3, dr, ds)
+    rng = np.random.default_rng(seed)
+    mask = np.zeros((C, G, 36), np.uint8)
+    sizes = rng.choice([12, 24, 36], (C, G))
+    for c in range(C):
+        for g in range(G):
+            mask[c, g, rng.choice(36, sizes[c, g], replace=False)] = 1
+    k = np.repeat(act.sum(1).astype(np.int32)[:, None, None], 2, axis=1).repeat(G, axis=2)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+    p.set_weights(syn.random_weights(shape, seed=seed + 1)[0])
+    p.set_fixed_k(act.sum(1).astype(np.int32)[:, None].repeat(2, axis=1))
+    p.set_subarrays(mask, k)
+    tuned = None if args.no_autotune else p.autotune([dr], [ds], iters=2)
+    p.set_graphs(True)
+    p.set_profiling(False)
+    for i in range(3):
+        p.execute(dr, ds, on_device=True)
+    p.sync()
+    p.dump_visibilities_i64()
+    nblk = max(8, args.steps)
+    t0 = time.perf_counter()
+    for i in range(nblk):
+        p.execute(dr, ds, on_device=True)
+        if (i + 1) % 4 == 0:
+            p.dump_visibilities_i64()
+    p.sync()
+    graph_ms = (time.perf_counter() - t0) / nblk * 1e3
+    p.set_profiling(True)
+    p.kernel_times()
+    for i in range(5):
+        p.execute(dr, ds, on_device=True)
+    kt = p.kernel_times()
+    res = p.subarray_results()
+    p.close()
+    sub_ms = kt["subarray_eig"][0] / kt["subarray_eig"][1]
+    nprob = C * 2 * G
+    return dict(workload=f"{C} ch x 2 pol x {G} groups of 12/24/36 antennas ({nprob} eigenproblems), "
+                         f"1-3 CW interferers at INR 10-30 dB, {shape.T} samples/block",
+                graph_ms_per_block=round(graph_ms, 4),
+                real_time_factor=round((shape.T / CHANNEL_RATE_HZ) / (graph_ms * 1e-3), 3),
+                subarray_launch_us=round(sub_ms * 1e3, 1),
+                eigenproblems_per_s=round(nprob / (sub_ms * 1e-3), 1),
+                mean_sweeps=round(float(np.mean(res["sweeps"])), 2),
+                sizes_hist={int(n): int(np.sum(res["sizes"] == n)) for n in (12, 24, 36)},
+                kernels_us={kk: round(v[0] / v[1] * 1e3, 1) for kk, v in kt.items()},
+                autotune=tuned)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,6 +351,7 @@ def main():
     ap.add_argument("--refine", type=int, default=2, help="warm-start refinement steps")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-subarrays", action="store_true")
     ap.add_argument("--spw", type=int, default=0, help="samples per voltage workgroup (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -388,6 +453,11 @@ def main():
     if not args.no_precision:
         precision = run_precision(args)
 
+    subarrays = None
+    if not args.no_subarrays:
+        subarrays = run_subarrays(args)
+        log(f"subarrays: {subarrays}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -419,6 +489,7 @@ def main():
         "autotune": tuned,
         "learned_vs_plain": learned,
         "precision_sweep": precision,
+        "uneven_subarrays": subarrays,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

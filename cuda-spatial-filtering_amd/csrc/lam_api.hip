@@ -106,7 +106,7 @@ struct lam_ctx {
   int wprec = kWprecF16HiLo;
   int num_cus = 256;
   // subarrays of uneven size (persistent eigen launch)
-  int sub_G = 0;
+  int sub_G = 0, sub_max_n = 0;
   uint8_t *d_sub_mask = nullptr;
   int *d_sub_k = nullptr, *d_sub_order = nullptr, *d_sub_counter = nullptr;
   float *d_sub_evals = nullptr, *d_sub_evecs = nullptr, *d_sub_proj = nullptr;
@@ -330,6 +330,7 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
     q.k = ctx->d_sub_k;
     q.order = ctx->d_sub_order;
     q.nprob = ctx->C * 2 * ctx->sub_G;
+    q.max_n = ctx->sub_max_n;
     q.max_sweeps = ctx->max_sweeps;
     q.tol = ctx->tol;
     q.counter = ctx->d_sub_counter;
@@ -340,7 +341,7 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
     q.sweeps = ctx->d_sub_sweeps;
     q.sizes = ctx->d_sub_sizes;
     Prof pr(ctx, s, LAM_K_SUBARRAY);
-    HIPC(ctx, launch_subarray_null(q, ctx->num_cus * 2, s));
+    HIPC(ctx, launch_subarray_null(q, ctx->num_cus, s));
   }
   if (capture) {  // labels: this block's plain-Jacobi vectors and mode counts
     const size_t nu = (size_t)L.KW * ctx->R;
@@ -1163,6 +1164,7 @@ int lam_set_subarrays(lam_ctx *ctx, int ngroups, const uint8_t *mask, const int3
   HIPC(ctx, hipMemcpy(ctx->d_sub_k, k, (size_t)np * sizeof(int), hipMemcpyHostToDevice));
   HIPC(ctx, hipMemcpy(ctx->d_sub_order, order.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
   ctx->sub_G = G;
+  ctx->sub_max_n = std::max(1, *std::max_element(size.begin(), size.end()));
   return LAM_OK;
 }
 

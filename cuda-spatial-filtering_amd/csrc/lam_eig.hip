@@ -1065,15 +1065,17 @@ __global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P)
   }
 }
 
-hipError_t launch_subarray_null(const SubParams &p, int max_wgs, hipStream_t s) {
-  const size_t sm = eig_smem_bytes(64, 0, false);
+hipError_t launch_subarray_null(const SubParams &p, int num_cus, hipStream_t s) {
+  // LDS for the largest group; as many workgroups as fit on the machine at once
+  const size_t sm = eig_smem_bytes(p.max_n, 0, false);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(subarray_null_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
   }
   if (hipError_t e = launch_zero(p.counter, sizeof(int), s)) return e;
-  const int grid = std::max(1, std::min(p.nprob, max_wgs));
+  const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / (sm + 1024))));
+  const int grid = std::max(1, std::min(p.nprob, num_cus * per_cu));
   hipLaunchKernelGGL(subarray_null_kernel, dim3(grid), dim3(kEigThreads), sm, s, p);
   return hipGetLastError();
 }

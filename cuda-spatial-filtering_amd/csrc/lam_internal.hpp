@@ -112,6 +112,7 @@ struct SubParams {
   const int *k;            // [C*2*G] modes to remove
   const int *order;        // [nprob] problem ids, largest first
   int nprob;               // C * 2 * G
+  int max_n;               // largest group
   int max_sweeps;
   float tol;
   int *counter;            // work-queue counter (cleared by the launcher)
