@@ -150,13 +150,15 @@ def run_learned(args):
         p.set_eig_method(method, refine_sweeps=steps, residual_tol=0.1)
         p.set_profiling(True)
         p.kernel_times()
-        depth, fb = [], []
+        depth, fb, orth = [], [], []
         for b in range(ntr, nb):
             stc, act = gen(b)
             p.execute(dr, ds, evecs_out=evecs, counts_out=counts, on_device=True)
             p.sync()
+            _, fbb, orb = p.eig_stats(with_orthogonality=True)
+            orth.append(float(orb.max()))
             if method != lam.EIG_PLAIN:
-                fb.append(p.eig_stats()[1].mean())
+                fb.append(fbb.mean())
             U = evecs[..., 0] + 1j * evecs[..., 1]
             for c in range(C):
                 a = stc[c][act[c]]
@@ -173,6 +175,7 @@ def run_learned(args):
                              autotune=tuned,
                              kernels_us={k: round(v, 1) for k, v in per.items()},
                              fallback_rate=float(np.mean(fb)) if fb else 0.0,
+                             null_basis_orthogonality_fp64_max=max(orth),
                              depth=np.array(depth))
         log(f"learned: {name}: eig stage {stage_us:.1f} us")
     p.close()
@@ -410,6 +413,9 @@ def main():
     p.sync()
     t1 = time.perf_counter()
     kt = p.kernel_times()
+    orth_max = None
+    if mode == lam.MODE_ADAPTIVE:
+        orth_max = float(p.eig_stats(with_orthogonality=True)[2].max())
     p.close()
 
     sec = (t1 - t0) / args.steps
@@ -484,6 +490,7 @@ def main():
                    "eig": "plain"},
         "real_time_factor": rtf,
         "real_time_factor_ref_convention": (shape.T / REF_RATE_HZ) / sec,
+        "null_basis_orthogonality_fp64_max": orth_max,
         "roofline": roofline,
         "kernels": kstats,
         "autotune": tuned,

@@ -39,6 +39,7 @@ struct Buf {
   v8h *d_wop2 = nullptr;        // adaptive: this block's 2B-beam weight operands
   float *d_evals = nullptr, *d_evecs = nullptr;
   int *d_counts = nullptr, *d_sweeps = nullptr, *d_fallback = nullptr, *d_nsig = nullptr;
+  float *d_orth = nullptr;
   hipGraphExec_t graph = nullptr;
   const void *graph_samples = nullptr, *graph_scales = nullptr;
   hipEvent_t done = nullptr;
@@ -290,6 +291,7 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
     e.sweeps = b.d_sweeps;
     e.fallback = b.d_fallback;
     e.nsig = b.d_nsig;
+    e.orth = b.d_orth;
     e.wprec = ctx->wprec;
     e.plain_threads = ctx->eig_plain_threads;
     e.fallback_threads = ctx->eig_fallback_threads;
@@ -673,6 +675,7 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
       chk(dalloc(ctx, &b.d_sweeps, (size_t)ctx->C * 2 * sizeof(int)));
       chk(dalloc(ctx, &b.d_fallback, (size_t)ctx->C * 2 * sizeof(int)));
       chk(dalloc(ctx, &b.d_nsig, (size_t)ctx->C * 2 * sizeof(int)));
+      chk(dalloc(ctx, &b.d_orth, (size_t)ctx->C * 2 * sizeof(float)));
     }
   }
   chk(dalloc(ctx, &ctx->d_vis_acc, vis_elems * sizeof(long long)));
@@ -729,6 +732,7 @@ int lam_destroy(lam_ctx *ctx) {
     hipFree(b.d_sweeps);
     hipFree(b.d_fallback);
     hipFree(b.d_nsig);
+    hipFree(b.d_orth);
     if (b.stream) hipStreamDestroy(b.stream);
   }
   hipFree(ctx->d_vis_acc);
@@ -1200,7 +1204,7 @@ int lam_set_eig_method(lam_ctx *ctx, int method, int refine_sweeps, int power, f
   return LAM_OK;
 }
 
-int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback) {
+int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback, float *orthogonality) {
   if (!ctx) return LAM_ERR_ARG;
   if (ctx->mode != LAM_MODE_ADAPTIVE || ctx->last_buf < 0) return fail(ctx, LAM_ERR_STATE, "no adaptive block run yet");
   if (int r = sync_all(ctx)) return r;
@@ -1208,6 +1212,7 @@ int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback) {
   const size_t nb = (size_t)ctx->C * 2 * sizeof(int);
   if (sweeps) HIPC(ctx, hipMemcpy(sweeps, b.d_sweeps, nb, hipMemcpyDeviceToHost));
   if (fallback) HIPC(ctx, hipMemcpy(fallback, b.d_fallback, nb, hipMemcpyDeviceToHost));
+  if (orthogonality) HIPC(ctx, hipMemcpy(orthogonality, b.d_orth, nb, hipMemcpyDeviceToHost));
   return LAM_OK;
 }
 

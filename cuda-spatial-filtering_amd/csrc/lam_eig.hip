@@ -38,6 +38,19 @@ __device__ __forceinline__ int rr_pos(int j, int r, int ne) {
 }
 
 __device__ __forceinline__ float wave_sum(float v);
+__device__ __forceinline__ float wave_max(float v);
+// Maximum over the workgroup of non-negative values.
+__device__ __forceinline__ float block_max(float v, float *red) {
+  v = wave_max(v);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  float t = 0.0f;
+  for (int i = 0; i < nw; ++i) t = fmaxf(t, red[i]);
+  __syncthreads();
+  return t;
+}
 __device__ __forceinline__ float block_sum(float v, float *red) {
   v = wave_sum(v);
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -140,7 +153,7 @@ __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_s
   const int ld = n | 1;
   for (int i = tid; i < n * n; i += nthr) {
     const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
-    if (init_v) V[i] = make_float2(r == c ? 1.0f : 0.0f, 0.0f);
+    if (init_v) V[c * ld + r] = make_float2(r == c ? 1.0f : 0.0f, 0.0f);
     if (r == c) A[c * ld + r].y = 0.0f;
   }
   // this thread's blocks (kp <= kq) of the upper triangle, decoded once
@@ -234,10 +247,10 @@ __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_s
         const int2 Q = rpq[k];
         if (Q.y >= n) continue;
         const Rot RQ = rot[k];
-        const float2 vp = V[Q.x * n + row], vq = V[Q.y * n + row];
+        const float2 vp = V[Q.x * ld + row], vq = V[Q.y * ld + row];
         const float2 y = cmul(make_float2(RQ.er, -RQ.ei), vq);
-        V[Q.x * n + row] = make_float2(RQ.c * vp.x - RQ.s * y.x, RQ.c * vp.y - RQ.s * y.y);
-        V[Q.y * n + row] = make_float2(RQ.s * vp.x + RQ.c * y.x, RQ.s * vp.y + RQ.c * y.y);
+        V[Q.x * ld + row] = make_float2(RQ.c * vp.x - RQ.s * y.x, RQ.c * vp.y - RQ.s * y.y);
+        V[Q.y * ld + row] = make_float2(RQ.s * vp.x + RQ.c * y.x, RQ.s * vp.y + RQ.c * y.y);
       }
       __syncthreads();
     }
@@ -289,7 +302,7 @@ __device__ __forceinline__ EigSmem carve(char *base, int n, int B, bool warm) {
   const int ld = n | 1;  // odd column stride of A: conflict-free strided LDS access
   s.A = reinterpret_cast<float2 *>(base);
   s.V = s.A + ld * n;
-  s.W = s.V + n * n;
+  s.W = s.V + ld * n;
   s.G = s.W + 2 * B * n;
   float2 *q = s.G + B * n;
   s.A0 = s.Y = s.hv = s.t1 = s.t2 = nullptr;
@@ -323,7 +336,7 @@ __device__ __forceinline__ EigSmem carve(char *base, int n, int B, bool warm) {
 
 size_t eig_smem_bytes(int n, int B, bool warm) {
   const int ld = n | 1;
-  size_t f2 = (size_t)ld * n + (size_t)n * n + 3 * (size_t)B * n;
+  size_t f2 = 2 * (size_t)ld * n + 3 * (size_t)B * n;
   if (warm) f2 += (3 * B * n >= ld * n ? 0 : (size_t)ld * n) + 2 * kMaxWarm * n + 128;
   return sizeof(float2) * f2 + sizeof(float) * (216 + (warm ? kMaxWarm * 64 : 0)) + sizeof(int) * 128 +
          sizeof(int2) * 32 + sizeof(Rot) * 32 + 16;
@@ -344,6 +357,17 @@ __device__ __forceinline__ float wave_sum(float v) {
   v += dpp_move<0x118, 0xf>(v);  // row_shr:8
   v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
   v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+// Maximum over the 64 lanes of non-negative values (lanes outside a row read
+// 0 in the DPP moves, neutral for non-negative inputs).
+__device__ __forceinline__ float wave_max(float v) {
+  v = fmaxf(v, dpp_move<0x111, 0xf>(v));
+  v = fmaxf(v, dpp_move<0x112, 0xf>(v));
+  v = fmaxf(v, dpp_move<0x114, 0xf>(v));
+  v = fmaxf(v, dpp_move<0x118, 0xf>(v));
+  v = fmaxf(v, dpp_move<0x142, 0xa>(v));
+  v = fmaxf(v, dpp_move<0x143, 0xc>(v));
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
 }
 
@@ -563,7 +587,7 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
       z.x += t2.x;
       z.y += t2.y;
     }
-    S.V[(m + k) * n + i] = u;
+    S.V[(m + k) * ld + i] = u;
     z.x -= th * u.x;
     z.y -= th * u.y;
     S.res[k * 64 + i] = z.x * z.x + z.y * z.y;
@@ -576,7 +600,7 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
   if (tid < 64) {
     const int i = tid;
     for (int k = 0; k < KW; ++k)
-      if (i < n) Y[k * n + i] = S.V[(m + k) * n + i];
+      if (i < n) Y[k * n + i] = S.V[(m + k) * ld + i];
     for (int e = i; e < KW * KW; e += 64) Tm[e] = make_float2(0.f, 0.f);
     wave_sync();
     float2 vk[kMaxWarm];
@@ -655,14 +679,14 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
     S.A0[e] = q;
   }
   __syncthreads();
-  for (int e = tid; e < m * n; e += nthr) S.V[e] = S.A0[e];
+  for (int e = tid; e < m * n; e += nthr) S.V[(e / n) * ld + e % n] = S.A0[e];
   __syncthreads();
   // Rayleigh quotients of the complement: lambda_j = q_j^H A q_j
   for (int e = tid; e < m * n; e += nthr) {  // W = A Q2 into A0
     const int j = e / n, i = e - j * n;
     float2 w = make_float2(0.f, 0.f);
     for (int c2 = 0; c2 < n; ++c2) {
-      const float2 t = cmul(A[c2 * ld + i], S.V[j * n + c2]);
+      const float2 t = cmul(A[c2 * ld + i], S.V[j * ld + c2]);
       w.x += t.x;
       w.y += t.y;
     }
@@ -674,7 +698,7 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
     if (j < m) {
       float lam = 0.0f;
       for (int i = 0; i < n; ++i) {
-        const float2 q = S.V[j * n + i], w = S.A0[j * n + i];
+        const float2 q = S.V[j * ld + i], w = S.A0[j * n + i];
         lam += q.x * w.x + q.y * w.y;
       }
       evu[j] = lam;
@@ -804,16 +828,36 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   if (P.basis_out) {
     for (int e = tid; e < P.KW * n; e += nthr) {
       const int k = e / n, i = e - k * n;
-      P.basis_out[cp * P.KW * n + e] = k < n ? S.V[S.perm[n - 1 - k] * n + i] : make_float2(0.f, 0.f);
+      P.basis_out[cp * P.KW * n + e] = k < n ? S.V[S.perm[n - 1 - k] * ld + i] : make_float2(0.f, 0.f);
     }
   }
   if (tid == 0 && P.fallback) P.fallback[cp] = fell;
+  // FP64 orthogonality check of the null basis: max |u_a^H u_b - delta_ab| over the
+  // K removed (whitened) eigenvectors, one (a, b) pair per thread
+  if (P.orth) {
+    double e = 0.0;
+    for (int ab = tid; ab < K * K; ab += nthr) {
+      const int a = ab / K, b = ab - a * K;
+      const int ca = S.perm[n - 1 - a], cb = S.perm[n - 1 - b];
+      double re = 0.0, im = 0.0;
+#pragma unroll 1
+      for (int i = 0; i < n; ++i) {
+        const float2 x = S.V[ca * ld + i], y = S.V[cb * ld + i];
+        re += (double)x.x * y.x + (double)x.y * y.y;
+        im += (double)x.x * y.y - (double)x.y * y.x;
+      }
+      if (a == b) re -= 1.0;
+      e = fmax(e, sqrt(re * re + im * im));
+    }
+    const float ef = block_max((float)e, S.red);
+    if (tid == 0) P.orth[cp] = ef;
+  }
   // 5. output basis U (sorted columns) into S.A: unwhitened+normalised (null) or whitened (shrink)
   if (tid < n) {
     const int src = S.perm[tid];
     float nrm = 0.0f;
     for (int i = 0; i < n; ++i) {
-      float2 v = S.V[src * n + i];
+      float2 v = S.V[src * ld + i];
       if (!P.shrink) {
         v.x *= S.d[i];
         v.y *= S.d[i];
@@ -1028,7 +1072,7 @@ __global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P)
       const int src = S.perm[tid];
       float nrm = 0.0f;
       for (int i = 0; i < n; ++i) {
-        float2 v = S.V[src * n + i];
+        float2 v = S.V[src * ld + i];
         v.x *= S.d[i];
         v.y *= S.d[i];
         S.A[tid * n + i] = v;
@@ -1105,7 +1149,7 @@ eigh_batched_kernel(const float2 *Ain, int n, int whiten, int max_sweeps, float 
   if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
   for (int i = tid; i < n * n; i += blockDim.x) {
     const int k = i / n, r = i % n;
-    evecs[(long)m * n * n + i] = S.V[S.perm[k] * n + r];
+    evecs[(long)m * n * n + i] = S.V[S.perm[k] * ld + r];
   }
   if (tid == 0 && sweeps_out) sweeps_out[m] = sw;
 }
@@ -1126,7 +1170,7 @@ eigh_warm_kernel(const float2 *Ain, const float2 *prop, int n, int KW, int K, in
   if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
   for (int i = tid; i < n * n; i += blockDim.x) {
     const int k = i / n, r = i % n;
-    evecs[(long)m * n * n + i] = S.V[S.perm[k] * n + r];
+    evecs[(long)m * n * n + i] = S.V[S.perm[k] * ld + r];
   }
   if (tid == 0) ratio[m] = S.red[1];
 }

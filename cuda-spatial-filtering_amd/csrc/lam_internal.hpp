@@ -97,6 +97,7 @@ struct EigParams {
   float2 *basis_out;       // out: [C*2][KW][R] top-KW whitened eigenvectors, or nullptr
   int *fallback;           // out: [C*2] 1 where the plain solver ran after the check failed
   int *nsig;               // out: [C*2] modes above 4x the noise floor among the K removed, or nullptr
+  float *orth;             // out: [C*2] FP64 check max |U_K^H U_K - I| of the removed (whitened) basis
   int plain_threads;       // workgroup width of the plain solver (256, 512, 1024)
   int fallback_threads;    // workgroup width of the deferred fallback launch
   int only_failed;         // set by launch_eig for the second launch of the warm start

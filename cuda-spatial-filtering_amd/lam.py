@@ -108,7 +108,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_eigh_batched.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_float, vp, vp, vp, vp]
     L.lam_set_eig_method.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float]
-    L.lam_eig_stats.argtypes = [vp, vp, vp]
+    L.lam_eig_stats.argtypes = [vp, vp, vp, vp]
     L.lam_set_eig_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.lam_set_weight_precision.argtypes = [vp, ctypes.c_int]
     L.lam_set_subarrays.argtypes = [vp, ctypes.c_int, vp, vp]
@@ -370,12 +370,13 @@ class Pipeline:
     def set_eig_tuning(self, plain_threads=256, fallback_threads=1024):
         self._chk(self.L.lam_set_eig_tuning(self.h, plain_threads, fallback_threads))
 
-    def eig_stats(self):
+    def eig_stats(self, with_orthogonality=False):
         s = self.shape
         sw = np.zeros((s.C, 2), np.int32)
         fb = np.zeros((s.C, 2), np.int32)
-        self._chk(self.L.lam_eig_stats(self.h, sw.ctypes.data, fb.ctypes.data))
-        return sw, fb
+        orth = np.zeros((s.C, 2), np.float32)
+        self._chk(self.L.lam_eig_stats(self.h, sw.ctypes.data, fb.ctypes.data, orth.ctypes.data))
+        return (sw, fb, orth) if with_orthogonality else (sw, fb)
 
     def learn_init(self, warm_vectors=4, hidden=256, seed=1):
         self._chk(self.L.lam_learn_init(self.h, warm_vectors, hidden, seed))

@@ -242,8 +242,10 @@ int lam_device_count(void);
 enum { LAM_EIG_PLAIN = 0, LAM_EIG_LEARNED = 1, LAM_EIG_PREVIOUS = 2 };
 int lam_set_eig_method(lam_ctx *ctx, int method, int refine_sweeps, int power, float residual_tol);
 /* Per (channel, pol) of the most recent block: Jacobi sweeps run (refinement
- * plus fallback) and whether the plain fallback ran.  Synchronous. */
-int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback);
+ * plus fallback), whether the plain fallback ran, and the FP64 orthogonality
+ * check max |u_a^H u_b - delta_ab| of the K removed whitened eigenvectors.
+ * Any pointer may be NULL.  Synchronous. */
+int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback, float *orthogonality);
 
 /* Subarrays (no counterpart in the reference): ngroups antenna groups per
  * channel, mask[C][ngroups][R] (non-zero = member), k[C][2][ngroups] modes to

@@ -161,7 +161,10 @@ def test_learned_pipeline_trains_and_matches_plain_null():
             stc, act = gen(b)
             p.execute(dr, ds, evecs_out=evecs, counts_out=counts, on_device=True)
             p.sync()
-            fb.append(p.eig_stats()[1].mean())
+            _, fbb, orth = p.eig_stats(with_orthogonality=True)
+            fb.append(fbb.mean())
+            # FP64 check of the removed basis (warm Ritz vectors or plain Jacobi)
+            assert orth.max() < 1e-4, (name, b, float(orth.max()))
             U = evecs[..., 0] + 1j * evecs[..., 1]
             for c in range(C):
                 a = stc[c][act[c]]

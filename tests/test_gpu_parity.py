@@ -335,10 +335,18 @@ def test_adaptive_scene_vs_oracle(shrink):
     p.set_eig_params(20, 1e-7)
     out = p.run_host(ring, scales)
     vis = p.dump_visibilities_i64()
+    orth = p.eig_stats(with_orthogonality=True)[2]
     p.close()
     s = _o(shape)
     y = orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)
     ref = orc.adaptive_block(s, y, w, fixed_k=np.full((shape.C, 2), 2), shrink=shrink)
+    # FP64 orthogonality check of the removed basis, max |U^H U - I|
+    assert orth.max() < 1e-5
+    if shrink:  # shrink mode returns the whitened vectors themselves: recompute on the CPU
+        V = out["evecs_out"][..., 0].astype(np.float64) + 1j * out["evecs_out"][..., 1]
+        U = V[:, :, -2:, :]
+        G = np.einsum("cpai,cpbi->cpab", U.conj(), U) - np.eye(2)
+        assert np.allclose(orth, np.abs(G).max(axis=(2, 3)), rtol=1e-3, atol=1e-10)
     assert np.array_equal(vis, ref["vis"])
     evg = out["evals_out"]
     assert np.max(np.abs(evg - ref["evals"]) / np.max(np.abs(ref["evals"]), axis=-1, keepdims=True)) < 1e-4
