@@ -40,9 +40,10 @@ struct Buf {
   float *d_evals = nullptr, *d_evecs = nullptr;
   int *d_counts = nullptr, *d_sweeps = nullptr, *d_fallback = nullptr, *d_nsig = nullptr;
   float *d_orth = nullptr;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // one per block phase
   const void *graph_samples = nullptr, *graph_scales = nullptr;
   hipEvent_t done = nullptr;
+  hipEvent_t eig_done = nullptr;  // the eigen stage of this buffer's latest block
 };
 
 // Learned warm start: MLP parameters, Adam state, per-block buffers and the
@@ -212,12 +213,25 @@ Skip skip_of(const Learn &L, const uint16_t *X) {
   return k;
 }
 
-// The per-block kernel sequence of one buffer (captured into a graph when enabled).
-int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
+// Phases of an adaptive block: 0 = correlate, 1 = eigen stage (features, MLP,
+// eigensolvers, subarrays, label capture), 2 = 2B beams + visibility
+// accumulation.  Consecutive blocks on different buffers overlap; only the
+// eigen stages that share state across blocks (warm-start basis, MLP scratch,
+// subarray outputs) are ordered, phase 1 after the previous block's phase 1.
+constexpr int kPhases = 3;
+
+bool eig_chained(const lam_ctx *ctx) {
+  return ctx->mode == LAM_MODE_ADAPTIVE && (ctx->lr.ready || ctx->sub_G > 0);
+}
+
+// One phase of the per-block kernel sequence of one buffer (captured into a
+// graph per phase when enabled).  The correlator-beamformer mode runs in phase 0.
+int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales, int phase) {
   hipStream_t s = b.stream;
   const long long vis_count = (long long)ctx->C * ctx->NBL * 8;
   const bool atomic_acc = ctx->nbuf > 1;
   if (ctx->mode == LAM_MODE_CORR_BEAM) {
+    if (phase != 0) return LAM_OK;
     VoltParams p = base_volt(ctx, samples, scales);
     p.window_base = 0;  // ring slots 0..NP-1 (spatial.cuh:254-264)
     p.vis_part = b.d_vis_part;
@@ -238,11 +252,25 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
   p.delays = ctx->delays_set ? ctx->d_delays : nullptr;
   p.recv_perm = ctx->perm_set ? ctx->d_recv_perm : nullptr;
   p.pol_perm = ctx->perm_set ? ctx->d_pol_perm : nullptr;
-  {
+  if (phase == 0) {
     VoltParams pc = p;
     pc.vis_part = b.d_vis_part;
     Prof pr(ctx, s, LAM_K_CORR);
     HIPC(ctx, launch_volt(pc, true, false, s));
+    return LAM_OK;
+  }
+  if (phase == 2) {
+    VoltParams pb = p;
+    pb.wop = b.d_wop2;
+    pb.nbeams_out = 2 * ctx->B;
+    pb.NMT = ctx->NMT2;
+    pb.beams = b.d_beams;
+    {
+      Prof pr(ctx, s, LAM_K_BEAM2B);
+      HIPC(ctx, launch_volt(pb, false, true, s));
+    }
+    HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, ctx->d_vis_acc, atomic_acc, s));
+    return LAM_OK;
   }
   const int M = ctx->C * 2;
   Learn &L = ctx->lr;
@@ -353,40 +381,32 @@ int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scale
                              hipMemcpyDeviceToDevice, s));
     L.cap_count += 1;
   }
-  {
-    VoltParams pb = p;
-    pb.wop = b.d_wop2;
-    pb.nbeams_out = 2 * ctx->B;
-    pb.NMT = ctx->NMT2;
-    pb.beams = b.d_beams;
-    Prof pr(ctx, s, LAM_K_BEAM2B);
-    HIPC(ctx, launch_volt(pb, false, true, s));
-  }
-  HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, ctx->d_vis_acc, atomic_acc, s));
   return LAM_OK;
 }
 
-int run_block(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
-  const Learn &L = ctx->lr;
-  const bool capturing = L.ready && ctx->eig_method == LAM_EIG_PLAIN && L.cap_count < L.cap_max;
-  const bool graphs = ctx->use_graphs && !ctx->profiling && !capturing;
-  if (!graphs) return enqueue_kernels(ctx, b, samples, scales);
-  if (ctx->graphs_dirty) {
-    for (auto &bb : ctx->bufs)
-      if (bb.graph) {
-        hipGraphExecDestroy(bb.graph);
-        bb.graph = nullptr;
-      }
-    ctx->graphs_dirty = false;
-  }
-  if (b.graph && (b.graph_samples != samples || b.graph_scales != scales)) {
-    hipGraphExecDestroy(b.graph);
-    b.graph = nullptr;
-  }
-  if (!b.graph) {
+// The whole per-block sequence on one stream (autotuner timing, no graphs).
+int enqueue_kernels(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
+  for (int ph = 0; ph < kPhases; ++ph)
+    if (int r = enqueue_phase(ctx, b, samples, scales, ph)) return r;
+  return LAM_OK;
+}
+
+void destroy_graphs(Buf &b) {
+  for (auto &g : b.graph)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// One phase, replayed from a graph captured on first use when graphs are on.
+int run_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales, int phase, bool graphs) {
+  if (!graphs) return enqueue_phase(ctx, b, samples, scales, phase);
+  hipGraphExec_t &gx = b.graph[phase];
+  if (!gx) {
     hipGraph_t g = nullptr;
     HIPC(ctx, hipStreamBeginCapture(b.stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_kernels(ctx, b, samples, scales);
+    const int rc = enqueue_phase(ctx, b, samples, scales, phase);
     const hipError_t ec = hipStreamEndCapture(b.stream, &g);
     if (rc != LAM_OK || ec != hipSuccess || !g) {
       if (g) hipGraphDestroy(g);
@@ -394,7 +414,7 @@ int run_block(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
       // capture failure falls back to eager launches
       // (lambda_corr_beam_only_gpu_pipeline.hpp:346-383, 561-586)
       ctx->use_graphs = false;
-      return enqueue_kernels(ctx, b, samples, scales);
+      return enqueue_phase(ctx, b, samples, scales, phase);
     }
     hipGraphExec_t ex = nullptr;
     const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -402,13 +422,42 @@ int run_block(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
     if (ei != hipSuccess || !ex) {
       (void)hipGetLastError();
       ctx->use_graphs = false;
-      return enqueue_kernels(ctx, b, samples, scales);
+      return enqueue_phase(ctx, b, samples, scales, phase);
     }
-    b.graph = ex;
-    b.graph_samples = samples;
-    b.graph_scales = scales;
+    gx = ex;
   }
-  HIPC(ctx, hipGraphLaunch(b.graph, b.stream));
+  HIPC(ctx, hipGraphLaunch(gx, b.stream));
+  return LAM_OK;
+}
+
+// One block: correlate; [after the previous block's eigen stage] eigen stage;
+// 2B beams.  Blocks on different buffers run concurrently where they may.
+int run_block(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
+  const Learn &L = ctx->lr;
+  const bool capturing = L.ready && ctx->eig_method == LAM_EIG_PLAIN && L.cap_count < L.cap_max;
+  const bool graphs = ctx->use_graphs && !ctx->profiling && !capturing;
+  if (graphs) {
+    // executable graphs are destroyed only once their last replay has finished
+    if (ctx->graphs_dirty) {
+      for (auto &bb : ctx->bufs) {
+        HIPC(ctx, hipStreamSynchronize(bb.stream));
+        destroy_graphs(bb);
+      }
+      ctx->graphs_dirty = false;
+    }
+    if (b.graph_samples != samples || b.graph_scales != scales) {
+      HIPC(ctx, hipStreamSynchronize(b.stream));
+      destroy_graphs(b);
+      b.graph_samples = samples;
+      b.graph_scales = scales;
+    }
+  }
+  const bool chain = eig_chained(ctx) && ctx->last_buf >= 0 && ctx->last_buf != ctx->cur;
+  for (int ph = 0; ph < kPhases; ++ph) {
+    if (ph == 1 && chain) HIPC(ctx, hipStreamWaitEvent(b.stream, ctx->bufs[ctx->last_buf].eig_done, 0));
+    if (int r = run_phase(ctx, b, samples, scales, ph, graphs && ctx->use_graphs)) return r;
+    if (ph == 1) HIPC(ctx, hipEventRecord(b.eig_done, b.stream));
+  }
   return LAM_OK;
 }
 
@@ -663,6 +712,7 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
   for (auto &b : ctx->bufs) {
     if (hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking) != hipSuccess) rc = LAM_ERR_HIP;
     if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) rc = LAM_ERR_HIP;
+    if (hipEventCreateWithFlags(&b.eig_done, hipEventDisableTiming) != hipSuccess) rc = LAM_ERR_HIP;
     chk(dalloc(ctx, &b.d_samples, ctx->samples_bytes));
     chk(dalloc(ctx, &b.d_scales, ctx->scales_bytes));
     chk(dalloc(ctx, &b.d_beams, ctx->beams_bytes));
@@ -719,8 +769,9 @@ int lam_destroy(lam_ctx *ctx) {
     hipEventDestroy(e.stop);
   }
   for (auto &b : ctx->bufs) {
-    if (b.graph) hipGraphExecDestroy(b.graph);
+    destroy_graphs(b);
     if (b.done) hipEventDestroy(b.done);
+    if (b.eig_done) hipEventDestroy(b.eig_done);
     hipFree(b.d_samples);
     hipFree(b.d_scales);
     hipFree(b.d_beams);
@@ -920,9 +971,6 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
   }
   // never accumulate into a visibility buffer that a dump has not reset yet
   HIPC(ctx, hipStreamWaitEvent(s, ctx->reset_done, 0));
-  // the warm-start chain (and the capture counter) orders consecutive blocks
-  if (ctx->lr.ready && ctx->last_buf >= 0 && ctx->last_buf != ctx->cur)
-    HIPC(ctx, hipStreamWaitEvent(s, ctx->bufs[ctx->last_buf].done, 0));
   if (int r = run_block(ctx, b, smp, scl)) return r;
   if (io->inputs_on_device && io->release_fn)
     HIPC(ctx, hipLaunchHostFunc(s, io->release_fn, io->release_user));

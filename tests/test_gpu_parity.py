@@ -483,3 +483,64 @@ def test_subarrays_uneven_sizes_persistent():
                 assert np.max(np.abs(Pg - Pg.conj().T)) < 1e-5
                 if K == 1:
                     assert np.max(np.abs(Pg @ Pg - Pg)) < 1e-4
+
+
+def test_overlapped_buffers_match_serial():
+    """Three buffers with graph replay (blocks overlap: the correlator and the
+    2B beams of neighbouring blocks run beside each other's eigen stages, only
+    the eigen stages are chained) against one buffer run eagerly: the same
+    beams, eigenpairs, subarray results and accumulated visibilities, bit for
+    bit, over a warm-started block sequence (pipeline/common.hpp:388-396,
+    lambda_adaptive_beamformed_spectra_pipeline.hpp:820-880)."""
+    shape = syn.BlockShape(C=8, F=1, NT=64, R=36, P=2, RPP=36, NP=16, B=4)
+    nblk = 7
+    blocks = []
+    for i in range(nblk):
+        scene = syn.make_scene(shape, seed=61 + i, n_tones=2, amp=30.0)
+        blocks.append(syn.make_block(shape, scene, seed=161 + i))
+    dev = [(lam.DeviceArray.from_numpy(r), lam.DeviceArray.from_numpy(s)) for r, s in blocks]
+    rng = np.random.default_rng(62)
+    G = 2
+    mask = np.zeros((shape.C, G, shape.R), np.uint8)
+    for c in range(shape.C):
+        for g in range(G):
+            mask[c, g, rng.choice(shape.R, rng.choice([12, 24, 36]), replace=False)] = 1
+    sub_k = rng.integers(1, 3, (shape.C, 2, G)).astype(np.int32)
+    w16 = syn.random_weights(shape, seed=63)[0]
+
+    def run(nbuf, graphs):
+        _log(f"overlap run nbuf={nbuf} graphs={graphs}")
+        p = lam.Pipeline(shape, lam.MODE_ADAPTIVE, num_buffers=nbuf)
+        p.set_weights(w16)
+        p.set_fixed_k(np.full((shape.C, 2), 2, np.int32))
+        p.set_subarrays(mask, sub_k)
+        p.learn_init(8, 256, seed=4)
+        p.set_graphs(graphs)
+        outs = []
+        for i in range(nblk):
+            if i == 1:
+                p.set_eig_method(lam.EIG_PREVIOUS, refine_sweeps=2)
+            beams = p.beams_array()
+            ev = np.zeros((shape.C, 2, shape.R), np.float32)
+            vec = np.zeros((shape.C, 2, shape.R, shape.R, 2), np.float32)
+            cnt = np.zeros((shape.C, 2), np.int32)
+            p.execute(dev[i][0], dev[i][1], beams_out=beams, evals_out=ev, evecs_out=vec,
+                      counts_out=cnt, on_device=True)
+            outs.append((beams, ev, vec, cnt))
+        p.sync()
+        sub = p.subarray_results()
+        vis = p.dump_visibilities_i64()
+        p.close()
+        return outs, sub, vis
+
+    o1, s1, v1 = run(1, False)
+    o3, s3, v3 = run(3, True)
+    assert np.array_equal(v1, v3)
+    for i in range(nblk):
+        for a, b in zip(o1[i], o3[i]):
+            assert np.array_equal(a, b), i
+    for key in s1:
+        assert np.array_equal(s1[key], s3[key]), key
+    s = _o(shape)
+    ref = sum(orc.correlate(s, orc.ingest(s, r, sc, orc.WINDOW_ADAPTIVE)) for r, sc in blocks)
+    assert np.array_equal(v3, ref)
