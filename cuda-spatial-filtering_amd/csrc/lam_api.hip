@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -645,6 +646,37 @@ int time_sequence(lam_ctx *ctx, const void *const *smp, const void *const *scl, 
   return LAM_OK;
 }
 
+// Streaming milliseconds per block: the blocks run back to back through all
+// buffers as lam_execute runs them (eager launches, consecutive blocks
+// overlapping), `iters` passes after one untimed pass.
+int time_stream(lam_ctx *ctx, const void *const *smp, const void *const *scl, int nblocks, int iters,
+                double &out) {
+  const bool prof = ctx->profiling, graphs = ctx->use_graphs;
+  ctx->profiling = false;
+  ctx->use_graphs = false;
+  int rc = LAM_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int pass = 0; pass <= iters && rc == LAM_OK; ++pass) {
+    if (pass == 1) {
+      if ((rc = sync_all(ctx))) break;
+      t0 = std::chrono::steady_clock::now();
+    }
+    for (int i = 0; i < nblocks && rc == LAM_OK; ++i) {
+      Buf &b = ctx->bufs[ctx->cur];
+      if ((rc = run_block(ctx, b, smp[i], scl[i]))) break;
+      if (hipEventRecord(b.done, b.stream) != hipSuccess) rc = fail(ctx, LAM_ERR_HIP, "autotune: event record");
+      ctx->last_buf = ctx->cur;
+      ctx->cur = (ctx->cur + 1) % ctx->nbuf;
+    }
+  }
+  if (rc == LAM_OK) rc = sync_all(ctx);
+  const auto t1 = std::chrono::steady_clock::now();
+  out = std::chrono::duration<double, std::milli>(t1 - t0).count() / ((double)iters * nblocks);
+  ctx->profiling = prof;
+  ctx->use_graphs = graphs;
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1127,10 +1159,15 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
     ctx->eig_method = LAM_EIG_PLAIN;
     int best = ctx->eig_plain_threads;
     double best_t = 1e30;
+    // with several buffers the eigen stage shares the GPU with the voltage
+    // kernels of neighbouring blocks: judge by streaming time per block
+    const bool stream = ctx->nbuf > 1;
     for (int th : {256, 512, 1024}) {
       ctx->eig_plain_threads = th;
       double t = 0.0;
-      if ((rc = time_sequence(ctx, samples, scales, nblocks, iters, {LAM_K_EIG}, t))) break;
+      if ((rc = stream ? time_stream(ctx, samples, scales, nblocks, iters, t)
+                       : time_sequence(ctx, samples, scales, nblocks, iters, {LAM_K_EIG}, t)))
+        break;
       if (t < best_t) {
         best_t = t;
         best = th;
@@ -1146,8 +1183,9 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
       for (int st : {1, 2, 3}) {
         ctx->refine_sweeps = st;
         double t = 0.0;
-        if ((rc = time_sequence(ctx, samples, scales, nblocks, iters,
-                                {LAM_K_EIG, LAM_K_EIGFB, LAM_K_FEAT, LAM_K_MLP}, t)))
+        if ((rc = stream ? time_stream(ctx, samples, scales, nblocks, iters, t)
+                         : time_sequence(ctx, samples, scales, nblocks, iters,
+                                         {LAM_K_EIG, LAM_K_EIGFB, LAM_K_FEAT, LAM_K_MLP}, t)))
           break;
         if (t < bt) {
           bt = t;

@@ -86,6 +86,20 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
 }
 
+// Workgroup barrier for the chunk loop, without the release fence of
+// __syncthreads: that fence makes every wave drain vmcnt, which would stall
+// the loader wave on the chunk it just prefetched and the beam waves on their
+// global stores.  LDS hazards are covered explicitly: the loader waits for its
+// LDS-DMA chunk with s_waitcnt vmcnt before arriving, and every wave drains
+// lgkmcnt (its own LDS reads and writes) here.  The empty asm statements keep
+// the compiler from moving memory accesses across the barrier.
+__device__ __forceinline__ void chunk_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt, expcnt unconstrained
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // ------------------------------------------------------------- geometry
 template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT, bool LOADER_WAVE, bool TIGHT = false>
 struct Geo {
@@ -190,56 +204,167 @@ __device__ __forceinline__ void corr_store(const VoltParams &P, const v4i acc[12
 }
 
 // ------------------------------------------------------------ beamformer
-// One chunk of beams for (pol, 16-beam tile mt): 4 time tiles of 32 samples.
-template <int RSD, int NK>
-__device__ __forceinline__ void beam_chunk(const VoltParams &P, const uint32_t *L, const v8h wop[2 * NK],
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+
+// N (<= 4) dwordx4 beam operands at LDS byte address a, a + 32, ...: the
+// dwords of 4 antennas x both pols for consecutive K steps.  The reads and
+// their wait sit in one asm block.  Read through C++, the compiler would drain
+// vmcnt before them (they may alias the loader wave's LDS-DMA writes), and for
+// a beam wave vmcnt counts its own pending beam stores; the loader's vmcnt
+// wait plus the chunk barrier already order the DMA before these reads.
+template <int N>
+__device__ __forceinline__ void lds_read_ops(uint32_t a, v4u *d) {
+  static_assert(N >= 1 && N <= 4, "group of K steps");
+  if constexpr (N == 1) {
+    asm volatile("ds_read_b128 %0, %1\n\ts_waitcnt lgkmcnt(0)" : "=&v"(d[0]) : "v"(a) : "memory");
+  } else if constexpr (N == 2) {
+    asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:32\n\ts_waitcnt lgkmcnt(0)"
+                 : "=&v"(d[0]), "=&v"(d[1]) : "v"(a) : "memory");
+  } else if constexpr (N == 3) {
+    asm volatile(
+        "ds_read_b128 %0, %3\n\tds_read_b128 %1, %3 offset:32\n\tds_read_b128 %2, %3 offset:64\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(d[0]), "=&v"(d[1]), "=&v"(d[2]) : "v"(a) : "memory");
+  } else {
+    asm volatile(
+        "ds_read_b128 %0, %4\n\tds_read_b128 %1, %4 offset:32\n\tds_read_b128 %2, %4 offset:64\n\t"
+        "ds_read_b128 %3, %4 offset:96\n\ts_waitcnt lgkmcnt(0)"
+        : "=&v"(d[0]), "=&v"(d[1]), "=&v"(d[2]), "=&v"(d[3]) : "v"(a) : "memory");
+  }
+}
+
+__device__ __forceinline__ uint32_t lds_addr(const uint32_t *p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) uint32_t *)(p);
+}
+
+// (re, im) int8 of one pol (byte pair `sel` picks) as exact FP16: the bytes,
+// offset to unsigned, become the low mantissa bits of 1024 (f16 0x6400), and
+// one packed subtraction of 1152 restores the signed values.
+__device__ __forceinline__ h2v i8pair_to_f16(uint32_t w, uint32_t sel) {
+  const uint32_t m = __builtin_amdgcn_perm(0x64646464u, w ^ 0x80808080u, sel);
+  return __builtin_bit_cast(h2v, m) - h2v{(_Float16)1152.0f, (_Float16)1152.0f};
+}
+
+// B operand of one K step: (re, im) int8 of 4 antennas of pol `pol`, as
+// FP16 (exact) or BF16 (exact) bits.
+template <int WP>
+__device__ __forceinline__ v8h beam_operand(v4u d, int pol, uint32_t sel) {
+  if constexpr (WP == kWprecBF16) {
+    v8bf xb;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t pr = d[i] >> (16 * pol);
+      xb[2 * i] = (__bf16)(float)(int)(int8_t)(pr & 0xffu);
+      xb[2 * i + 1] = (__bf16)(float)(int)(int8_t)((pr >> 8) & 0xffu);
+    }
+    return __builtin_bit_cast(v8h, xb);
+  } else {
+    v8h x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const h2v h = i8pair_to_f16(d[i], sel);
+      x[2 * i] = h[0];
+      x[2 * i + 1] = h[1];
+    }
+    return x;
+  }
+}
+
+template <int WP>
+__device__ __forceinline__ v16f beam_mfma(v8h w, v8h x, v16f acc) {
+  if constexpr (WP == kWprecBF16)
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, w), __builtin_bit_cast(v8bf, x), acc,
+                                                   0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(w, x, acc, 0, 0, 0);
+}
+
+// One chunk of beams for (pol, 16-beam tile mt): kChunk/32 time tiles of 32
+// samples, TT tiles at a time (independent accumulators keep the matrix core
+// busy while the next operands convert).  GK: K steps whose operands are read
+// together in one asm block (fewer when the kernel also holds correlator
+// state, to stay within the register budget); GK == 0: plain LDS reads
+// (kernels without an LDS-DMA loader).  WP: weight precision.
+template <int RSD, int NK, int GK, int TT, int WP>
+__device__ __forceinline__ void beam_tiles(const VoltParams &P, const uint32_t *L, const v8h wop[2 * NK],
                                            int lane, int c, int pol, int mt, int s_chunk, int s_end) {
+  static_assert((kChunk / 32) % TT == 0, "tiles per pass");
+  constexpr int G = GK < 1 ? 1 : (GK < 4 ? GK : 4);
   const int kg = lane >> 5;
+  const uint32_t sel = pol ? 0x04030402u : 0x04010400u;
+  const bool full_tile = 16 * (mt + 1) <= P.nbeams_out;
+  __half2 *const prow = reinterpret_cast<__half2 *>(P.beams) + ((long)c * 2 + pol) * P.nbeams_out * P.T;
 #pragma unroll 1
-  for (int tt = 0; tt < kChunk / 32; ++tt) {
-    const int t = 32 * tt + (lane & 31);
-    v16f acc;
+  for (int t0 = 0; t0 < kChunk / 32; t0 += TT) {
+    v4u d[TT][NK];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    for (int u = 0; u < TT; ++u) {
+      const int t = 32 * (t0 + u) + (lane & 31);
+      if constexpr (GK == 0) {
 #pragma unroll
-    for (int j = 0; j < NK; ++j) {
-      // 4 antennas x both pols: 16-byte aligned since RSD and 8j + 4kg are multiples of 4
-      const uint4 d = *reinterpret_cast<const uint4 *>(L + t * RSD + 8 * j + 4 * kg);
-      const uint32_t w4[4] = {d.x, d.y, d.z, d.w};
-      if (P.wprec == kWprecBF16) {
-        // int8 samples are exact in BF16 too; the weight operand holds BF16 bits
-        v8bf xb;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const uint32_t pr = w4[i] >> (16 * pol);
-          xb[2 * i] = (__bf16)(float)(int)(int8_t)(pr & 0xffu);
-          xb[2 * i + 1] = (__bf16)(float)(int)(int8_t)((pr >> 8) & 0xffu);
-        }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8bf, wop[2 * j]), xb, acc, 0, 0, 0);
+        for (int j = 0; j < NK; ++j) d[u][j] = *reinterpret_cast<const v4u *>(L + t * RSD + 8 * j + 4 * kg);
       } else {
-        v8h xop;
+        const uint32_t a = lds_addr(L + t * RSD + 4 * kg);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const uint32_t pr = w4[i] >> (16 * pol);
-          xop[2 * i] = (_Float16)(int)(int8_t)(pr & 0xffu);
-          xop[2 * i + 1] = (_Float16)(int)(int8_t)((pr >> 8) & 0xffu);
+        for (int j = 0; j < NK; j += G) {
+          if (NK - j >= G) lds_read_ops<G>(a + 32 * j, &d[u][j]);
+          else if (NK - j == 3) lds_read_ops<(G >= 3 ? 3 : 1)>(a + 32 * j, &d[u][j]);
+          else if (NK - j == 2) lds_read_ops<(G >= 2 ? 2 : 1)>(a + 32 * j, &d[u][j]);
+          else lds_read_ops<1>(a + 32 * j, &d[u][j]);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[2 * j], xop, acc, 0, 0, 0);
-        if (P.wprec == kWprecF16HiLo)
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(wop[2 * j + 1], xop, acc, 0, 0, 0);
       }
     }
-    const int s = s_chunk + t;
-    if (s < s_end) {
-      __half2 *row = reinterpret_cast<__half2 *>(P.beams) + ((long)c * 2 + pol) * P.nbeams_out * P.T + s;
+    v16f acc[TT];
 #pragma unroll
-      for (int v = 0; v < 16; v += 2) {
-        const int r = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        const int e = 16 * mt + (r >> 1);
-        if (e < P.nbeams_out) row[(long)e * P.T] = __floats2half2_rn(acc[v], acc[v + 1]);
+    for (int u = 0; u < TT; ++u)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[u][i] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+      v8h x[TT];
+#pragma unroll
+      for (int u = 0; u < TT; ++u) x[u] = beam_operand<WP>(d[u][j], pol, sel);
+#pragma unroll
+      for (int u = 0; u < TT; ++u) acc[u] = beam_mfma<WP>(wop[2 * j], x[u], acc[u]);
+      if constexpr (WP == kWprecF16HiLo) {
+#pragma unroll
+        for (int u = 0; u < TT; ++u) acc[u] = beam_mfma<WP>(wop[2 * j + 1], x[u], acc[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TT; ++u) {
+      const int s = s_chunk + 32 * (t0 + u) + (lane & 31);
+      if (s < s_end) {
+        __half2 *row = prow + s;
+        if (full_tile) {
+#pragma unroll
+          for (int v = 0; v < 16; v += 2) {
+            const int r = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            row[(long)(16 * mt + (r >> 1)) * P.T] = __floats2half2_rn(acc[u][v], acc[u][v + 1]);
+          }
+        } else {
+#pragma unroll
+          for (int v = 0; v < 16; v += 2) {
+            const int r = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            const int e = 16 * mt + (r >> 1);
+            if (e < P.nbeams_out) row[(long)e * P.T] = __floats2half2_rn(acc[u][v], acc[u][v + 1]);
+          }
+        }
       }
     }
   }
+}
+
+template <int RSD, int NK, int GK, int TT>
+__device__ __forceinline__ void beam_chunk(const VoltParams &P, const uint32_t *L, const v8h wop[2 * NK],
+                                           int lane, int c, int pol, int mt, int s_chunk, int s_end) {
+  if (P.wprec == kWprecF16HiLo)
+    beam_tiles<RSD, NK, GK, TT, kWprecF16HiLo>(P, L, wop, lane, c, pol, mt, s_chunk, s_end);
+  else if (P.wprec == kWprecF16)
+    beam_tiles<RSD, NK, GK, TT, kWprecF16>(P, L, wop, lane, c, pol, mt, s_chunk, s_end);
+  else
+    beam_tiles<RSD, NK, GK, TT, kWprecBF16>(P, L, wop, lane, c, pol, mt, s_chunk, s_end);
 }
 
 template <int NK>
@@ -316,6 +441,15 @@ volt_fast_kernel(VoltParams P) {
     const int s0 = s_begin + ch * kChunk;
     const int valid = min(kChunk, s_end - s0);
     uint32_t *dst = lds + (ch % NS) * STAGE;
+    if (TIGHT && valid == kChunk) {
+      // rows packed as in the ring: one contiguous run of STAGE dwords
+      const uint32_t *base = chan32 + (long)(s0 + off) * P.R;
+#pragma unroll
+      for (int j = 0; j < NI; ++j)
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(base + j * 256 + lane * 4),
+                                         (__attribute__((address_space(3))) void *)(dst + j * 256), 16, 0, 0);
+      return;
+    }
 #pragma unroll 2
     for (int j = 0; j < NI; ++j) {
       const int pos = j * 256 + lane * 4;
@@ -335,7 +469,12 @@ volt_fast_kernel(VoltParams P) {
   const int bpol = bw & 1, bmt = bw >> 1;
   v8h wop[DO_BEAM ? 2 * NK : 1];
   if constexpr (DO_BEAM) {
-    if (!is_corr && !is_loader) load_weights<NK>(P, c, bpol, bmt, NMT, lane, wop);
+    if (!is_corr && !is_loader) {
+      load_weights<NK>(P, c, bpol, bmt, NMT, lane, wop);
+      // the weights arrive before the chunk loop: inside it a beam wave's
+      // vmcnt then counts only its own beam stores, which nothing waits for
+      wait_vmcnt<0>();
+    }
   }
 
   if (is_loader) {
@@ -373,13 +512,14 @@ volt_fast_kernel(VoltParams P) {
         }
       }
     }
-    __syncthreads();
+    chunk_barrier();
     if (is_loader && ch + NS - 1 < nchunks) issue(ch + NS - 1);
     if constexpr (DO_CORR) {
       if (is_corr) corr_chunk<RSD>(L, gA, gB, lane, acc);
     }
     if constexpr (DO_BEAM) {
-      if (!is_corr && !is_loader) beam_chunk<RSD, NK>(P, L, wop, lane, c, bpol, bmt, s0, s_end);
+      if (!is_corr && !is_loader)
+        beam_chunk<RSD, NK, DO_CORR ? 1 : 4, (DO_CORR || NK > 5) ? 1 : 2>(P, L, wop, lane, c, bpol, bmt, s0, s_end);
     }
   }
   if (bad) atomicOr(P.err, kErrRange);
@@ -526,7 +666,7 @@ volt_general_kernel(VoltParams P) {
       if (is_corr) corr_chunk<RSD>(L, gA, gB, lane, acc);
     }
     if constexpr (DO_BEAM) {
-      if (!is_corr) beam_chunk<RSD, NK>(P, L, wop, lane, c, bpol, bmt, s0, s_end);
+      if (!is_corr) beam_chunk<RSD, NK, 0, 1>(P, L, wop, lane, c, bpol, bmt, s0, s_end);
     }
     if (ch + 1 < nchunks) stash(buf ^ 1);
     __syncthreads();
