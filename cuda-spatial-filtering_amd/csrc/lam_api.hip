@@ -36,7 +36,7 @@ struct Buf {
   int8_t *d_samples = nullptr;  // staging for host inputs
   int16_t *d_scales = nullptr;
   __half *d_beams = nullptr;
-  int *d_vis_part = nullptr;    // this block's partial visibilities [split][C][NBL][8]
+  int *d_vis_part = nullptr;    // this block's partial visibilities [split][C][pq][NBL][2]
   v8h *d_wop2 = nullptr;        // adaptive: this block's 2B-beam weight operands
   float *d_evals = nullptr, *d_evecs = nullptr;
   int *d_counts = nullptr, *d_sweeps = nullptr, *d_fallback = nullptr, *d_nsig = nullptr;
@@ -244,7 +244,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
       Prof pr(ctx, s, LAM_K_CORRBEAM);
       HIPC(ctx, launch_volt(p, true, true, s));
     }
-    HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, ctx->d_vis_acc, atomic_acc, s));
+    HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, (int)ctx->NBL, ctx->d_vis_acc, atomic_acc, s));
     return LAM_OK;
   }
   // adaptive: correlate -> eig/null -> 2B beams
@@ -270,7 +270,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
       Prof pr(ctx, s, LAM_K_BEAM2B);
       HIPC(ctx, launch_volt(pb, false, true, s));
     }
-    HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, ctx->d_vis_acc, atomic_acc, s));
+    HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, (int)ctx->NBL, ctx->d_vis_acc, atomic_acc, s));
     return LAM_OK;
   }
   const int M = ctx->C * 2;

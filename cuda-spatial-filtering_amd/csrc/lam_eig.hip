@@ -765,7 +765,7 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
     const int i = idx - j * (j + 1) / 2;
     long re = 0, im = 0;
     for (int k = 0; k < P.nsplit; ++k) {
-      const int *v = P.vis_part + ((((long)k * P.C + c) * nbl + idx) * 4 + p * 2 + p) * 2;
+      const int *v = P.vis_part + ((((long)k * P.C + c) * 4 + 3 * p) * nbl + idx) * 2;
       re += v[0];
       im += v[1];
     }
@@ -1046,7 +1046,7 @@ __global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P)
       const long bl = (long)aj * (aj + 1) / 2 + ai;
       long re = 0, im = 0;
       for (int k = 0; k < P.nsplit; ++k) {
-        const int *v = P.vis_part + ((((long)k * P.C + c) * nbl_full + bl) * 4 + pol * 2 + pol) * 2;
+        const int *v = P.vis_part + ((((long)k * P.C + c) * 4 + 3 * pol) * nbl_full + bl) * 2;
         re += v[0];
         im += v[1];
       }

@@ -57,7 +57,7 @@ struct VoltParams {
   int samples_per_wg;      // multiple of kChunk
   int nsplit;              // workgroups per channel
   int xcd_map;             // 1: keep all workgroups of a channel on one XCD
-  // correlator output: one int32 partial per time split [nsplit][C][NBL][2][2][2]
+  // correlator output: one int32 partial per time split [nsplit][C][2*p+q][NBL][2]
   int *vis_part;
   // beamformer
   const v8h *wop;          // [C][2][NMT][NK][2][64] weight operands (hi, lo)
@@ -72,7 +72,7 @@ struct VoltParams {
 // Parameters of the eigen / RFI-null kernel, one workgroup per (channel, pol).
 struct EigParams {
   int C, R, B;
-  const int *vis_part;     // per-block partial visibilities [nsplit][C][NBL][2][2][2]
+  const int *vis_part;     // per-block partial visibilities [nsplit][C][2*p+q][NBL][2]
   int nsplit;
   const __half *weights;   // [C][2][B][R][2]
   const int *fixed_k;      // [C][2]
@@ -107,7 +107,7 @@ struct EigParams {
 // Eigen stage of subarrays (uneven sizes), one persistent launch.
 struct SubParams {
   int C, R, G;             // channels, receivers, groups per channel
-  const int *vis_part;     // [nsplit][C][NBL][2][2][2]
+  const int *vis_part;     // [nsplit][C][2*p+q][NBL][2]
   int nsplit;
   const uint8_t *mask;     // [C][G][R] antenna membership
   const int *k;            // [C*2*G] modes to remove
@@ -140,7 +140,7 @@ struct GenParams {
 
 // Features of the learned warm start, one row per (channel, pol).
 struct FeatParams {
-  const int *vis_part;     // [nsplit][C][NBL][2][2][2]
+  const int *vis_part;     // [nsplit][C][2*p+q][NBL][2]
   int nsplit, C, n, KW, Dp;
   const float2 *basis;     // [C][2][KW][n] previous block's top-KW whitened eigenvectors or nullptr
   uint16_t *X;             // out: [C*2][Dp] BF16
@@ -174,7 +174,7 @@ bool eig_is_warm(const EigParams &p);
 hipError_t launch_prep_weights(const __half *w, const float2 *gains, const float2 *phase,
                                int C, int B, int R, int NMT, int NK, int wprec, v8h *wop,
                                hipStream_t s);
-hipError_t launch_accumulate_vis(const int *vis_part, int nsplit, long long count, long long *acc,
+hipError_t launch_accumulate_vis(const int *vis_part, int nsplit, long long count, int nbl, long long *acc,
                                  bool atomic, hipStream_t s);
 hipError_t launch_dump_vis(long long *acc, int C, int R, const float2 *gains,
                            const float2 *phase, float *out_f32, long long *out_i64,

@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) feature_kernel(FeatParams P) {
     const long idx = (long)tid * (tid + 1) / 2 + tid;  // diagonal baseline (tid, tid)
     long re = 0;
     for (int k = 0; k < P.nsplit; ++k)
-      re += P.vis_part[((((long)k * P.C + c) * nbl + idx) * 4 + p * 2 + p) * 2];
+      re += P.vis_part[((((long)k * P.C + c) * 4 + 3 * p) * nbl + idx) * 2];
     d[tid] = sqrtf(fmaxf((float)re, 1e-10f));
   }
   __syncthreads();
@@ -44,7 +44,7 @@ __global__ void __launch_bounds__(256) feature_kernel(FeatParams P) {
     const long bl = (long)j * (j + 1) / 2 + i;
     long re = 0, im = 0;
     for (int k = 0; k < P.nsplit; ++k) {
-      const int *v = P.vis_part + ((((long)k * P.C + c) * nbl + bl) * 4 + p * 2 + p) * 2;
+      const int *v = P.vis_part + ((((long)k * P.C + c) * 4 + 3 * p) * nbl + bl) * 2;
       re += v[0];
       im += v[1];
     }

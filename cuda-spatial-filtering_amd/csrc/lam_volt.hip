@@ -182,7 +182,8 @@ __device__ __forceinline__ void corr_chunk(const uint32_t *L, int gA, int gB, in
   }
 }
 
-// Partial visibilities of this workgroup: vis_part[split][c][bl(a,b)][p][q][re,im].
+// Partial visibilities of this workgroup: vis_part[split][c][p*2+q][bl(a,b)][re,im]
+// (polarisation pair major, so the XX and YY covariances are contiguous runs).
 __device__ __forceinline__ void corr_store(const VoltParams &P, const v4i acc[12], int gA, int gB,
                                            int lane, int c, int split) {
   const long nbl = (long)P.R * (P.R + 1) / 2;
@@ -192,13 +193,13 @@ __device__ __forceinline__ void corr_store(const VoltParams &P, const v4i acc[12
   for (int v = 0; v < 4; ++v) {
     const int a = 16 * gA + 4 * (lane >> 4) + v;
     if (a > b || b >= P.R) continue;
-    int *o = base + ((long)b * (b + 1) / 2 + a) * 8;
+    int2 *o = reinterpret_cast<int2 *>(base) + (long)b * (b + 1) / 2 + a;
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
         const v4i *a3 = &acc[(p * 2 + q) * 3];
-        reinterpret_cast<int2 *>(o)[p * 2 + q] = make_int2(a3[0][v], a3[1][v] - a3[2][v]);
+        o[(p * 2 + q) * nbl] = make_int2(a3[0][v], a3[1][v] - a3[2][v]);
       }
   }
 }
@@ -734,28 +735,38 @@ hipError_t launch_volt(const VoltParams &p, bool do_corr, bool do_beam, hipStrea
 #undef LAM_NG
 }
 
-// Sum of the per-split partial visibilities into the int64 accumulator.
-// (atomic when several buffers' blocks may be in flight on different streams)
+// Sum of the per-split partial visibilities ([split][c][pq][bl][re,im]) into the
+// int64 accumulator [c][bl][pq][re,im] (atomic when several buffers' blocks may
+// be in flight on different streams).  nbl8 = 8 * baselines.
 template <bool ATOMIC>
-__global__ void accumulate_vis_kernel(const int *part, int nsplit, long long count, long long *acc) {
+__global__ void accumulate_vis_kernel(const int *part, int nsplit, long long count, int nbl8, long long *acc) {
+  const int nbl = nbl8 >> 3;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
        i += (long long)gridDim.x * blockDim.x) {
+    // i walks the partial layout (contiguous reads): c, pq, bl, ri
+    const long long c = i / nbl8;
+    const int r = (int)(i - c * nbl8);
+    const int pq = r / (2 * nbl), r2 = r - pq * 2 * nbl;
+    const int bl = r2 >> 1, ri = r2 & 1;
     long long s = 0;
     for (int k = 0; k < nsplit; ++k) s += part[(long long)k * count + i];
+    long long *o = acc + c * nbl8 + ((long long)bl * 4 + pq) * 2 + ri;
     if (ATOMIC)
-      atomicAdd(reinterpret_cast<unsigned long long *>(acc + i), (unsigned long long)s);
+      atomicAdd(reinterpret_cast<unsigned long long *>(o), (unsigned long long)s);
     else
-      acc[i] += s;
+      *o += s;
   }
 }
 
-hipError_t launch_accumulate_vis(const int *vis_part, int nsplit, long long count, long long *acc,
+hipError_t launch_accumulate_vis(const int *vis_part, int nsplit, long long count, int nbl, long long *acc,
                                  bool atomic, hipStream_t s) {
   const int blocks = (int)std::min<long long>((count + 255) / 256, 8192);
   if (atomic)
-    hipLaunchKernelGGL(accumulate_vis_kernel<true>, dim3(blocks), dim3(256), 0, s, vis_part, nsplit, count, acc);
+    hipLaunchKernelGGL(accumulate_vis_kernel<true>, dim3(blocks), dim3(256), 0, s, vis_part, nsplit, count, 8 * nbl,
+                       acc);
   else
-    hipLaunchKernelGGL(accumulate_vis_kernel<false>, dim3(blocks), dim3(256), 0, s, vis_part, nsplit, count, acc);
+    hipLaunchKernelGGL(accumulate_vis_kernel<false>, dim3(blocks), dim3(256), 0, s, vis_part, nsplit, count,
+                       8 * nbl, acc);
   return hipGetLastError();
 }
 
