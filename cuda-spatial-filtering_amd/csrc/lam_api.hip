@@ -954,7 +954,7 @@ int lam_set_weight_precision(lam_ctx *ctx, int precision) {
 }
 
 int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads, int fallback_threads) {
-  auto ok = [](int t) { return t == 256 || t == 512 || t == 1024; };
+  auto ok = [](int t) { return t == 64 || t == 128 || t == 256 || t == 512 || t == 1024; };
   if (!ctx || !ok(plain_threads) || !ok(fallback_threads)) return LAM_ERR_ARG;
   ctx->eig_plain_threads = plain_threads;
   ctx->eig_fallback_threads = fallback_threads;
@@ -1162,7 +1162,7 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
     // with several buffers the eigen stage shares the GPU with the voltage
     // kernels of neighbouring blocks: judge by streaming time per block
     const bool stream = ctx->nbuf > 1;
-    for (int th : {256, 512, 1024}) {
+    for (int th : {64, 128, 256, 512, 1024}) {
       ctx->eig_plain_threads = th;
       double t = 0.0;
       if ((rc = stream ? time_stream(ctx, samples, scales, nblocks, iters, t)
@@ -1344,6 +1344,21 @@ int lam_learn_capture(lam_ctx *ctx, int max_blocks) {
 }
 
 int lam_learn_captured(lam_ctx *ctx) { return ctx ? ctx->lr.cap_count : 0; }
+
+int lam_learn_sample(lam_ctx *ctx, int block, uint16_t *x, float *labels, int32_t *k) {
+  if (!ctx) return LAM_ERR_ARG;
+  Learn &L = ctx->lr;
+  if (!L.ready || block < 0 || block >= L.cap_count)
+    return fail(ctx, LAM_ERR_ARG, "learn_sample: block %d not captured", block);
+  if (int r = sync_all(ctx)) return r;
+  const size_t M = (size_t)ctx->C * 2, nu = (size_t)L.KW * ctx->R;
+  if (x)
+    HIPC(ctx, hipMemcpy(x, L.d_dsX + (size_t)block * M * L.Dp, M * L.Dp * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  if (labels)
+    HIPC(ctx, hipMemcpy(labels, L.d_dsU + (size_t)block * M * nu, M * nu * sizeof(float2), hipMemcpyDeviceToHost));
+  if (k) HIPC(ctx, hipMemcpy(k, L.d_dsK + (size_t)block * M, M * sizeof(int), hipMemcpyDeviceToHost));
+  return LAM_OK;
+}
 
 int lam_learn_train(lam_ctx *ctx, int epochs, int batch, float lr, uint64_t seed, float *loss_per_epoch) {
   if (!ctx || epochs < 0 || batch < 1 || !(lr > 0.0f)) return LAM_ERR_ARG;

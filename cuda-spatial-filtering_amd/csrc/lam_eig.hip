@@ -136,13 +136,26 @@ __device__ __forceinline__ Rot make_rotation(float2 b, float app, float aqq) {
 // Per-thread block list of one Jacobi round for NTHR threads and n <= 64:
 // upper-triangle 2x2 blocks (at most 32 * 33 / 2).
 template <int NTHR> struct JacobiLists {
-  static constexpr int kMaxBlk = (32 * 33 / 2 + NTHR - 1) / NTHR;
+  static constexpr int kMaxBlk = (32 * 33 / 2 + NTHR - 1) / NTHR;  // blocks per thread at n = 64
 };
+
+// Entry (r, c) of a Hermitian matrix kept as its upper triangle (column-major,
+// A[c * ld + r] for r <= c); the lower triangle is never read.
+__device__ __forceinline__ float2 herm_get(const float2 *A, int ld, int r, int c) {
+  if (r <= c) return A[c * ld + r];
+  const float2 v = A[r * ld + c];
+  return make_float2(v.x, -v.y);
+}
+__device__ __forceinline__ void herm_set(float2 *A, int ld, int r, int c, float2 v) {
+  if (r <= c) A[c * ld + r] = v;
+  else A[r * ld + c] = make_float2(v.x, -v.y);
+}
 
 // Cyclic Jacobi on the Hermitian matrix A (column-major, leading dim n) in LDS;
 // V receives the eigenvectors (columns).  Each round rotates n/2 disjoint pairs
-// (round-robin order); the update visits only blocks kp <= kq and writes the
-// mirrored block as its conjugate transpose.  Returns the number of sweeps.
+// (round-robin order); the update visits only blocks kp <= kq and keeps only
+// the upper triangle current (the lower one goes stale during the sweeps).
+// Returns the number of sweeps.
 template <int NTHR>
 __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol,
                           int2 *rpq, Rot *rot, float *red, bool init_v = true) {
@@ -158,30 +171,35 @@ __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_s
   }
   // this thread's blocks (kp <= kq) of the upper triangle, decoded once
   constexpr int kMaxBlk = JacobiLists<NTHR>::kMaxBlk;
-  static_assert(kMaxBlk <= 3, "block list holds three entries per thread");
-  int blk0 = 0, blk1 = 0, blk2 = 0;
+  int blk[kMaxBlk];
   int nmine = 0;
-  for (int idx = tid; idx < nblk && nmine < kMaxBlk; idx += nthr) {
-    int kp = (int)((2.0f * h + 1.0f - sqrtf((2.0f * h + 1.0f) * (2.0f * h + 1.0f) - 8.0f * idx)) * 0.5f);
-    while (kp > 0 && kp * (2 * h - kp + 1) / 2 > idx) --kp;
-    while ((kp + 1) * (2 * h - kp) / 2 <= idx) ++kp;
-    const int kq = kp + idx - kp * (2 * h - kp + 1) / 2;
-    const int code = kp | (kq << 8);
-    if (nmine == 0) blk0 = code;
-    else if (nmine == 1) blk1 = code;
-    else blk2 = code;
-    ++nmine;
+#pragma unroll
+  for (int j = 0; j < kMaxBlk; ++j) {
+    const int idx = tid + j * NTHR;
+    blk[j] = 0;
+    if (idx < nblk) {
+      int kp = (int)((2.0f * h + 1.0f - sqrtf((2.0f * h + 1.0f) * (2.0f * h + 1.0f) - 8.0f * idx)) * 0.5f);
+      while (kp > 0 && kp * (2 * h - kp + 1) / 2 > idx) --kp;
+      while ((kp + 1) * (2 * h - kp) / 2 <= idx) ++kp;
+      const int kq = kp + idx - kp * (2 * h - kp + 1) / 2;
+      blk[j] = kp | (kq << 8);
+      nmine = j + 1;
+    }
   }
+  // eigenvector update: threads grouped by pair, vrg consecutive threads per pair
+  const int vrg = nthr >= h ? nthr / h : 1;
+  const int vk = tid / vrg, vr0 = tid - vk * vrg;
   __syncthreads();
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
     float off = 0.0f, tot = 0.0f;
-    for (int i = tid; i < n * n; i += nthr) {
+    for (int i = tid; i < n * n; i += nthr) {  // upper triangle; off-diagonal entries count twice
       const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
+      if (r > c) continue;
       const float2 a = A[c * ld + r];
       const float m = a.x * a.x + a.y * a.y;
-      tot += m;
-      if (r != c) off += m;
+      tot += r == c ? m : 2.0f * m;
+      if (r != c) off += 2.0f * m;
     }
     off = block_sum(off, red);
     tot = block_sum(tot, red);
@@ -201,56 +219,47 @@ __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_s
       }
       __syncthreads();
       // A <- J^H A J on blocks (kp <= kq); the (kq, kp) block is the conjugate transpose
-#pragma unroll 1
-      for (int j = 0; j < nmine; ++j) {
-        const int code = j == 0 ? blk0 : (j == 1 ? blk1 : blk2);
+#pragma unroll
+      for (int j = 0; j < kMaxBlk; ++j) {
+        if (j >= nmine) break;
+        const int code = blk[j];
         const int kp = code & 0xff, kq = code >> 8;
         const int2 P = rpq[kp], Q = rpq[kq];
         const Rot RP = rot[kp], RQ = rot[kq];
         const bool pv = P.y < n, qv = Q.y < n;  // the second index of a pair may be the dummy
-        float2 x00 = A[Q.x * ld + P.x];
-        float2 x01 = qv ? A[Q.y * ld + P.x] : make_float2(0.f, 0.f);
-        float2 x10 = pv ? A[Q.x * ld + P.y] : make_float2(0.f, 0.f);
-        float2 x11 = (pv && qv) ? A[Q.y * ld + P.y] : make_float2(0.f, 0.f);
+        float2 x00 = herm_get(A, ld, P.x, Q.x);
+        float2 x01 = qv ? herm_get(A, ld, P.x, Q.y) : make_float2(0.f, 0.f);
+        float2 x10 = pv ? herm_get(A, ld, P.y, Q.x) : make_float2(0.f, 0.f);
+        float2 x11 = (pv && qv) ? herm_get(A, ld, P.y, Q.y) : make_float2(0.f, 0.f);
         rotate_block(RP, RQ, x00, x01, x10, x11);
-        if (kp == kq) {  // the annihilated pair: exact zeros, real diagonal
-          x01 = make_float2(0.f, 0.f);
-          x10 = make_float2(0.f, 0.f);
+        if (kp == kq) {  // the annihilated pair: exact zeros, real diagonal (P.x < P.y)
           x00.y = 0.f;
           x11.y = 0.f;
           A[P.x * ld + P.x] = x00;
           if (pv) {
-            A[P.y * ld + P.x] = x01;
-            A[P.x * ld + P.y] = x10;
+            A[P.y * ld + P.x] = make_float2(0.f, 0.f);
             A[P.y * ld + P.y] = x11;
           }
         } else {
-          A[Q.x * ld + P.x] = x00;
-          A[P.x * ld + Q.x] = make_float2(x00.x, -x00.y);
-          if (qv) {
-            A[Q.y * ld + P.x] = x01;
-            A[P.x * ld + Q.y] = make_float2(x01.x, -x01.y);
-          }
-          if (pv) {
-            A[Q.x * ld + P.y] = x10;
-            A[P.y * ld + Q.x] = make_float2(x10.x, -x10.y);
-          }
-          if (pv && qv) {
-            A[Q.y * ld + P.y] = x11;
-            A[P.y * ld + Q.y] = make_float2(x11.x, -x11.y);
-          }
+          herm_set(A, ld, P.x, Q.x, x00);
+          if (qv) herm_set(A, ld, P.x, Q.y, x01);
+          if (pv) herm_set(A, ld, P.y, Q.x, x10);
+          if (pv && qv) herm_set(A, ld, P.y, Q.y, x11);
         }
       }
-      // V <- V J
-      for (int idx = tid; idx < n * h; idx += NTHR) {
-        const int k = (int)(((float)idx + 0.5f) * inv_n), row = idx - k * n;
-        const int2 Q = rpq[k];
-        if (Q.y >= n) continue;
-        const Rot RQ = rot[k];
-        const float2 vp = V[Q.x * ld + row], vq = V[Q.y * ld + row];
-        const float2 y = cmul(make_float2(RQ.er, -RQ.ei), vq);
-        V[Q.x * ld + row] = make_float2(RQ.c * vp.x - RQ.s * y.x, RQ.c * vp.y - RQ.s * y.y);
-        V[Q.y * ld + row] = make_float2(RQ.s * vp.x + RQ.c * y.x, RQ.s * vp.y + RQ.c * y.y);
+      // V <- V J: this thread's pair k (rotation read once), rows r0, r0 + vrg, ...
+      if (vk < h) {
+        const int2 Q = rpq[vk];
+        if (Q.y < n) {
+          const Rot RQ = rot[vk];
+          const float2 eb = make_float2(RQ.er, -RQ.ei);
+          for (int row = vr0; row < n; row += vrg) {
+            const float2 vp = V[Q.x * ld + row], vq = V[Q.y * ld + row];
+            const float2 y = cmul(eb, vq);
+            V[Q.x * ld + row] = make_float2(RQ.c * vp.x - RQ.s * y.x, RQ.c * vp.y - RQ.s * y.y);
+            V[Q.y * ld + row] = make_float2(RQ.s * vp.x + RQ.c * y.x, RQ.s * vp.y + RQ.c * y.y);
+          }
+        }
       }
       __syncthreads();
     }
@@ -968,6 +977,8 @@ hipError_t launch_eig_t(const EigParams &p, hipStream_t s) {
 
 hipError_t launch_eig_plain(const EigParams &p, int nthr, hipStream_t s) {
   switch (nthr) {
+    case 64: return launch_eig_t<64, false>(p, s);
+    case 128: return launch_eig_t<128, false>(p, s);
     case 512: return launch_eig_t<512, false>(p, s);
     case 1024: return launch_eig_t<1024, false>(p, s);
     default: return launch_eig_t<256, false>(p, s);

@@ -21,50 +21,70 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ---- features: one workgroup per (channel, pol) ----
-__global__ void __launch_bounds__(256) feature_kernel(FeatParams P) {
-  extern __shared__ __align__(16) char smem_raw[];
-  const int n = P.n, cp = blockIdx.x, c = cp >> 1, p = cp & 1, tid = threadIdx.x;
-  float *d = reinterpret_cast<float *>(smem_raw);
-  const long nbl = (long)n * (n + 1) / 2;
-  if (tid < n) {
-    const long idx = (long)tid * (tid + 1) / 2 + tid;  // diagonal baseline (tid, tid)
-    long re = 0;
-    for (int k = 0; k < P.nsplit; ++k)
-      re += P.vis_part[((((long)k * P.C + c) * 4 + 3 * p) * nbl + idx) * 2];
-    d[tid] = sqrtf(fmaxf((float)re, 1e-10f));
+// Each thread issues the loads of all its baselines (contiguous XX or YY run
+// of the partial visibilities, every split) before using any of them, so the
+// kernel waits on memory once; diagonal baselines give the whitening.
+constexpr int kFeatThreads = 256;
+constexpr int kFeatPer = (64 * 65 / 2 + kFeatThreads - 1) / kFeatThreads;  // baselines per thread, n <= 64
+
+__global__ void __launch_bounds__(kFeatThreads) feature_kernel(FeatParams P) {
+  __shared__ float d[64];
+  const int tid = threadIdx.x, cp = blockIdx.x, c = cp >> 1, p = cp & 1;
+  const int n = P.n, nbl = n * (n + 1) / 2;
+  long re[kFeatPer], im[kFeatPer];
+#pragma unroll
+  for (int r = 0; r < kFeatPer; ++r) {
+    re[r] = 0;
+    im[r] = 0;
+  }
+  for (int k = 0; k < P.nsplit; ++k) {
+    const int2 *v = reinterpret_cast<const int2 *>(P.vis_part) + (((long)k * P.C + c) * 4 + 3 * p) * nbl;
+#pragma unroll
+    for (int r = 0; r < kFeatPer; ++r) {
+      const int bl = tid + r * kFeatThreads;
+      if (bl < nbl) {
+        const int2 x = v[bl];
+        re[r] += x.x;
+        im[r] += x.y;
+      }
+    }
+  }
+  int ii[kFeatPer], jj[kFeatPer];
+#pragma unroll
+  for (int r = 0; r < kFeatPer; ++r) {
+    const int bl = tid + r * kFeatThreads;
+    int j = (int)((sqrtf(8.0f * bl + 1.0f) - 1.0f) * 0.5f);
+    while (j * (j + 1) / 2 > bl) --j;
+    while ((j + 1) * (j + 2) / 2 <= bl) ++j;
+    ii[r] = bl - j * (j + 1) / 2;
+    jj[r] = j;
+    if (bl < nbl && ii[r] == j) d[j] = sqrtf(fmaxf((float)re[r], 1e-10f));
   }
   __syncthreads();
-  uint16_t *x = P.X + (long)cp * P.Dp;
-  const int noff = n * (n - 1) / 2;
-  for (int e = tid; e < noff; e += blockDim.x) {
-    int j = (int)((1.0f + sqrtf(1.0f + 8.0f * e)) * 0.5f);
-    while (j * (j - 1) / 2 > e) --j;
-    while ((j + 1) * j / 2 <= e) ++j;
-    const int i = e - j * (j - 1) / 2;
-    const long bl = (long)j * (j + 1) / 2 + i;
-    long re = 0, im = 0;
-    for (int k = 0; k < P.nsplit; ++k) {
-      const int *v = P.vis_part + ((((long)k * P.C + c) * 4 + 3 * p) * nbl + bl) * 2;
-      re += v[0];
-      im += v[1];
+  uint32_t *x32 = reinterpret_cast<uint32_t *>(P.X + (long)cp * P.Dp);
+  // whitened off-diagonal entries: baseline bl = j (j + 1) / 2 + i -> feature e = j (j - 1) / 2 + i
+#pragma unroll
+  for (int r = 0; r < kFeatPer; ++r) {
+    const int bl = tid + r * kFeatThreads, i = ii[r], j = jj[r];
+    if (bl < nbl && i != j) {
+      const float inv = 1.0f / (d[i] * d[j]);
+      x32[j * (j - 1) / 2 + i] = (uint32_t)f2bf((float)re[r] * inv) | ((uint32_t)f2bf((float)im[r] * inv) << 16);
     }
-    const float den = d[i] * d[j];
-    x[2 * e] = f2bf((float)re / den);
-    x[2 * e + 1] = f2bf((float)im / den);
   }
-  const int boff = 2 * noff, nb = P.KW * n;
+  // previous basis x sqrt(n), then zero padding (Dp and the offsets are even)
+  const int boff = n * (n - 1) / 2, nb = P.KW * n;
   const float sq = sqrtf((float)n);
-  for (int e = tid; e < nb; e += blockDim.x) {
+  for (int e = tid; e < nb; e += kFeatThreads) {
     float2 b = make_float2(0.f, 0.f);
     if (P.basis) b = P.basis[(long)cp * nb + e];
-    x[boff + 2 * e] = f2bf(sq * b.x);
-    x[boff + 2 * e + 1] = f2bf(sq * b.y);
+    x32[boff + e] = (uint32_t)f2bf(sq * b.x) | ((uint32_t)f2bf(sq * b.y) << 16);
   }
-  for (int e = boff + 2 * nb + tid; e < P.Dp; e += blockDim.x) x[e] = 0;
+  for (int e = boff + nb + tid; e < P.Dp / 2; e += kFeatThreads) x32[e] = 0u;
 }
 
 hipError_t launch_features(const FeatParams &p, hipStream_t s) {
-  hipLaunchKernelGGL(feature_kernel, dim3(p.C * 2), dim3(256), 64 * sizeof(float), s, p);
+  if (p.Dp % 2 || p.n > 64) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(feature_kernel, dim3(p.C * 2), dim3(kFeatThreads), 0, s, p);
   return hipGetLastError();
 }
 
@@ -72,8 +92,10 @@ hipError_t launch_features(const FeatParams &p, hipStream_t s) {
 // The output layer adds a skip connection from the previous block's vectors
 // held in the features: Z[m][j] += scale * X'[m][off + j] for j < len.
 // One workgroup per 32x32 output tile; its NW waves split K in 16-wide steps
-// and reduce through LDS.  Output: FP32 Z and/or BF16 relu(Z).
-template <int NW>
+// (U steps' operands loaded before their MFMAs, so a wave waits on memory
+// about K / (16 NW U) times) and reduce through LDS.  Output: FP32 Z and/or
+// BF16 relu(Z).
+template <int NW, int U>
 __global__ void __launch_bounds__(64 * NW)
 dense_fwd_kernel(const uint16_t *__restrict__ X, int M, int K, const uint16_t *__restrict__ Wt,
                  const float *__restrict__ b, int N, float *__restrict__ Z,
@@ -89,15 +111,15 @@ dense_fwd_kernel(const uint16_t *__restrict__ X, int M, int K, const uint16_t *_
   v16f acc = {};
   const int nks = K / 16;
   int ks = w;
-  for (; ks + 3 * NW < nks; ks += 4 * NW) {
-    v8bf a[4], bb[4];
+  for (; ks + (U - 1) * NW < nks; ks += U * NW) {
+    v8bf a[U], bb[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       a[u] = *reinterpret_cast<const v8bf *>(xa + (ks + NW * u) * 16);
       bb[u] = *reinterpret_cast<const v8bf *>(wb + (ks + NW * u) * 16);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       if (!mv) a[u] = v8bf{};
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[u], bb[u], acc, 0, 0, 0);
     }
@@ -133,10 +155,10 @@ hipError_t launch_dense_fwd(const uint16_t *X, int M, int K, const uint16_t *Wt,
                             int N, float *Z, uint16_t *Hbf, const Skip &sk, hipStream_t s) {
   if (K % 16 || N % 32) return hipErrorInvalidValue;
   const dim3 grid((M + 31) / 32, N / 32);
-  if (K >= 1024)  // long reductions: more waves per tile
-    hipLaunchKernelGGL(dense_fwd_kernel<8>, grid, dim3(512), 0, s, X, M, K, Wt, b, N, Z, Hbf, sk);
+  if (K >= 1024)  // long reductions: more waves per tile, deeper loads
+    hipLaunchKernelGGL((dense_fwd_kernel<8, 8>), grid, dim3(512), 0, s, X, M, K, Wt, b, N, Z, Hbf, sk);
   else
-    hipLaunchKernelGGL(dense_fwd_kernel<4>, grid, dim3(256), 0, s, X, M, K, Wt, b, N, Z, Hbf, sk);
+    hipLaunchKernelGGL((dense_fwd_kernel<4, 4>), grid, dim3(256), 0, s, X, M, K, Wt, b, N, Z, Hbf, sk);
   return hipGetLastError();
 }
 

@@ -120,6 +120,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_learn_dims.argtypes = [vp, vp, vp]
     L.lam_learn_capture.argtypes = [vp, ctypes.c_int]
     L.lam_learn_captured.argtypes = [vp]
+    L.lam_learn_sample.argtypes = [vp, ctypes.c_int, vp, vp, vp]
     L.lam_learn_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, vp]
     L.lam_learn_params.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
     L.lam_learn_grad.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, vp]
@@ -395,6 +396,16 @@ class Pipeline:
 
     def learn_captured(self):
         return int(self.L.lam_learn_captured(self.h))
+
+    def learn_sample(self, block):
+        """Captured features (BF16 bits), labels and mode counts of one block."""
+        d = self.learn_dims()
+        M = self.shape.C * 2
+        x = np.zeros((M, d["Dp"]), np.uint16)
+        lab = np.zeros((M, d["KW"], self.shape.R, 2), np.float32)
+        k = np.zeros(M, np.int32)
+        self._chk(self.L.lam_learn_sample(self.h, int(block), x.ctypes.data, lab.ctypes.data, k.ctypes.data))
+        return x, lab, k
 
     def learn_train(self, epochs, batch=768, lr=1e-3, seed=1):
         loss = np.zeros(max(1, epochs), np.float32)

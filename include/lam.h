@@ -183,8 +183,9 @@ int lam_set_tuning(lam_ctx *ctx, int samples_per_wg);
 enum { LAM_WPREC_F16_HILO = 0, LAM_WPREC_F16 = 1, LAM_WPREC_BF16 = 2 };
 int lam_set_weight_precision(lam_ctx *ctx, int precision);
 
-/* Workgroup width (256, 512 or 1024 threads) of the plain Jacobi launch and
- * of the second launch that re-solves the problems a warm start rejected. */
+/* Workgroup width (64, 128, 256, 512 or 1024 threads) of the plain Jacobi
+ * launch and of the second launch that re-solves the problems a warm start
+ * rejected. */
 int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads, int fallback_threads);
 
 /* Autotuner (no counterpart in the reference): times this context's kernels
@@ -275,6 +276,12 @@ int lam_learn_dims(lam_ctx *ctx, int32_t *dims, int64_t *params);
  * LAM_EIG_PLAIN; lam_learn_captured returns how many blocks are held. */
 int lam_learn_capture(lam_ctx *ctx, int max_blocks);
 int lam_learn_captured(lam_ctx *ctx);
+/* Captured training data of block `block` (0-based, < lam_learn_captured):
+ * x [C*2][padded features] BF16 bits, labels [C*2][warm_vectors][receivers][2]
+ * FP32 (the block's own plain-Jacobi vectors, the next block's warm-start
+ * input), k [C*2].  Any pointer may be NULL.  Synchronous.  No counterpart in
+ * the reference (inspection and tests of the learned warm start). */
+int lam_learn_sample(lam_ctx *ctx, int block, uint16_t *x, float *labels, int32_t *k);
 /* Adam on the captured set with the project's own forward / backward
  * kernels: loss = mean over samples of sum_{k<K} (1 - |u_k^H y_k|^2/|y_k|^2).
  * loss_per_epoch (may be NULL) receives `epochs` values.  Synchronous. */

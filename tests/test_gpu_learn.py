@@ -62,6 +62,48 @@ def test_mlp_gradient_matches_oracle():
     assert np.linalg.norm(gg - gc) <= 1e-3 * np.linalg.norm(gc)
 
 
+def test_features_match_oracle():
+    """Captured features of a block (whitened covariance upper triangle and
+    the previous block's warm vectors x sqrt(n), BF16) against the oracle's
+    feature layout built from the oracle correlator's exact visibilities
+    (covariance extraction and whitening: spatial.cuh:672-713, 1065-1093)."""
+    import oracle as orc
+    shp = syn.BlockShape(C=4, F=1, NT=64, R=36, P=2, RPP=36, NP=8, B=4)
+    scene = syn.make_scene(shp, seed=31, n_tones=2, amp=30.0)
+    blocks = [syn.make_block(shp, scene, seed=32 + i) for i in range(2)]
+    p = lam.Pipeline(shp, lam.MODE_ADAPTIVE)
+    p.set_weights(syn.random_weights(shp, seed=33)[0])
+    p.set_fixed_k(np.full((shp.C, 2), 2, np.int32))
+    p.learn_init(8, 256, seed=2)
+    p.learn_capture(2)
+    for ring, scales in blocks:
+        p.run_host(ring, scales)
+    assert p.learn_captured() == 2
+    x1, _, _ = p.learn_sample(1)
+    _, lab0, _ = p.learn_sample(0)
+    p.close()
+    d = lo.dims(36, 8, 256)
+    o = orc.Shape(C=shp.C, F=shp.F, NT=shp.NT, R=shp.R, P=shp.P, RPP=shp.RPP, NP=shp.NP, B=shp.B)
+    vis = orc.correlate(o, orc.ingest(o, blocks[1][0], blocks[1][1], orc.WINDOW_ADAPTIVE))
+    jj, ii = np.meshgrid(np.arange(36), np.arange(36))
+    up = ii <= jj
+    bl = (jj * (jj + 1) // 2 + ii)[up]
+    exact = 0
+    for c in range(shp.C):
+        for pol in range(2):
+            A = np.zeros((36, 36), np.complex128)
+            A[ii[up], jj[up]] = vis[c, bl, pol, pol, 0] + 1j * vis[c, bl, pol, pol, 1]
+            dd = np.sqrt(np.maximum(A.diagonal().real, 1e-10))
+            Aw = A / np.outer(dd, dd)
+            basis = lab0[c * 2 + pol, ..., 0] + 1j * lab0[c * 2 + pol, ..., 1]
+            ref = lo.features(Aw, basis, d).astype(np.int32)
+            got = x1[c * 2 + pol].astype(np.int32)
+            # FP32 on the GPU, FP64 in the oracle: at most one BF16 step apart
+            assert np.max(np.abs(got - ref)) <= 1, (c, pol)
+            exact += int(np.sum(got == ref))
+    assert exact >= 0.98 * shp.C * 2 * d["Dp"]
+
+
 def _whitened_covs(C=48, b0=45, seed=11):
     sh = syn.BlockShape(C=C, F=1, NT=64, R=36, P=2, RPP=36, NP=32, B=4)
     sc = syn.DriftScenario(C, 36, seed=seed, n_blocks=b0 + 2)
