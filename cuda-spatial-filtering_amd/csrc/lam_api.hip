@@ -47,6 +47,8 @@ struct Buf {
   hipEvent_t eig_done = nullptr;  // the eigen stage of this buffer's latest block
 };
 
+constexpr int kMlpParts = 4;  // K split of the inference layer 1
+
 // Learned warm start: MLP parameters, Adam state, per-block buffers and the
 // captured training set.
 struct Learn {
@@ -59,6 +61,7 @@ struct Learn {
   // per block (M = C * 2 rows)
   uint16_t *d_X = nullptr, *d_H = nullptr;
   float *d_Y = nullptr;
+  float *d_Zp = nullptr;  // inference: split-K partials of layer 1 [kMlpParts][M][H]
   float2 *d_basis = nullptr;
   // captured set
   int cap_max = 0, cap_count = 0;
@@ -293,9 +296,8 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
   }
   if (learned) {
     Prof pr(ctx, s, LAM_K_MLP);
-    HIPC(ctx, launch_dense_fwd(L.d_X, M, L.Dp, L.d_W1bf, L.d_prm + L.off_b1, L.H, nullptr, L.d_H, Skip{}, s));
-    HIPC(ctx, launch_dense_fwd(L.d_H, M, L.H, L.d_W2bf, L.d_prm + L.off_b2, L.Dyp, L.d_Y, nullptr,
-                               skip_of(L, L.d_X), s));
+    HIPC(ctx, launch_mlp_infer(L.d_X, M, L.Dp, L.d_W1bf, L.d_prm + L.off_b1, L.H, L.d_W2bf, L.d_prm + L.off_b2,
+                               L.Dyp, kMlpParts, L.d_Zp, L.d_H, L.d_Y, skip_of(L, L.d_X), s));
   }
   {
     EigParams e{};
@@ -495,7 +497,7 @@ void free_subarrays(lam_ctx *ctx) {
 void free_learn(Learn &L) {
   for (void *q : {(void *)L.d_prm, (void *)L.d_grad, (void *)L.d_m1, (void *)L.d_m2,
                   (void *)L.d_W1bf, (void *)L.d_W2bf, (void *)L.d_X, (void *)L.d_H, (void *)L.d_Y,
-                  (void *)L.d_basis, (void *)L.d_dsX, (void *)L.d_dsU, (void *)L.d_dsK,
+                  (void *)L.d_Zp, (void *)L.d_basis, (void *)L.d_dsX, (void *)L.d_dsU, (void *)L.d_dsK,
                   (void *)L.d_bX, (void *)L.d_bH, (void *)L.d_bU, (void *)L.d_bK, (void *)L.d_idx,
                   (void *)L.d_bZ, (void *)L.d_bY, (void *)L.d_bdY, (void *)L.d_bdH, (void *)L.d_loss})
     if (q) hipFree(q);
@@ -539,6 +541,7 @@ int ensure_learn(lam_ctx *ctx, int KW, int H, uint64_t seed, bool reinit) {
   chk(dalloc(ctx, &L.d_X, (size_t)M * L.Dp * sizeof(uint16_t)));
   chk(dalloc(ctx, &L.d_H, (size_t)M * H * sizeof(uint16_t)));
   chk(dalloc(ctx, &L.d_Y, (size_t)M * L.Dyp * sizeof(float)));
+  chk(dalloc(ctx, &L.d_Zp, (size_t)kMlpParts * M * H * sizeof(float)));
   chk(dalloc(ctx, &L.d_basis, (size_t)M * KW * n * sizeof(float2)));
   chk(dalloc(ctx, &L.d_loss, sizeof(float)));
   if (rc != LAM_OK) {
@@ -1134,7 +1137,9 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
   }
   ctx->profiling = true;
   int rc = LAM_OK;
-  // 1. time samples per voltage workgroup
+  // 1. time samples per voltage workgroup: more splits help the voltage
+  //    kernels but multiply the partial visibilities that the eigen stage,
+  //    the features and the accumulator read, so the whole sequence is timed
   {
     const int Tr = (ctx->T + kChunk - 1) / kChunk * kChunk;
     int best = ctx->samples_per_wg;
@@ -1144,7 +1149,9 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
       ctx->samples_per_wg = cand;
       double t = 0.0;
       if ((rc = time_sequence(ctx, samples, scales, nblocks, iters,
-                              {LAM_K_CORRBEAM, LAM_K_CORR, LAM_K_BEAM2B}, t)))
+                              {LAM_K_CORRBEAM, LAM_K_CORR, LAM_K_BEAM2B, LAM_K_EIG, LAM_K_EIGFB, LAM_K_FEAT,
+                               LAM_K_MLP, LAM_K_SUBARRAY},
+                              t)))
         break;
       if (t < best_t) {
         best_t = t;
@@ -1344,6 +1351,17 @@ int lam_learn_capture(lam_ctx *ctx, int max_blocks) {
 }
 
 int lam_learn_captured(lam_ctx *ctx) { return ctx ? ctx->lr.cap_count : 0; }
+
+int lam_learn_last(lam_ctx *ctx, uint16_t *x, float *y) {
+  if (!ctx) return LAM_ERR_ARG;
+  Learn &L = ctx->lr;
+  if (!L.ready) return fail(ctx, LAM_ERR_STATE, "learn_last: no network");
+  if (int r = sync_all(ctx)) return r;
+  const size_t M = (size_t)ctx->C * 2;
+  if (x) HIPC(ctx, hipMemcpy(x, L.d_X, M * L.Dp * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  if (y) HIPC(ctx, hipMemcpy(y, L.d_Y, M * L.Dyp * sizeof(float), hipMemcpyDeviceToHost));
+  return LAM_OK;
+}
 
 int lam_learn_sample(lam_ctx *ctx, int block, uint16_t *x, float *labels, int32_t *k) {
   if (!ctx) return LAM_ERR_ARG;

@@ -737,24 +737,24 @@ hipError_t launch_volt(const VoltParams &p, bool do_corr, bool do_beam, hipStrea
 
 // Sum of the per-split partial visibilities ([split][c][pq][bl][re,im]) into the
 // int64 accumulator [c][bl][pq][re,im] (atomic when several buffers' blocks may
-// be in flight on different streams).  nbl8 = 8 * baselines.
+// be in flight on different streams).  i walks the accumulator, so its
+// read-modify-write is contiguous; a wave reads four 64-byte runs of each
+// split.  nbl8 = 8 * baselines.
 template <bool ATOMIC>
 __global__ void accumulate_vis_kernel(const int *part, int nsplit, long long count, int nbl8, long long *acc) {
   const int nbl = nbl8 >> 3;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
        i += (long long)gridDim.x * blockDim.x) {
-    // i walks the partial layout (contiguous reads): c, pq, bl, ri
     const long long c = i / nbl8;
     const int r = (int)(i - c * nbl8);
-    const int pq = r / (2 * nbl), r2 = r - pq * 2 * nbl;
-    const int bl = r2 >> 1, ri = r2 & 1;
+    const int bl = r >> 3, pq = (r >> 1) & 3, ri = r & 1;
+    const long long src = c * nbl8 + ((long long)pq * nbl + bl) * 2 + ri;
     long long s = 0;
-    for (int k = 0; k < nsplit; ++k) s += part[(long long)k * count + i];
-    long long *o = acc + c * nbl8 + ((long long)bl * 4 + pq) * 2 + ri;
+    for (int k = 0; k < nsplit; ++k) s += part[(long long)k * count + src];
     if (ATOMIC)
-      atomicAdd(reinterpret_cast<unsigned long long *>(o), (unsigned long long)s);
+      atomicAdd(reinterpret_cast<unsigned long long *>(acc + i), (unsigned long long)s);
     else
-      *o += s;
+      acc[i] += s;
   }
 }
 

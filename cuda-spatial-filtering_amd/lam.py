@@ -121,6 +121,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_learn_capture.argtypes = [vp, ctypes.c_int]
     L.lam_learn_captured.argtypes = [vp]
     L.lam_learn_sample.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+    L.lam_learn_last.argtypes = [vp, vp, vp]
     L.lam_learn_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, vp]
     L.lam_learn_params.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
     L.lam_learn_grad.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, vp]
@@ -396,6 +397,15 @@ class Pipeline:
 
     def learn_captured(self):
         return int(self.L.lam_learn_captured(self.h))
+
+    def learn_last(self):
+        """Features (BF16 bits) and network output of the last learned block."""
+        d = self.learn_dims()
+        M = self.shape.C * 2
+        x = np.zeros((M, d["Dp"]), np.uint16)
+        y = np.zeros((M, d["Dyp"]), np.float32)
+        self._chk(self.L.lam_learn_last(self.h, x.ctypes.data, y.ctypes.data))
+        return x, y
 
     def learn_sample(self, block):
         """Captured features (BF16 bits), labels and mode counts of one block."""

@@ -282,6 +282,11 @@ int lam_learn_captured(lam_ctx *ctx);
  * input), k [C*2].  Any pointer may be NULL.  Synchronous.  No counterpart in
  * the reference (inspection and tests of the learned warm start). */
 int lam_learn_sample(lam_ctx *ctx, int block, uint16_t *x, float *labels, int32_t *k);
+/* Features x [C*2][padded features] (BF16 bits) and network output y
+ * [C*2][padded outputs] (FP32, skip connection included) of the most recent
+ * block run with LAM_EIG_LEARNED.  Either pointer may be NULL.  Synchronous.
+ * No counterpart in the reference (tests of the inference kernels). */
+int lam_learn_last(lam_ctx *ctx, uint16_t *x, float *y);
 /* Adam on the captured set with the project's own forward / backward
  * kernels: loss = mean over samples of sum_{k<K} (1 - |u_k^H y_k|^2/|y_k|^2).
  * loss_per_epoch (may be NULL) receives `epochs` values.  Synchronous. */

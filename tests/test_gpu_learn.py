@@ -104,6 +104,32 @@ def test_features_match_oracle():
     assert exact >= 0.98 * shp.C * 2 * d["Dp"]
 
 
+def test_mlp_inference_matches_oracle():
+    """The inference kernels (layer 1 split over K, layer 2 forming relu + BF16
+    on the fly, skip connection) against the oracle forward pass on the
+    block's own features, random parameters."""
+    shp = syn.BlockShape(C=8, F=1, NT=64, R=36, P=2, RPP=36, NP=8, B=4)
+    scene = syn.make_scene(shp, seed=41, n_tones=2, amp=30.0)
+    blocks = [syn.make_block(shp, scene, seed=42 + i) for i in range(2)]
+    p = lam.Pipeline(shp, lam.MODE_ADAPTIVE)
+    p.set_weights(syn.random_weights(shp, seed=43)[0])
+    p.set_fixed_k(np.full((shp.C, 2), 2, np.int32))
+    p.learn_init(8, 256, seed=5)
+    d = lo.dims(36, 8, 256)
+    rng = np.random.default_rng(6)
+    prm = rng.normal(0, 0.05, d["P"]).astype(np.float32)
+    p.learn_params(prm)
+    p.run_host(*blocks[0])
+    p.set_eig_method(lam.EIG_LEARNED, refine_sweeps=2)
+    p.run_host(*blocks[1])
+    x, y = p.learn_last()
+    p.close()
+    yr = lo.forward(prm, d, x)[5]
+    err = np.max(np.abs(y[:, :d["Dyp"]] - yr)) / np.max(np.abs(yr))
+    # FP32 sums on the GPU (other order), BF16 hidden values may land one step apart
+    assert err < 1e-2, err
+
+
 def _whitened_covs(C=48, b0=45, seed=11):
     sh = syn.BlockShape(C=C, F=1, NT=64, R=36, P=2, RPP=36, NP=32, B=4)
     sc = syn.DriftScenario(C, 36, seed=seed, n_blocks=b0 + 2)
