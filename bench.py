@@ -499,11 +499,20 @@ def main():
                     "kernel_ms": round(ms, 4), "algorithmic_bytes": per_kernel_bytes[dominant]}
     for k, v in kstats.items():
         v["GBps_algorithmic"] = round(per_kernel_bytes[k] / (v["ms_avg"] * 1e-3) / 1e9, 1)
-    # HBM bytes per launch measured with rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE scaled by a copy probe)
+    # HBM bytes per launch measured with rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE scaled by a copy
+    # probe), per time-split count of the voltage workgroups
     tfile = os.path.join(REPO, "profiles", "adaptive_384ch_hbm_traffic.json")
     if roofline and mode == lam.MODE_ADAPTIVE and shape.C == 384 and os.path.exists(tfile):
-        tr = json.load(open(tfile))["bytes_per_launch"]
-        roofline["traffic"] = tr.get(dominant)
+        spw = tuned["samples_per_wg"] if tuned else args.spw
+        if spw <= 0:  # the pipeline's default: about 1024 workgroups over the channels
+            target = max(1, -(-1024 // shape.C))
+            spw = -(-shape.T // target)
+        spw = -(-spw // 128) * 128
+        nsplit = -(-shape.T // spw)
+        ent = json.load(open(tfile))["by_nsplit"].get(str(nsplit))
+        if ent:
+            roofline["traffic"] = ent["bytes_per_launch"].get(dominant)
+            roofline["traffic_splits"] = nsplit
 
     learned = None
     if not args.no_learned:
