@@ -108,7 +108,7 @@ struct lam_ctx {
   // eigen-stage method
   int eig_method = LAM_EIG_PLAIN, refine_sweeps = 2, power = 0;
   float res_tol = 0.1f;
-  int eig_plain_threads = 256, eig_fallback_threads = 1024;
+  int eig_plain_threads = 256;
   int wprec = kWprecF16HiLo;
   int num_cus = 256;
   // subarrays of uneven size (persistent eigen launch)
@@ -325,7 +325,6 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
     e.orth = b.d_orth;
     e.wprec = ctx->wprec;
     e.plain_threads = ctx->eig_plain_threads;
-    e.fallback_threads = ctx->eig_fallback_threads;
     if (L.ready) {
       e.KW = L.KW;
       e.basis_out = L.d_basis;
@@ -340,16 +339,9 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
         e.prop_stride = L.KW * ctx->R;
       }
     }
-    if (eig_is_warm(e)) {
-      {
-        Prof pr(ctx, s, LAM_K_EIG);
-        HIPC(ctx, launch_eig(e, 1, s));
-      }
-      Prof pr(ctx, s, LAM_K_EIGFB);
-      HIPC(ctx, launch_eig(e, 2, s));
-    } else {
+    {
       Prof pr(ctx, s, LAM_K_EIG);
-      HIPC(ctx, launch_eig(e, 0, s));
+      HIPC(ctx, launch_eig(e, s));
     }
   }
   if (ctx->sub_G > 0) {  // subarrays: every (channel, pol, group) problem in one launch
@@ -956,11 +948,10 @@ int lam_set_weight_precision(lam_ctx *ctx, int precision) {
   return update_wop(ctx);
 }
 
-int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads, int fallback_threads) {
+int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads) {
   auto ok = [](int t) { return t == 64 || t == 128 || t == 256 || t == 512 || t == 1024; };
-  if (!ctx || !ok(plain_threads) || !ok(fallback_threads)) return LAM_ERR_ARG;
+  if (!ctx || !ok(plain_threads)) return LAM_ERR_ARG;
   ctx->eig_plain_threads = plain_threads;
-  ctx->eig_fallback_threads = fallback_threads;
   ctx->graphs_dirty = true;
   return LAM_OK;
 }
@@ -1149,7 +1140,7 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
       ctx->samples_per_wg = cand;
       double t = 0.0;
       if ((rc = time_sequence(ctx, samples, scales, nblocks, iters,
-                              {LAM_K_CORRBEAM, LAM_K_CORR, LAM_K_BEAM2B, LAM_K_EIG, LAM_K_EIGFB, LAM_K_FEAT,
+                              {LAM_K_CORRBEAM, LAM_K_CORR, LAM_K_BEAM2B, LAM_K_EIG, LAM_K_FEAT,
                                LAM_K_MLP, LAM_K_SUBARRAY},
                               t)))
         break;
@@ -1182,8 +1173,8 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
     }
     ctx->eig_plain_threads = best;
     ctx->eig_method = method;
-    // 3. warm start: refinement steps (rejections fall back to the plain solver and
-    //    show up in the time of the second launch)
+    // 3. warm start: refinement steps (rejections fall back to the plain solver
+    //    inside the warm launch and show up in its time)
     if (rc == LAM_OK && method != LAM_EIG_PLAIN && ctx->lr.ready) {
       int bst = ctx->refine_sweeps;
       double bt = 1e30;
@@ -1192,7 +1183,7 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
         double t = 0.0;
         if ((rc = stream ? time_stream(ctx, samples, scales, nblocks, iters, t)
                          : time_sequence(ctx, samples, scales, nblocks, iters,
-                                         {LAM_K_EIG, LAM_K_EIGFB, LAM_K_FEAT, LAM_K_MLP}, t)))
+                                         {LAM_K_EIG, LAM_K_FEAT, LAM_K_MLP}, t)))
           break;
         if (t < bt) {
           bt = t;
@@ -1215,7 +1206,6 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
     result[0] = ctx->samples_per_wg;
     result[1] = ctx->eig_plain_threads;
     result[2] = ctx->refine_sweeps;
-    result[3] = ctx->eig_fallback_threads;
   }
   return rc;
 }

@@ -762,8 +762,6 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   EigSmem S = carve(smem_raw, n, B, WARM);
   const int c = blockIdx.x >> 1, p = blockIdx.x & 1;
   const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1;
-  // second launch after a deferred fallback: only the problems whose warm solve failed
-  if (P.only_failed && !P.fallback[(long)c * 2 + p]) return;
   const long nbl = (long)n * (n + 1) / 2;
 
   // 1. dense Hermitian covariance of pol p (A(i,j) = V[bl(i,j)][p][p], i <= j)
@@ -773,10 +771,11 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
     while ((long)(j + 1) * (j + 2) / 2 <= idx) ++j;
     const int i = idx - j * (j + 1) / 2;
     long re = 0, im = 0;
-    for (int k = 0; k < P.nsplit; ++k) {
-      const int *v = P.vis_part + ((((long)k * P.C + c) * 4 + 3 * p) * nbl + idx) * 2;
-      re += v[0];
-      im += v[1];
+#pragma unroll 8
+    for (int k = 0; k < P.nsplit; ++k) {  // independent loads, issued together
+      const int2 v = reinterpret_cast<const int2 *>(P.vis_part)[(((long)k * P.C + c) * 4 + 3 * p) * nbl + idx];
+      re += v.x;
+      im += v.y;
     }
     const float2 a = make_float2((float)re, (float)im);
     S.A[j * ld + i] = a;
@@ -805,23 +804,21 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
     K = max(0, min(n, fixed));
     const bool ok = warm_solve(S.A, ld, n, KW, K, P.refine_sweeps, P.res_tol, S);
     sweeps = P.refine_sweeps;
-    if (!ok) {  // leave it to the second launch (plain solver)
-      if (tid == 0) {
-        P.fallback[cp] = 1;
-        if (P.sweeps) P.sweeps[cp] = sweeps;
-      }
-      return;
+    if (ok) {
+      if (tid == 0) S.kdet[0] = K;
+      __syncthreads();
+    } else {
+      // rejected: the plain solver on the same matrix (the warm solve leaves A
+      // untouched), in this workgroup, so no second launch is needed
+      sweeps += jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
+      sort_eigen(S.A, n, S.perm, S.ev);
+      K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
+      fell = 1;
     }
-    if (tid == 0) S.kdet[0] = K;
-    __syncthreads();
   } else {
     sweeps = jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
     sort_eigen(S.A, n, S.perm, S.ev);
     K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
-    if (P.only_failed) {
-      sweeps += P.refine_sweeps;
-      fell = 1;
-    }
   }
   // modes clearly above the noise floor (> 4x the mean of the remaining eigenvalues):
   // the label count of the warm-start training set
@@ -985,29 +982,19 @@ hipError_t launch_eig_plain(const EigParams &p, int nthr, hipStream_t s) {
   }
 }
 
-// Plain: one launch.  Warm start (fixed K only): the warm launch defers the
-// problems it rejects to a second launch that runs the plain solver on them.
-// part: 0 = everything, 1 = the first launch only, 2 = the second only.
+// One launch.  Warm start (fixed K only): a problem whose warm solve is
+// rejected runs the plain solver in the same workgroup.
 bool eig_is_warm(const EigParams &p) {
   return p.proposal && !p.detect && p.KW >= 2 && p.R >= 2 * p.KW;
 }
 
-hipError_t launch_eig(const EigParams &p, int part, hipStream_t s) {
+hipError_t launch_eig(const EigParams &p, hipStream_t s) {
   if (!eig_is_warm(p)) {
     EigParams q = p;
     q.proposal = nullptr;
-    q.only_failed = 0;
-    return part == 2 ? hipSuccess : launch_eig_plain(q, p.plain_threads, s);
+    return launch_eig_plain(q, p.plain_threads, s);
   }
-  if (part != 2) {
-    EigParams w = p;
-    w.only_failed = 0;
-    if (hipError_t e = launch_eig_t<256, true>(w, s)) return e;
-  }
-  if (part == 1) return hipSuccess;
-  EigParams f = p;
-  f.only_failed = 1;
-  return launch_eig_plain(f, p.fallback_threads, s);
+  return launch_eig_t<256, true>(p, s);
 }
 
 // ---- subarrays of uneven size: one persistent launch ----

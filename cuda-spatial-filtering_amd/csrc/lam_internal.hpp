@@ -99,8 +99,6 @@ struct EigParams {
   int *nsig;               // out: [C*2] modes above 4x the noise floor among the K removed, or nullptr
   float *orth;             // out: [C*2] FP64 check max |U_K^H U_K - I| of the removed (whitened) basis
   int plain_threads;       // workgroup width of the plain solver (256, 512, 1024)
-  int fallback_threads;    // workgroup width of the deferred fallback launch
-  int only_failed;         // set by launch_eig for the second launch of the warm start
   int wprec;               // precision of the [W; W'] beam operands (kWprec*)
 };
 
@@ -174,7 +172,7 @@ hipError_t launch_gather(const int *idx, int nb, const uint16_t *DX, int Dp, con
                          int nu, const int *DK, uint16_t *X, float2 *U, int *K, hipStream_t s);
 hipError_t launch_synth(const GenParams &g, hipStream_t s);
 hipError_t launch_volt(const VoltParams &p, bool do_corr, bool do_beam, hipStream_t s);
-hipError_t launch_eig(const EigParams &p, int part, hipStream_t s);
+hipError_t launch_eig(const EigParams &p, hipStream_t s);
 hipError_t launch_subarray_null(const SubParams &p, int max_wgs, hipStream_t s);
 bool eig_is_warm(const EigParams &p);
 hipError_t launch_prep_weights(const __half *w, const float2 *gains, const float2 *phase,

@@ -37,6 +37,7 @@ __global__ void __launch_bounds__(kFeatThreads) feature_kernel(FeatParams P) {
     re[r] = 0;
     im[r] = 0;
   }
+#pragma unroll 4
   for (int k = 0; k < P.nsplit; ++k) {
     const int2 *v = reinterpret_cast<const int2 *>(P.vis_part) + (((long)k * P.C + c) * 4 + 3 * p) * nbl;
 #pragma unroll

@@ -109,7 +109,7 @@ def load_library(path: str = LIB_PATH):
                                    ctypes.c_float, vp, vp, vp, vp]
     L.lam_set_eig_method.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float]
     L.lam_eig_stats.argtypes = [vp, vp, vp, vp]
-    L.lam_set_eig_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.lam_set_eig_tuning.argtypes = [vp, ctypes.c_int]
     L.lam_set_weight_precision.argtypes = [vp, ctypes.c_int]
     L.lam_set_subarrays.argtypes = [vp, ctypes.c_int, vp, vp]
     L.lam_subarray_results.argtypes = [vp, vp, vp, vp, vp, vp, vp]
@@ -340,10 +340,9 @@ class Pipeline:
         n = len(samples_list)
         sp = (ctypes.c_void_p * n)(*[_ptr(x) for x in samples_list])
         sc = (ctypes.c_void_p * n)(*[_ptr(x) for x in scales_list])
-        res = np.zeros(4, np.int32)
+        res = np.zeros(3, np.int32)
         self._chk(self.L.lam_autotune(self.h, sp, sc, n, iters, res.ctypes.data))
-        return dict(samples_per_wg=int(res[0]), plain_threads=int(res[1]), refine_steps=int(res[2]),
-                    fallback_threads=int(res[3]))
+        return dict(samples_per_wg=int(res[0]), plain_threads=int(res[1]), refine_steps=int(res[2]))
 
     def set_subarrays(self, mask, k):
         """mask [C][G][R] bool/uint8, k [C][2][G] int."""
@@ -369,8 +368,8 @@ class Pipeline:
     def set_weight_precision(self, precision):
         self._chk(self.L.lam_set_weight_precision(self.h, precision))
 
-    def set_eig_tuning(self, plain_threads=256, fallback_threads=1024):
-        self._chk(self.L.lam_set_eig_tuning(self.h, plain_threads, fallback_threads))
+    def set_eig_tuning(self, plain_threads=256):
+        self._chk(self.L.lam_set_eig_tuning(self.h, plain_threads))
 
     def eig_stats(self, with_orthogonality=False):
         s = self.shape

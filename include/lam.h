@@ -162,7 +162,9 @@ int lam_set_graphs(lam_ctx *ctx, int enable);
 
 /* Per-kernel timing: when enabled, HIP events bracket every kernel of each
  * block on its stream; lam_kernel_times returns the accumulated milliseconds
- * and launch counts per kernel id (see LAM_K_* below) and resets them. */
+ * and launch counts per kernel id (see LAM_K_* below) and resets them.
+ * LAM_K_EIGFB keeps its number but records nothing: a rejected warm solve is
+ * re-solved inside the warm launch. */
 enum {
   LAM_K_CORRBEAM = 0, LAM_K_CORR = 1, LAM_K_EIG = 2, LAM_K_BEAM2B = 3,
   LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_EIGFB = 6, LAM_K_SUBARRAY = 7, LAM_K_COUNT = 8
@@ -184,20 +186,22 @@ enum { LAM_WPREC_F16_HILO = 0, LAM_WPREC_F16 = 1, LAM_WPREC_BF16 = 2 };
 int lam_set_weight_precision(lam_ctx *ctx, int precision);
 
 /* Workgroup width (64, 128, 256, 512 or 1024 threads) of the plain Jacobi
- * launch and of the second launch that re-solves the problems a warm start
- * rejected. */
-int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads, int fallback_threads);
+ * launch. */
+int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads);
 
 /* Autotuner (no counterpart in the reference): times this context's kernels
  * on a sequence of nblocks device-resident blocks (samples[i], scales[i]),
  * `iters` timed passes after one untimed pass, and keeps the fastest
- *   - time samples per voltage workgroup: automatic, 1024, 2048, 3072, 4096, 8192;
- *   - plain Jacobi workgroup width: 256, 512, 1024 (adaptive mode);
+ *   - time samples per voltage workgroup: automatic, 1024, 2048, 3072, 4096,
+ *     8192, judged on the whole block sequence (more splits speed the voltage
+ *     kernels but multiply the partial visibilities the eigen stage reads);
+ *   - plain Jacobi workgroup width: 256, 512, 1024 (adaptive mode), judged on
+ *     streaming time per block when the context has several buffers;
  *   - warm-start refinement steps 1..3 when LAM_EIG_LEARNED / PREVIOUS is set
  *     (rejected problems fall back to the plain solver inside the timing, so a
  *     too-short refinement loses on time).
- * The blocks do not reach the visibility accumulator.  result[4] (may be NULL)
- * = {samples_per_wg, plain_threads, refine_steps, fallback_threads}. */
+ * The blocks do not reach the visibility accumulator.  result[3] (may be NULL)
+ * = {samples_per_wg, plain_threads, refine_steps}. */
 int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *scales, int nblocks,
                  int iters, int32_t *result);
 

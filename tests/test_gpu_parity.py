@@ -544,3 +544,33 @@ def test_overlapped_buffers_match_serial():
     s = _o(shape)
     ref = sum(orc.correlate(s, orc.ingest(s, r, sc, orc.WINDOW_ADAPTIVE)) for r, sc in blocks)
     assert np.array_equal(v3, ref)
+
+
+def test_warm_start_rejection_reruns_plain_solver():
+    """A warm start that fails its leakage test (stale proposal, tolerance 0)
+    re-solves the problem with the plain solver inside the same launch: the
+    outputs equal a plain-only run bit for bit and the fallback flags are set."""
+    shape = syn.BlockShape(C=4, F=1, NT=64, R=36, P=2, RPP=36, NP=8, B=4)
+    blocks = [syn.make_block(shape, syn.make_scene(shape, seed=71 + i, n_tones=2, amp=30.0), seed=81 + i)
+              for i in range(2)]
+    w16 = syn.random_weights(shape, seed=73)[0]
+
+    def run(warm):
+        p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+        p.set_weights(w16)
+        p.set_fixed_k(np.full((shape.C, 2), 2, np.int32))
+        p.learn_init(8, 256, seed=1)
+        p.run_host(*blocks[0])
+        if warm:
+            p.set_eig_method(lam.EIG_PREVIOUS, refine_sweeps=1, residual_tol=0.0)
+        out = p.run_host(*blocks[1])
+        sw, fb = p.eig_stats()
+        p.close()
+        return out, sw, fb
+
+    ref, sw0, fb0 = run(False)
+    got, sw1, fb1 = run(True)
+    assert fb0.sum() == 0 and np.all(fb1 == 1)
+    assert np.all(sw1 == sw0 + 1)  # the refinement step is counted with the plain sweeps
+    for key in ("beams", "evals_out", "evecs_out", "counts_out"):
+        assert np.array_equal(got[key], ref[key]), key
