@@ -218,10 +218,10 @@ __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_s
         rot[tid] = R;
       }
       __syncthreads();
-      // A <- J^H A J on blocks (kp <= kq); the (kq, kp) block is the conjugate transpose
+      // A <- J^H A J on blocks (kp <= kq), written to the upper triangle
 #pragma unroll
       for (int j = 0; j < kMaxBlk; ++j) {
-        if (j >= nmine) break;
+        if (j >= nmine) continue;
         const int code = blk[j];
         const int kp = code & 0xff, kq = code >> 8;
         const int2 P = rpq[kp], Q = rpq[kq];
@@ -771,8 +771,7 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
     while ((long)(j + 1) * (j + 2) / 2 <= idx) ++j;
     const int i = idx - j * (j + 1) / 2;
     long re = 0, im = 0;
-#pragma unroll 8
-    for (int k = 0; k < P.nsplit; ++k) {  // independent loads, issued together
+    for (int k = 0; k < P.nsplit; ++k) {
       const int2 v = reinterpret_cast<const int2 *>(P.vis_part)[(((long)k * P.C + c) * 4 + 3 * p) * nbl + idx];
       re += v.x;
       im += v.y;
