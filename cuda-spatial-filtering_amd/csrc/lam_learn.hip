@@ -21,38 +21,48 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ---- features: one workgroup per (channel, pol) ----
-// Each thread issues the loads of all its baselines (contiguous XX or YY run
-// of the partial visibilities, every split) before using any of them, so the
-// kernel waits on memory once; diagonal baselines give the whitening.
+// Each thread owns PER baselines of the problem's contiguous XX or YY run and
+// issues their loads for four splits at a time unconditionally (indices
+// clamped, surplus discarded), so the kernel waits on memory about
+// nsplit / 4 times; diagonal baselines give the whitening.
 constexpr int kFeatThreads = 256;
-constexpr int kFeatPer = (64 * 65 / 2 + kFeatThreads - 1) / kFeatThreads;  // baselines per thread, n <= 64
 
+template <int PER>
 __global__ void __launch_bounds__(kFeatThreads) feature_kernel(FeatParams P) {
   __shared__ float d[64];
   const int tid = threadIdx.x, cp = blockIdx.x, c = cp >> 1, p = cp & 1;
   const int n = P.n, nbl = n * (n + 1) / 2;
-  long re[kFeatPer], im[kFeatPer];
+  const int2 *vis = reinterpret_cast<const int2 *>(P.vis_part) + ((long)c * 4 + 3 * p) * nbl;
+  const long split_stride = (long)P.C * 4 * nbl;
+  int blc[PER];
+  long re[PER], im[PER];
 #pragma unroll
-  for (int r = 0; r < kFeatPer; ++r) {
+  for (int r = 0; r < PER; ++r) {
+    blc[r] = min(tid + r * kFeatThreads, nbl - 1);
     re[r] = 0;
     im[r] = 0;
   }
-#pragma unroll 4
-  for (int k = 0; k < P.nsplit; ++k) {
-    const int2 *v = reinterpret_cast<const int2 *>(P.vis_part) + (((long)k * P.C + c) * 4 + 3 * p) * nbl;
+  for (int k0 = 0; k0 < P.nsplit; k0 += 4) {
+    int2 x[4][PER];
 #pragma unroll
-    for (int r = 0; r < kFeatPer; ++r) {
-      const int bl = tid + r * kFeatThreads;
-      if (bl < nbl) {
-        const int2 x = v[bl];
-        re[r] += x.x;
-        im[r] += x.y;
+    for (int u = 0; u < 4; ++u) {
+      const int2 *v = vis + (long)(k0 + u < P.nsplit ? k0 + u : 0) * split_stride;
+#pragma unroll
+      for (int r = 0; r < PER; ++r) x[u][r] = v[blc[r]];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (k0 + u >= P.nsplit) break;
+#pragma unroll
+      for (int r = 0; r < PER; ++r) {
+        re[r] += x[u][r].x;
+        im[r] += x[u][r].y;
       }
     }
   }
-  int ii[kFeatPer], jj[kFeatPer];
+  int ii[PER], jj[PER];
 #pragma unroll
-  for (int r = 0; r < kFeatPer; ++r) {
+  for (int r = 0; r < PER; ++r) {
     const int bl = tid + r * kFeatThreads;
     int j = (int)((sqrtf(8.0f * bl + 1.0f) - 1.0f) * 0.5f);
     while (j * (j + 1) / 2 > bl) --j;
@@ -65,7 +75,7 @@ __global__ void __launch_bounds__(kFeatThreads) feature_kernel(FeatParams P) {
   uint32_t *x32 = reinterpret_cast<uint32_t *>(P.X + (long)cp * P.Dp);
   // whitened off-diagonal entries: baseline bl = j (j + 1) / 2 + i -> feature e = j (j - 1) / 2 + i
 #pragma unroll
-  for (int r = 0; r < kFeatPer; ++r) {
+  for (int r = 0; r < PER; ++r) {
     const int bl = tid + r * kFeatThreads, i = ii[r], j = jj[r];
     if (bl < nbl && i != j) {
       const float inv = 1.0f / (d[i] * d[j]);
@@ -84,8 +94,16 @@ __global__ void __launch_bounds__(kFeatThreads) feature_kernel(FeatParams P) {
 }
 
 hipError_t launch_features(const FeatParams &p, hipStream_t s) {
-  if (p.Dp % 2 || p.n > 64) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(feature_kernel, dim3(p.C * 2), dim3(kFeatThreads), 0, s, p);
+  if (p.Dp % 2 || p.n < 1 || p.n > 64 || p.nsplit < 1) return hipErrorInvalidValue;
+  const int per = (p.n * (p.n + 1) / 2 + kFeatThreads - 1) / kFeatThreads;  // 1..9
+  const dim3 g(p.C * 2), b(kFeatThreads);
+  switch (per) {
+#define LAM_FEAT(N) \
+  case N: hipLaunchKernelGGL(feature_kernel<N>, g, b, 0, s, p); break;
+    LAM_FEAT(1) LAM_FEAT(2) LAM_FEAT(3) LAM_FEAT(4) LAM_FEAT(5) LAM_FEAT(6) LAM_FEAT(7) LAM_FEAT(8) LAM_FEAT(9)
+#undef LAM_FEAT
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -220,19 +238,22 @@ mlp_l1_splitk_kernel(const uint16_t *__restrict__ X, int M, int K, const uint16_
   }
 }
 
-template <int NW, int NP, int KS>
+// NG output tiles of 32 per workgroup share each BF16 operand the wave forms.
+template <int NW, int NP, int KS, int NG>
 __global__ void __launch_bounds__(64 * NW)
 mlp_l2_kernel(const float *__restrict__ Zp, const float *__restrict__ b1, int M, int K,
               const uint16_t *__restrict__ Wt, const float *__restrict__ b2, int N, float *__restrict__ Y, Skip sk) {
-  __shared__ float red[NW][16][64];
+  __shared__ float red[NW][NG][16][64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32 * NG;
   const int row = lane & 31, kh = lane >> 5;
   const int m = m0 + row;
   const bool mv = m < M;
   const long zrow = (long)(mv ? m : 0) * K + kh * 8;
   const uint16_t *wb = Wt + (long)(n0 + row) * K + kh * 8;
-  v16f acc = {};
+  v16f acc[NG];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) acc[g] = v16f{};
   // K = 16 * KS * NW: every load of the wave is issued before the first MFMA
 #pragma unroll
   for (int u = 0; u < KS; ++u) {
@@ -251,21 +272,26 @@ mlp_l2_kernel(const float *__restrict__ Zp, const float *__restrict__ b1, int M,
     v8bf a;
 #pragma unroll
     for (int i = 0; i < 8; ++i) a[i] = __builtin_bit_cast(__bf16, f2bf(mv ? fmaxf(hv[i], 0.0f) : 0.0f));
-    const v8bf bb = *reinterpret_cast<const v8bf *>(wb + k0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const v8bf bb = *reinterpret_cast<const v8bf *>(wb + (long)g * 32 * K + k0);
+      acc[g] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[g], 0, 0, 0);
+    }
   }
 #pragma unroll
-  for (int v = 0; v < 16; ++v) red[w][v][lane] = acc[v];
+  for (int g = 0; g < NG; ++g)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) red[w][g][v][lane] = acc[g][v];
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 16 / NW; ++j) {
+  for (int j = 0; j < 16 * NG / NW; ++j) {
     const int o = threadIdx.x + 64 * NW * j;
-    const int l = o & 63, v = o >> 6;
+    const int l = o & 63, v = (o >> 6) & 15, g = o >> 10;
     float s = 0.0f;
 #pragma unroll
-    for (int q = 0; q < NW; ++q) s += red[q][v][l];
+    for (int q = 0; q < NW; ++q) s += red[q][g][v][l];
     const int i = 8 * (v >> 2) + 4 * (l >> 5) + (v & 3), col = l & 31;
-    const int mm = m0 + i, nn = n0 + col;
+    const int mm = m0 + i, nn = n0 + 32 * g + col;
     if (mm < M) {
       float y = s + b2[nn];
       if (sk.X && nn < sk.len) y += sk.scale * bf2f(sk.X[(long)mm * sk.ld + sk.off + nn]);
@@ -277,10 +303,10 @@ mlp_l2_kernel(const float *__restrict__ Zp, const float *__restrict__ b1, int M,
 hipError_t launch_mlp_infer(const uint16_t *X, int M, int K, const uint16_t *W1, const float *b1, int H,
                             const uint16_t *W2, const float *b2, int N, int nparts, float *Zp, uint16_t *Hbf,
                             float *Y, const Skip &sk, hipStream_t s) {
-  constexpr int kParts = 4, kL2Waves = 8;
+  constexpr int kParts = 4, kL2Waves = 4, kGroup = 3;
   const int ksw = H / (16 * kL2Waves);
-  if (nparts != kParts || H % (16 * kL2Waves) || !(ksw == 1 || ksw == 2 || ksw == 4)) {
-    // other hidden widths: the training forward kernels (hidden layer through memory)
+  if (nparts != kParts || H % (16 * kL2Waves) || !(ksw == 2 || ksw == 4 || ksw == 8) || (N / 32) % kGroup) {
+    // other shapes: the training forward kernels (hidden layer through memory)
     if (hipError_t e = launch_dense_fwd(X, M, K, W1, b1, H, nullptr, Hbf, Skip{}, s)) return e;
     return launch_dense_fwd(Hbf, M, H, W2, b2, N, Y, nullptr, sk, s);
   }
@@ -289,18 +315,16 @@ hipError_t launch_mlp_infer(const uint16_t *X, int M, int K, const uint16_t *W1,
   hipLaunchKernelGGL((mlp_l1_splitk_kernel<4, 8>), dim3((M + 31) / 32, H / 32, nparts), dim3(256), 0, s, X, M, K,
                      W1, H, kper, Zp);
   if (hipError_t e = hipGetLastError()) return e;
+  const dim3 g2((M + 31) / 32, N / (32 * kGroup)), b2d(64 * kL2Waves);
   switch (ksw) {  // K steps per wave in layer 2
-    case 1:
-      hipLaunchKernelGGL((mlp_l2_kernel<kL2Waves, kParts, 1>), dim3((M + 31) / 32, N / 32), dim3(64 * kL2Waves), 0,
-                         s, Zp, b1, M, H, W2, b2, N, Y, sk);
-      break;
     case 2:
-      hipLaunchKernelGGL((mlp_l2_kernel<kL2Waves, kParts, 2>), dim3((M + 31) / 32, N / 32), dim3(64 * kL2Waves), 0,
-                         s, Zp, b1, M, H, W2, b2, N, Y, sk);
+      hipLaunchKernelGGL((mlp_l2_kernel<kL2Waves, kParts, 2, kGroup>), g2, b2d, 0, s, Zp, b1, M, H, W2, b2, N, Y, sk);
       break;
     case 4:
-      hipLaunchKernelGGL((mlp_l2_kernel<kL2Waves, kParts, 4>), dim3((M + 31) / 32, N / 32), dim3(64 * kL2Waves), 0,
-                         s, Zp, b1, M, H, W2, b2, N, Y, sk);
+      hipLaunchKernelGGL((mlp_l2_kernel<kL2Waves, kParts, 4, kGroup>), g2, b2d, 0, s, Zp, b1, M, H, W2, b2, N, Y, sk);
+      break;
+    case 8:
+      hipLaunchKernelGGL((mlp_l2_kernel<kL2Waves, kParts, 8, kGroup>), g2, b2d, 0, s, Zp, b1, M, H, W2, b2, N, Y, sk);
       break;
   }
   return hipGetLastError();
