@@ -907,8 +907,11 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   }
   // 7. W' = W P with P = I - U_K U_K^H (null) or V diag(f) V^H (shrink):
   //    G(b,k) = sum_i W(b,i) u_k(i);  W'(b,i) = [null] W(b,i) + sum_k f_k G(b,k) conj(u_k(i))
-  for (int idx = tid; idx < B * n; idx += nthr) {
-    const int b = idx / n, k = idx % n;
+  //    null: f_k = 0 below the K removed modes, so only k >= n - K take part
+  //    (same sums, same order: the skipped terms are exact zeros)
+  const int klo = P.shrink ? 0 : n - K, nk = n - klo;
+  for (int idx = tid; idx < B * nk; idx += nthr) {
+    const int b = idx / nk, k = klo + idx % nk;
     float2 g = make_float2(0.f, 0.f);
     if (S.f[k] != 0.0f) {
       for (int i = 0; i < n; ++i) {
@@ -923,7 +926,7 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   for (int idx = tid; idx < B * n; idx += nthr) {
     const int b = idx / n, i = idx % n;
     float2 acc = P.shrink ? make_float2(0.f, 0.f) : S.W[b * n + i];
-    for (int k = 0; k < n; ++k) {
+    for (int k = klo; k < n; ++k) {
       const float2 t = cmulc(S.G[b * n + k], S.A[k * n + i]);
       acc.x += t.x;
       acc.y += t.y;
