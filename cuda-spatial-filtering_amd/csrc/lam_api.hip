@@ -36,6 +36,7 @@ struct Buf {
   int8_t *d_samples = nullptr;  // staging for host inputs
   int16_t *d_scales = nullptr;
   __half *d_beams = nullptr;
+  float *d_spectra = nullptr;   // adaptive: post-beam spectra [C][2][2B][NT-10]
   int *d_vis_part = nullptr;    // this block's partial visibilities [split][C][pq][NBL][2]
   v8h *d_wop2 = nullptr;        // adaptive: this block's 2B-beam weight operands
   float *d_evals = nullptr, *d_evecs = nullptr;
@@ -103,6 +104,7 @@ struct lam_ctx {
   hipEvent_t reset_done = nullptr;
   int blocks_acc = 0;
   bool use_graphs = true, graphs_dirty = true, profiling = false;
+  bool spectra = false;  // compute the post-beam spectra in the beam phase
   int samples_per_wg = 0;
   std::vector<ProfEvent> prof;
   // eigen-stage method
@@ -272,6 +274,10 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
     {
       Prof pr(ctx, s, LAM_K_BEAM2B);
       HIPC(ctx, launch_volt(pb, false, true, s));
+    }
+    if (ctx->spectra) {  // adaptive 808-848: FFT per packet, edge bins dropped, |X| summed
+      Prof pr(ctx, s, LAM_K_SPECTRA);
+      HIPC(ctx, launch_spectra(b.d_beams, ctx->C, 2 * ctx->B, ctx->T, ctx->NT, b.d_spectra, s));
     }
     HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, (int)ctx->NBL, ctx->d_vis_acc, atomic_acc, s));
     return LAM_OK;
@@ -743,6 +749,8 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
     chk(dalloc(ctx, &b.d_samples, ctx->samples_bytes));
     chk(dalloc(ctx, &b.d_scales, ctx->scales_bytes));
     chk(dalloc(ctx, &b.d_beams, ctx->beams_bytes));
+    if (mode == LAM_MODE_ADAPTIVE && ctx->NT > 10)
+      chk(dalloc(ctx, &b.d_spectra, (size_t)ctx->C * 2 * ctx->Bout * (ctx->NT - 10) * sizeof(float)));
     chk(dalloc(ctx, &b.d_vis_part, (size_t)kMaxSplits * vis_elems * sizeof(int)));
     if (mode == LAM_MODE_ADAPTIVE) {
       chk(dalloc(ctx, &b.d_wop2, (size_t)ctx->C * 2 * ctx->NMT2 * ctx->NK * 2 * 64 * sizeof(v8h)));
@@ -802,6 +810,7 @@ int lam_destroy(lam_ctx *ctx) {
     hipFree(b.d_samples);
     hipFree(b.d_scales);
     hipFree(b.d_beams);
+    hipFree(b.d_spectra);
     hipFree(b.d_vis_part);
     hipFree(b.d_wop2);
     hipFree(b.d_evals);
@@ -912,6 +921,16 @@ int lam_set_nulling(lam_ctx *ctx, int detect, float delta, int shrink) {
   return LAM_OK;
 }
 
+int lam_set_spectra(lam_ctx *ctx, int enable) {
+  if (!ctx) return LAM_ERR_ARG;
+  if (enable && (ctx->mode != LAM_MODE_ADAPTIVE || ctx->NT != 64 || ctx->T % 64 != 0 || !ctx->bufs[0].d_spectra))
+    return fail(ctx, LAM_ERR_ARG, "spectra need the adaptive mode and 64 time steps per packet");
+  if (int r = sync_all(ctx)) return r;
+  ctx->spectra = enable != 0;
+  ctx->graphs_dirty = true;
+  return LAM_OK;
+}
+
 int lam_set_eig_params(lam_ctx *ctx, int max_sweeps, float tol) {
   if (!ctx || max_sweeps < 0 || !(tol >= 0.0f)) return LAM_ERR_ARG;
   ctx->max_sweeps = max_sweeps;
@@ -984,6 +1003,7 @@ int lam_blocks_accumulated(const lam_ctx *ctx) { return ctx ? ctx->blocks_acc : 
 
 int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
   if (!ctx || !io || !io->samples || !io->scales) return LAM_ERR_ARG;
+  if (io->spectra_dst && !ctx->spectra) return fail(ctx, LAM_ERR_ARG, "spectra_dst given but spectra are off");
   Buf &b = ctx->bufs[ctx->cur];
   hipStream_t s = b.stream;
   const void *smp = io->samples, *scl = io->scales;
@@ -1022,6 +1042,12 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
       any = true;
     }
     if (any && io->eigen_done_fn) HIPC(ctx, hipLaunchHostFunc(s, io->eigen_done_fn, io->eigen_done_user));
+    if (io->spectra_dst) {
+      HIPC(ctx, hipMemcpyAsync(io->spectra_dst, b.d_spectra,
+                               (size_t)ctx->C * 2 * ctx->Bout * (ctx->NT - 10) * sizeof(float),
+                               hipMemcpyDefault, s));
+      if (io->spectra_done_fn) HIPC(ctx, hipLaunchHostFunc(s, io->spectra_done_fn, io->spectra_done_user));
+    }
   }
   HIPC(ctx, hipEventRecord(b.done, s));
   ctx->last_buf = ctx->cur;

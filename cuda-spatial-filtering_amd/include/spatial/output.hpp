@@ -53,6 +53,11 @@ class SingleHostMemoryOutput : public Output {
   using Eigenvectors =
       std::complex<float>[T::NR_CHANNELS][T::NR_POLARIZATIONS][T::NR_RECEIVERS][T::NR_RECEIVERS];
   using EigenmodeCounts = int32_t[T::NR_CHANNELS][T::NR_POLARIZATIONS];
+  // FFTOutputType (lambda_adaptive_beamformed_spectra_pipeline.hpp:77-80): per
+  // output beam, 64 - 2*5 fine-channel magnitudes summed over the block.
+  static constexpr size_t kFineChannels =
+      T::NR_TIME_STEPS_PER_PACKET > 10 ? T::NR_TIME_STEPS_PER_PACKET - 10 : 1;
+  using FFTOutput = float[T::NR_CHANNELS][T::NR_POLARIZATIONS][NR_OUT_BEAMS][kFineChannels];
 
   BeamOutput *beam_data;
   Visibilities *visibilities;
@@ -60,6 +65,7 @@ class SingleHostMemoryOutput : public Output {
   Eigenvalues *eigenvalues;
   Eigenvectors *eigenvectors;
   EigenmodeCounts *nulled_eigenmode_counts;
+  FFTOutput *fft_output;
 
   size_t register_beam_data_block(const size_t, const size_t) override { return 1; }
   size_t register_visibilities_block(const size_t, const size_t, const int missing,
@@ -80,17 +86,17 @@ class SingleHostMemoryOutput : public Output {
     return nulled_eigenmode_counts;
   }
   void *get_beam_counts_landing_pointer(const size_t) override { return nulled_eigenmode_counts; }
-  void *get_fft_landing_pointer(const size_t) override { return nullptr; }
+  void *get_fft_landing_pointer(const size_t) override { return fft_output; }
 
   void register_beam_data_transfer_complete(const size_t) override { ++beam_blocks_done; }
   void register_visibilities_transfer_complete(const size_t) override { ++vis_blocks_done; }
   void register_arrivals_transfer_complete(const size_t) override {}
   void register_eigendecomposition_data_transfer_complete(const size_t) override { ++eig_blocks_done; }
   void register_beam_counts_transfer_complete(const size_t) override {}
-  void register_fft_transfer_complete(const size_t) override {}
+  void register_fft_transfer_complete(const size_t) override { ++fft_blocks_done; }
 
   int last_missing_packets = 0, last_total_packets = 0;
-  int beam_blocks_done = 0, vis_blocks_done = 0, eig_blocks_done = 0;
+  int beam_blocks_done = 0, vis_blocks_done = 0, eig_blocks_done = 0, fft_blocks_done = 0;
 
   SingleHostMemoryOutput() {
     beam_data = alloc<BeamOutput>();
@@ -99,6 +105,7 @@ class SingleHostMemoryOutput : public Output {
     eigenvalues = alloc<Eigenvalues>();
     eigenvectors = alloc<Eigenvectors>();
     nulled_eigenmode_counts = alloc<EigenmodeCounts>();
+    fft_output = alloc<FFTOutput>();
   }
   ~SingleHostMemoryOutput() override {
     lam_host_free(beam_data);
@@ -107,6 +114,7 @@ class SingleHostMemoryOutput : public Output {
     lam_host_free(eigenvalues);
     lam_host_free(eigenvectors);
     lam_host_free(nulled_eigenmode_counts);
+    lam_host_free(fft_output);
   }
 
  private:

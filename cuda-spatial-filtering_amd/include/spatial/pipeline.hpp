@@ -76,6 +76,12 @@ inline void beam_done_host_func(void *data) {
   delete ctx;
 }
 
+inline void fft_done_host_func(void *data) {  // pipeline/common.hpp:453-459
+  auto *ctx = static_cast<OutputDoneContext *>(data);
+  ctx->output->register_fft_transfer_complete(ctx->block_index);
+  delete ctx;
+}
+
 inline void vis_done_host_func(void *data) {
   auto *ctx = static_cast<OutputDoneContext *>(data);
   ctx->output->register_visibilities_transfer_complete(ctx->block_index);
@@ -270,6 +276,10 @@ class LambdaAdaptiveBeamformedSpectraPipeline : public GPUPipeline {
     ctx_(lam_set_fixed_k(ctx_.get(), k.data()));
     ctx_(lam_set_nulling(ctx_.get(), detect_signal_eigenmodes_, detection_threshold_delta_,
                          shrink_eigenvalues_));
+    // post-beam spectra: 64-point FFT per packet (adaptive 1046-1050, 808-848)
+    static_assert(T::NR_TIME_STEPS_PER_PACKET == 64,
+                  "the adaptive pipeline's 64-point spectra need 64 time steps per packet");
+    ctx_(lam_set_spectra(ctx_.get(), 1));
     for (int i = 0; i < num_buffers_; ++i) {
       auto *p = static_cast<int32_t *>(lam_host_alloc(sizeof(int32_t) * kCounts));
       if (!p) throw std::runtime_error("lam_host_alloc failed");
@@ -340,6 +350,12 @@ class LambdaAdaptiveBeamformedSpectraPipeline : public GPUPipeline {
         ectx->eig_block = eig_block;
         ectx->eig_registered = true;
         ectx->eig_counts_dst = output_->get_nulled_eigenmode_counts_landing_pointer(eig_block);
+      }
+      const size_t fft_block = output_->register_fft_block(start_seq_num, end_seq_num);
+      if (fft_block != std::numeric_limits<size_t>::max()) {  // adaptive 900-912
+        io.spectra_dst = static_cast<float *>(output_->get_fft_landing_pointer(fft_block));
+        io.spectra_done_fn = spatial::detail::fft_done_host_func;
+        io.spectra_done_user = new spatial::detail::OutputDoneContext{output_, fft_block};
       }
       io.counts_dst = counts_staging_[current_buffer_];
       io.eigen_done_fn = spatial::detail::eigen_done_host_func;

@@ -25,7 +25,7 @@ LAM_OK, LAM_ERR_ARG, LAM_ERR_HIP, LAM_ERR_RANGE, LAM_ERR_STATE, LAM_ERR_UNSUPPOR
 MODE_CORR_BEAM, MODE_ADAPTIVE = 0, 1
 K_CORRBEAM, K_CORR, K_EIG, K_BEAM2B = range(4)
 KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp", "eig_fallback",
-                "subarray_eig"]
+                "subarray_eig", "spectra"]
 EIG_PLAIN, EIG_LEARNED, EIG_PREVIOUS = 0, 1, 2
 WPREC_F16_HILO, WPREC_F16, WPREC_BF16 = 0, 1, 2
 
@@ -56,6 +56,7 @@ class LamBlockIO(ctypes.Structure):
         ("beams_dst", ctypes.c_void_p), ("beams_done_fn", HOSTFN), ("beams_done_user", ctypes.c_void_p),
         ("eigenvalues_dst", ctypes.c_void_p), ("eigenvectors_dst", ctypes.c_void_p),
         ("counts_dst", ctypes.c_void_p), ("eigen_done_fn", HOSTFN), ("eigen_done_user", ctypes.c_void_p),
+        ("spectra_dst", ctypes.c_void_p), ("spectra_done_fn", HOSTFN), ("spectra_done_user", ctypes.c_void_p),
     ]
 
 
@@ -85,6 +86,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_set_stream_permutation.argtypes = [vp, vp, vp, ctypes.c_size_t]
     L.lam_set_fixed_k.argtypes = [vp, vp]
     L.lam_set_nulling.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_int]
+    L.lam_set_spectra.argtypes = [vp, ctypes.c_int]
     L.lam_set_eig_params.argtypes = [vp, ctypes.c_int, ctypes.c_float]
     L.lam_set_graphs.argtypes = [vp, ctypes.c_int]
     L.lam_set_profiling.argtypes = [vp, ctypes.c_int]
@@ -218,6 +220,7 @@ class Pipeline:
             raise LamError(rc, "lam_create failed")
         self.h = h
         self._keep = []
+        self.spectra = False
 
     # -- plumbing
     def _chk(self, rc):
@@ -270,6 +273,15 @@ class Pipeline:
     def set_nulling(self, detect=False, delta=3.0, shrink=False):
         self._chk(self.L.lam_set_nulling(self.h, int(bool(detect)), float(delta), int(bool(shrink))))
 
+    def set_spectra(self, enable=True):
+        """Post-beam spectra (adaptive mode, 64 time steps per packet)."""
+        self._chk(self.L.lam_set_spectra(self.h, int(bool(enable))))
+        self.spectra = bool(enable)
+
+    def spectra_array(self):
+        s = self.shape
+        return np.zeros((s.C, 2, 2 * s.B, s.NT - 10), np.float32)
+
     def set_eig_params(self, max_sweeps=12, tol=1e-6):
         self._chk(self.L.lam_set_eig_params(self.h, int(max_sweeps), float(tol)))
 
@@ -284,7 +296,7 @@ class Pipeline:
 
     # -- per block
     def execute(self, samples, scales, beams_out=None, evals_out=None, evecs_out=None,
-                counts_out=None, on_device=None):
+                counts_out=None, on_device=None, spectra_out=None):
         """Enqueue one block.  samples/scales: numpy (host) or torch (device)."""
         if on_device is None:
             on_device = not isinstance(samples, np.ndarray)
@@ -299,6 +311,8 @@ class Pipeline:
         io.eigenvectors_dst = _ptr(evecs_out)
         io.counts_dst = _ptr(counts_out)
         io.eigen_done_fn = _NULLFN
+        io.spectra_dst = _ptr(spectra_out)
+        io.spectra_done_fn = _NULLFN
         self._chk(self.L.lam_execute(self.h, ctypes.byref(io)))
 
     def sync(self):
@@ -456,6 +470,8 @@ class Pipeline:
             extra = dict(evals_out=np.zeros((s.C, 2, s.R), np.float32),
                          evecs_out=np.zeros((s.C, 2, s.R, s.R, 2), np.float32),
                          counts_out=np.zeros((s.C, 2), np.int32))
+            if self.spectra:
+                extra["spectra_out"] = self.spectra_array()
         self.execute(np.ascontiguousarray(ring, np.int8), np.ascontiguousarray(scales, np.int16),
                      beams_out=beams, **extra)
         self.sync()

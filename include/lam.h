@@ -24,6 +24,9 @@
  *   eigenvalues  float32  [C][P][R]     ascending       adaptive 41-45, 914-950
  *   eigenvectors float32  [C][P][R][R][2] column-major (eigenvector k at [k][0..R))
  *   counts       int32    [C][P]                        adaptive 709-715
+ *   spectra      float32  [C][P][2B][NT-10]             adaptive 808-848, FFTOutputType
+ *                sum over the block's packets of |64-point FFT| of each beam
+ *                packet, 5 edge bins dropped each side, DC centred
  *
  * Threading: a context is used from one host thread (the reference's
  * pipeline_feeder, spatial.hpp:1275-1299).  Host callbacks run on the HIP
@@ -110,6 +113,16 @@ int lam_set_stream_permutation(lam_ctx *ctx, const int32_t *recv_perm,
 int lam_set_fixed_k(lam_ctx *ctx, const int32_t *k);
 /* Null/shrink and detection switches (adaptive ctor 1014-1020, 687-757). */
 int lam_set_nulling(lam_ctx *ctx, int detect, float delta, int shrink);
+
+/* Post-beam spectra of the adaptive pipeline (register_fft_block,
+ * lambda_adaptive_beamformed_spectra_pipeline.hpp:808-848, 1173-1178): when
+ * enabled, each block also computes, per (channel, pol, output beam), the
+ * 64-point forward FFT of every packet's time series, keeps the 54 bins
+ * that FineChannelCopy keeps (DC centred), and sums their magnitudes over
+ * the block's packets into the spectra layout above.  Needs NT = 64.  Off
+ * by default (it reads the whole beam output once more per block).
+ * LAM_ERR_ARG in corr-beam mode or for another NT. */
+int lam_set_spectra(lam_ctx *ctx, int enable);
 /* Eigensolver controls: maximum Jacobi sweeps and convergence tolerance on
  * the relative off-diagonal Frobenius norm. */
 int lam_set_eig_params(lam_ctx *ctx, int max_sweeps, float tol);
@@ -131,6 +144,9 @@ typedef struct lam_block_io {
   int32_t *counts_dst;
   lam_host_fn eigen_done_fn;
   void *eigen_done_user;
+  float *spectra_dst;      /* adaptive with lam_set_spectra(ctx, 1) only (NULL: skip) */
+  lam_host_fn spectra_done_fn;
+  void *spectra_done_user;
 } lam_block_io;
 
 /* Enqueue one block on the next buffer's stream (execute_pipeline,
@@ -167,7 +183,8 @@ int lam_set_graphs(lam_ctx *ctx, int enable);
  * re-solved inside the warm launch. */
 enum {
   LAM_K_CORRBEAM = 0, LAM_K_CORR = 1, LAM_K_EIG = 2, LAM_K_BEAM2B = 3,
-  LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_EIGFB = 6, LAM_K_SUBARRAY = 7, LAM_K_COUNT = 8
+  LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_EIGFB = 6, LAM_K_SUBARRAY = 7, LAM_K_SPECTRA = 8,
+  LAM_K_COUNT = 9
 };
 int lam_set_profiling(lam_ctx *ctx, int enable);
 int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);

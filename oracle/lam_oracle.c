@@ -361,6 +361,33 @@ void orc_fine_delay_table(int C, int R, int N, const float *delays_ns,
   }
 }
 
+/* --------------------------------------------------------- beam spectra */
+void orc_beam_spectra(int C, int P, int E, int np, int nfft, int drop,
+                      const double *beams, double *out) {
+  const int keep = nfft / 2 - drop, nout = nfft - 2 * drop;
+  const long rows = (long)C * P * E;
+  for (long row = 0; row < rows; ++row) {
+    const double *x = beams + 2 * row * (long)np * nfft;
+    double *o = out + row * nout;
+    for (int g = 0; g < nout; ++g) o[g] = 0.0;
+    for (int pk = 0; pk < np; ++pk) {
+      const double *xp = x + 2L * pk * nfft;
+      for (int g = 0; g < nout; ++g) {
+        const int u = g < keep ? g + nfft / 2 + drop : g - keep;
+        double re = 0.0, im = 0.0;
+        for (int t = 0; t < nfft; ++t) {
+          /* exact phase index: u*t mod nfft, so the twiddle is one of nfft values */
+          const double a = -2.0 * M_PI * (double)((long)u * t % nfft) / nfft;
+          const double c = cos(a), s = sin(a);
+          re += xp[2 * t] * c - xp[2 * t + 1] * s;
+          im += xp[2 * t] * s + xp[2 * t + 1] * c;
+        }
+        o[g] += sqrt(re * re + im * im);
+      }
+    }
+  }
+}
+
 /* ------------------------------------------------------------ block driver */
 int orc_rfi_null_cp(int R, int P, int B, const int64_t *vis_c, int p,
                     const double *w_cp, int fixed_k, double delta, int shrink,

@@ -127,6 +127,21 @@ int orc_adaptive_block(const orc_shape *s, const int32_t *y, const double *w,
                        int shrink, int64_t *vis, double *beams2b,
                        double *evals, double *evecs, int32_t *counts);
 
+/* Post-beam spectra of the adaptive pipeline (register_fft_block product):
+ * for every (c, p, beam e) and packet o, the unnormalised forward DFT
+ *   X[u] = sum_t x[o*nfft + t] exp(-2 pi i u t / nfft)
+ * of the packet's nfft samples (cuFFT C2C forward, adaptive 812-814,
+ * 1046-1050), drop `drop` bins each side of nfft/2 and put the negative
+ * frequencies first (FineChannelCopy / FineChannelSeek flat copies over the
+ * frequency-outermost tensor, adaptive 66-76, 816-833, 461-464):
+ *   out bin g < keep: u = g + nfft/2 + drop;  g >= keep: u = g - keep,
+ *   keep = nfft/2 - drop,
+ * and sum |X[u]| over the packets (sum_fft_over_packets, spatial.cuh:1508-1567).
+ * beams: [C][P][E][np*nfft][2]; out: [C][P][E][nfft - 2*drop]. Direct O(n^2)
+ * DFT in FP64 (no FFT factorisation) so that it checks the FFT independently. */
+void orc_beam_spectra(int C, int P, int E, int np, int nfft, int drop,
+                      const double *beams, double *out);
+
 /* Number of OpenMP threads the oracle would use (1 when built without it). */
 int orc_num_threads(void);
 

@@ -97,6 +97,7 @@ def _declare(L):
                                      ctypes.c_double, ctypes.c_int, _p, _p, _p, _p, _p]
     L.orc_adaptive_block.restype = ctypes.c_int
     L.orc_num_threads.restype = ctypes.c_int
+    L.orc_beam_spectra.argtypes = [ctypes.c_int] * 6 + [_p, _p]
 
 
 def _ptr(a):
@@ -265,6 +266,18 @@ def adaptive_block(shape: Shape, y, w, fixed_k=None, detect=False, delta=3.0, sh
                              _ptr(V), _ptr(counts))
     return dict(vis=vis, beams2b=beams[..., 0] + 1j * beams[..., 1], evals=ev,
                 evecs=V[..., 0] + 1j * V[..., 1], counts=counts)
+
+
+def beam_spectra(beams, nfft=64, drop=5):
+    """beams complex [C][P][E][T] -> float64 [C][P][E][nfft - 2*drop]: per packet
+    DFT magnitudes with the edge bins dropped, DC centred, summed over packets."""
+    beams = np.asarray(beams)
+    C, P, E, T = beams.shape
+    assert T % nfft == 0
+    b = _c(np.stack([beams.real, beams.imag], -1), np.float64)
+    out = np.empty((C, P, E, nfft - 2 * drop), np.float64)
+    lib().orc_beam_spectra(C, P, E, T // nfft, nfft, drop, _ptr(b), _ptr(out))
+    return out
 
 
 def num_threads() -> int:

@@ -284,6 +284,18 @@ TEST(AdaptiveNullRankOne) {
     }
   CHECK((*r.output->nulled_eigenmode_counts)[0][0] == 1, "count");
   CHECK(r.output->eig_blocks_done == 1 && r.output->beam_blocks_done == 1, "callbacks");
+  // spectra: a constant beam is all DC, which lands on output bin 27 (adaptive 816-833)
+  CHECK(r.output->fft_blocks_done == 1, "fft callback");
+  auto &fft = *r.output->fft_output;
+  const float dc = 64.0f * std::sqrt(128.0f);
+  for (int p = 0; p < 2; ++p)
+    for (int g = 0; g < 54; ++g) {
+      CHECK(std::isfinite(fft[0][p][0][g]) && std::isfinite(fft[0][p][1][g]), "finite");
+      if (g == 27)
+        CHECK(std::fabs(fft[0][p][0][g] - dc) < 1e-3f * dc, "dc %f", fft[0][p][0][g]);
+      else
+        CHECK(fft[0][p][0][g] < 1e-3f * dc, "bin %d = %f", g, fft[0][p][0][g]);
+    }
   CHECK(r.state.released.size() == 1, "released once");
 }
 

@@ -362,6 +362,80 @@ def test_adaptive_scene_vs_oracle(shrink):
             assert np.max(np.abs(Ug @ Ug.conj().T - Ur @ Ur.conj().T)) < 1e-4
 
 
+def test_adaptive_spectra_vs_oracle():
+    """Post-beam spectra (register_fft_block product, adaptive 808-848): the
+    GPU spectra of the GPU's own fp16 beams match the FP64 DFT oracle within
+    1e-5 of each row's largest bin, and the FP64 oracle's beams within 1e-3
+    (the reference transforms its FP32 beams; ours reads the fp16 output, so
+    the fp16 rounding of the beams is the whole difference).  Two blocks in a
+    row through the captured graphs, with three buffers."""
+    shape = syn.BlockShape(C=4, F=1, NT=64, R=36, P=2, RPP=36, NP=4, B=4)
+    _log("adaptive spectra scene")
+    scene = syn.make_scene(shape, seed=31, n_tones=2, inr_db=(10, 25))
+    w16, w = syn.random_weights(shape, 33)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE, num_buffers=3)
+    p.set_weights(w16)
+    p.set_fixed_k(np.full((shape.C, 2), 2))
+    p.set_eig_params(20, 1e-7)
+    p.set_spectra(True)
+    s = _o(shape)
+    for blk in range(2):
+        ring, scales = syn.make_block(shape, scene, seed=40 + blk, time_offset=blk * shape.T)
+        out = p.run_host(ring, scales)
+        sg = out["spectra_out"].astype(np.float64)
+        own = orc.beam_spectra(out["beams"].astype(np.complex128))
+        rowmax = np.max(own, axis=-1, keepdims=True)
+        assert np.max(np.abs(sg - own) / rowmax) < 1e-5
+        y = orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)
+        ref = orc.adaptive_block(s, y, w, fixed_k=np.full((shape.C, 2), 2))
+        fp64 = orc.beam_spectra(ref["beams2b"])
+        assert rel_err(sg, fp64) < BEAM_TOL
+    p.close()
+
+
+def test_adaptive_spectra_tone_bins():
+    """A tone at DFT bin k on every receiver peaks at output bin k + 27
+    (k < 27) or k - 37 (k >= 37) in both beams with K = 0; bins 27..36 are
+    removed, so such a tone leaves only leakage from int8 rounding."""
+    for k in (0, 5, 20, 40, 63, 30):
+        def tone(c, f, pk, t, r, pol, k=k):
+            a = 2 * np.pi * k * t / 64
+            return (int(round(40 * np.cos(a))), int(round(40 * np.sin(a))))
+        ring, scales = syn.ring_from_fn(ADAPT, tone)
+        p = lam.Pipeline(ADAPT, lam.MODE_ADAPTIVE)
+        p.set_weights(syn.unity_weights(ADAPT)[0])
+        p.set_fixed_k(np.zeros((1, 2)))
+        p.set_spectra(True)
+        out = p.run_host(ring, scales)
+        p.close()
+        sp = out["spectra_out"]
+        own = orc.beam_spectra(out["beams"].astype(np.complex128))
+        assert np.max(np.abs(sp - own)) <= 1e-5 * np.max(own)
+        for pol in range(2):
+            for b in range(2):
+                row = sp[0, pol, b]
+                if 27 <= k <= 36:
+                    assert np.max(row) < 0.02 * 4 * 40 * 64
+                else:
+                    assert int(np.argmax(row)) == (k + 27 if k < 27 else k - 37)
+                    assert abs(row.max() - 4 * 40 * 64) < 0.01 * 4 * 40 * 64
+
+
+def test_spectra_argument_errors():
+    """Spectra exist only in adaptive mode; asking for them when off is an error."""
+    p = lam.Pipeline(SMALL, lam.MODE_CORR_BEAM)
+    with pytest.raises(lam.LamError):
+        p.set_spectra(True)
+    p.close()
+    p = lam.Pipeline(ADAPT, lam.MODE_ADAPTIVE)
+    ring, scales = syn.ring_from_fn(ADAPT, lambda *a: (1, 0))
+    with pytest.raises(lam.LamError):
+        p.execute(ring, scales, spectra_out=p.spectra_array())
+    p.set_spectra(True)
+    p.set_spectra(False)
+    p.close()
+
+
 def test_adaptive_delays_and_permutation():
     """apply_delays (spatial.cuh:1420-1506) and reorder_streams
     (spatial.cuh:593-649) folded into the voltage loads; K=0 beams = W x."""

@@ -350,3 +350,42 @@ def test_null_depth_synthetic_scene():
             W2 = orc.rfi_null_cp(12, 2, out["vis"][c], p, w[c, p], fixed_k=1)[0]
             depth = 10 * np.log10(np.abs(W2[2:] @ a) ** 2 / np.abs(W2[:2] @ a) ** 2)
             assert np.all(depth < -20)
+
+
+# ------------------------------------------------------------ beam spectra
+def _tone(k, n=64, amp=1.0, phase=0.0):
+    t = np.arange(n)
+    return amp * np.exp(1j * (2 * np.pi * k * t / n + phase))
+
+
+def test_beam_spectra_tone_closed_form():
+    """A complex exponential at DFT bin k has |X[k]| = n*amp and zero
+    elsewhere.  Kept bins land at g = k + 27 (k < 27) or k - 37 (k >= 37),
+    i.e. DC at output bin 27 and negative frequencies first (adaptive 816-833
+    flat copies over the frequency-outermost tensor, 461-464); bins 27..36 are
+    dropped.  A sign error in the exponent or a swapped half fails here."""
+    for k in range(64):
+        x = _tone(k, amp=3.0, phase=0.3 * k)[None, None, None, :]
+        out = orc.beam_spectra(x)[0, 0, 0]
+        if 27 <= k <= 36:
+            assert np.max(out) < 1e-9, k
+            continue
+        g = k + 27 if k < 27 else k - 37
+        assert abs(out[g] - 192.0) < 1e-9, (k, out[g])
+        assert np.max(np.delete(out, g)) < 1e-9, k
+
+
+def test_beam_spectra_packets_sum_and_library_fft():
+    """Per-packet magnitudes are summed (sum_fft_over_packets, spatial.cuh:
+    1508-1567), and each packet's spectrum equals numpy's FFT."""
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 2, 3, 4 * 64)) + 1j * rng.standard_normal((2, 2, 3, 4 * 64))
+    out = orc.beam_spectra(x)
+    X = np.fft.fft(x.reshape(2, 2, 3, 4, 64), axis=-1)
+    ref = np.concatenate([np.abs(X[..., 37:]), np.abs(X[..., :27])], axis=-1).sum(axis=3)
+    assert np.max(np.abs(out - ref)) < 1e-10 * np.max(ref)
+    # packets are independent: spectra of a concatenation are the sum of spectra
+    a = orc.beam_spectra(x[..., :128]) + orc.beam_spectra(x[..., 128:])
+    assert np.allclose(a, out, rtol=1e-12)
+    # |X| is positively homogeneous and phase-blind
+    assert np.allclose(orc.beam_spectra(2.5 * np.exp(0.7j) * x), 2.5 * out, rtol=1e-12)
