@@ -86,6 +86,43 @@ def cpu_baseline(shape, ring, scales, w, channels, min_seconds=10.0, max_passes=
                 seconds=dt)
 
 
+def run_spectra(p, shape, d_ring, d_scales, args):
+    """Adaptive pipeline with the post-beam spectra on (the reference's
+    register_fft_block product): streaming ms/step, and the spectra kernel's
+    own duration against the HBM bytes it must move (read the 2B fp16 beams,
+    write the [C][2][2B][54] float spectra)."""
+    n = max(10, args.steps // 3)
+    p.set_spectra(True)
+    for _ in range(4):
+        p.execute(d_ring, d_scales, on_device=True)
+    p.sync()
+    p.dump_visibilities_i64()
+    t0 = time.perf_counter()
+    for i in range(n):
+        p.execute(d_ring, d_scales, on_device=True)
+    p.sync()
+    t1 = time.perf_counter()
+    p.dump_visibilities_i64()
+    p.set_profiling(True)
+    p.kernel_times()
+    for i in range(n):
+        p.execute(d_ring, d_scales, on_device=True)
+        p.sync()
+    kt = p.kernel_times()
+    p.set_profiling(False)
+    p.set_spectra(False)
+    p.dump_visibilities_i64()
+    ms, launches = kt.get("spectra", (0.0, 0))
+    k_ms = ms / max(1, launches)
+    nbytes = shape.C * 2 * 2 * shape.B * shape.T * 4 + shape.C * 2 * 2 * shape.B * (shape.NT - 10) * 4
+    gbs = nbytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else None
+    sec = (t1 - t0) / n
+    return {"ms_per_step": sec * 1e3, "real_time_factor": (shape.T / CHANNEL_RATE_HZ) / sec,
+            "kernel_ms": k_ms, "algorithmic_bytes": nbytes,
+            "GBps_algorithmic": None if gbs is None else round(gbs, 1),
+            "hbm_frac": None if gbs is None else round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def run_learned(args):
     """Learned warm start on 384 ch x 2 pol x 36 ant with drifting, switching
     interferers: blocks [0, train) run the plain solver and are captured as the
@@ -404,6 +441,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-subarrays", action="store_true")
+    ap.add_argument("--no-spectra", action="store_true")
     ap.add_argument("--spw", type=int, default=0, help="samples per voltage workgroup (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -474,6 +512,10 @@ def main():
     orth_max = None
     if mode == lam.MODE_ADAPTIVE:
         orth_max = float(p.eig_stats(with_orthogonality=True)[2].max())
+    spectra = None
+    if mode == lam.MODE_ADAPTIVE and not args.no_spectra:
+        spectra = run_spectra(p, shape, d_ring, d_scales, args)
+        log(f"spectra: {spectra}")
     p.close()
 
     sec = (t1 - t0) / args.steps
@@ -560,6 +602,7 @@ def main():
         "autotune": tuned,
         "learned_vs_plain": learned,
         "precision_sweep": precision,
+        "spectra_on": spectra,
         "uneven_subarrays": subarrays,
         "cpu_baseline": cpu,
     }

@@ -96,6 +96,7 @@ struct lam_ctx {
   bool perm_set = false, delays_set = false;
   v8h *d_wop = nullptr;  // corr-beam: B-beam operands
   int *d_fixed_k = nullptr;
+  void *d_spec_table = nullptr;  // DFT operand fragments of the spectra kernel
   int detect = 0, shrink = 0, max_sweeps = 12;
   float delta = 3.0f, tol = 1e-6f;
   int *d_err = nullptr;
@@ -277,7 +278,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
     }
     if (ctx->spectra) {  // adaptive 808-848: FFT per packet, edge bins dropped, |X| summed
       Prof pr(ctx, s, LAM_K_SPECTRA);
-      HIPC(ctx, launch_spectra(b.d_beams, ctx->C, 2 * ctx->B, ctx->T, ctx->NT, b.d_spectra, s));
+      HIPC(ctx, launch_spectra(b.d_beams, ctx->d_spec_table, ctx->C, 2 * ctx->B, ctx->T, ctx->NT, b.d_spectra, s));
     }
     HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, (int)ctx->NBL, ctx->d_vis_acc, atomic_acc, s));
     return LAM_OK;
@@ -772,6 +773,13 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
   chk(dalloc(ctx, &ctx->d_pol_perm, (size_t)ctx->R * 2 * sizeof(int)));
   chk(dalloc(ctx, &ctx->d_wop, (size_t)ctx->C * 2 * ctx->NMT1 * ctx->NK * 2 * 64 * sizeof(v8h)));
   chk(dalloc(ctx, &ctx->d_fixed_k, (size_t)ctx->C * 2 * sizeof(int)));
+  if (mode == LAM_MODE_ADAPTIVE && ctx->NT == 64 && rc == LAM_OK) {
+    std::vector<uint16_t> tab(spectra_table_bytes() / 2);
+    spectra_table_host(tab.data());
+    if (chk(dalloc(ctx, &ctx->d_spec_table, spectra_table_bytes())) &&
+        hipMemcpy(ctx->d_spec_table, tab.data(), spectra_table_bytes(), hipMemcpyHostToDevice) != hipSuccess)
+      rc = LAM_ERR_HIP;
+  }
   chk(dalloc(ctx, &ctx->d_err, sizeof(int)));
   chk(dalloc(ctx, &ctx->d_dump_f32, vis_elems * sizeof(float)));
   chk(dalloc(ctx, &ctx->d_dump_i64, vis_elems * sizeof(long long)));
@@ -831,6 +839,7 @@ int lam_destroy(lam_ctx *ctx) {
   hipFree(ctx->d_pol_perm);
   hipFree(ctx->d_wop);
   hipFree(ctx->d_fixed_k);
+  hipFree(ctx->d_spec_table);
   hipFree(ctx->d_err);
   hipFree(ctx->d_dump_f32);
   hipFree(ctx->d_dump_i64);
@@ -923,7 +932,8 @@ int lam_set_nulling(lam_ctx *ctx, int detect, float delta, int shrink) {
 
 int lam_set_spectra(lam_ctx *ctx, int enable) {
   if (!ctx) return LAM_ERR_ARG;
-  if (enable && (ctx->mode != LAM_MODE_ADAPTIVE || ctx->NT != 64 || ctx->T % 64 != 0 || !ctx->bufs[0].d_spectra))
+  if (enable && (ctx->mode != LAM_MODE_ADAPTIVE || ctx->NT != 64 || ctx->T % 64 != 0 || !ctx->bufs[0].d_spectra ||
+                 !ctx->d_spec_table))
     return fail(ctx, LAM_ERR_ARG, "spectra need the adaptive mode and 64 time steps per packet");
   if (int r = sync_all(ctx)) return r;
   ctx->spectra = enable != 0;

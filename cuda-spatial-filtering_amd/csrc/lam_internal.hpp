@@ -147,7 +147,13 @@ struct FeatParams {
 // Launchers (defined in the .hip files).
 hipError_t launch_features(const FeatParams &p, hipStream_t s);
 // Post-beam spectra: out [C][2][nbeams][NT - 10] (NT = 64) from FP16 beams [C][2][nbeams][T].
-hipError_t launch_spectra(const __half *beams, int C, int nbeams, int T, int NT, float *out, hipStream_t s);
+// Post-beam spectra: 64-point DFT magnitudes per packet, 54 kept bins, summed
+// over packets.  table: spectra_table_bytes() of device memory filled from
+// spectra_table_host().
+size_t spectra_table_bytes();
+void spectra_table_host(uint16_t *dst);
+hipError_t launch_spectra(const __half *beams, const void *table, int C, int nbeams, int T, int NT, float *out,
+                          hipStream_t s);
 struct Skip {
   const uint16_t *X = nullptr;  // BF16 rows holding the skip source
   int ld = 0, off = 0, len = 0;
