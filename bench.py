@@ -123,6 +123,36 @@ def run_spectra(p, shape, d_ring, d_scales, args):
             "hbm_frac": None if gbs is None else round(gbs / HBM_PEAK_GBS, 4)}
 
 
+def run_host_fed(p, shape, ring, scales, args):
+    """Blocks from pinned host buffers recycled on the pipeline's release
+    callback (the reference's ProcessorState -> execute_pipeline path): each
+    step includes the block's H2D copy, overlapped with the previous blocks'
+    kernels on the pipeline's buffers.  The buffers keep the same synthetic
+    block (a reassembly engine writes them in place as packets arrive, so its
+    copy is not part of a step)."""
+    n = max(8, args.steps // 4)
+    hr = lam.HostRing(p, args.buffers)
+    for i in range(args.buffers):
+        hr.samples[i].array[...] = ring
+        hr.scales[i].array[...] = scales
+    for _ in range(args.buffers):
+        hr.submit(hr.acquire())
+    p.sync()
+    p.dump_visibilities_i64()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        hr.submit(hr.acquire())
+    p.sync()
+    t1 = time.perf_counter()
+    p.dump_visibilities_i64()
+    hr.close()
+    sec = (t1 - t0) / n
+    nbytes = int(np.prod(shape.ring_shape())) + 2 * int(np.prod(shape.scale_shape()))
+    return {"ms_per_step": sec * 1e3, "real_time_factor": (shape.T / CHANNEL_RATE_HZ) / sec,
+            "h2d_bytes_per_step": nbytes, "h2d_GBps_effective": round(nbytes / sec / 1e9, 1),
+            "steps": n, "buffers": args.buffers}
+
+
 def run_learned(args):
     """Learned warm start on 384 ch x 2 pol x 36 ant with drifting, switching
     interferers: blocks [0, train) run the plain solver and are captured as the
@@ -442,6 +472,7 @@ def main():
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-subarrays", action="store_true")
     ap.add_argument("--no-spectra", action="store_true")
+    ap.add_argument("--no-host-fed", action="store_true")
     ap.add_argument("--spw", type=int, default=0, help="samples per voltage workgroup (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -512,6 +543,10 @@ def main():
     orth_max = None
     if mode == lam.MODE_ADAPTIVE:
         orth_max = float(p.eig_stats(with_orthogonality=True)[2].max())
+    host_fed = None
+    if not args.no_host_fed:
+        host_fed = run_host_fed(p, shape, ring, scales, args)
+        log(f"host-fed: {host_fed}")
     spectra = None
     if mode == lam.MODE_ADAPTIVE and not args.no_spectra:
         spectra = run_spectra(p, shape, d_ring, d_scales, args)
@@ -602,6 +637,7 @@ def main():
         "autotune": tuned,
         "learned_vs_plain": learned,
         "precision_sweep": precision,
+        "host_fed_pcie_inclusive": host_fed,
         "spectra_on": spectra,
         "uneven_subarrays": subarrays,
         "cpu_baseline": cpu,

@@ -3,6 +3,7 @@
 // (output.hpp:60-164) with pinned buffers from lam_host_alloc.  The buffered
 // writer machinery (BufferedOutput, HDF5/PSRDADA writers) is not part of this path.
 #pragma once
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -96,7 +97,8 @@ class SingleHostMemoryOutput : public Output {
   void register_fft_transfer_complete(const size_t) override { ++fft_blocks_done; }
 
   int last_missing_packets = 0, last_total_packets = 0;
-  int beam_blocks_done = 0, vis_blocks_done = 0, eig_blocks_done = 0, fft_blocks_done = 0;
+  // completions, counted from the HIP callback thread(s)
+  std::atomic<int> beam_blocks_done{0}, vis_blocks_done{0}, eig_blocks_done{0}, fft_blocks_done{0};
 
   SingleHostMemoryOutput() {
     beam_data = alloc<BeamOutput>();

@@ -9,7 +9,9 @@ raises on import of the library.
 """
 from __future__ import annotations
 
+import collections
 import ctypes
+import threading
 import os
 import re
 
@@ -188,10 +190,80 @@ def device_count() -> int:
     return load_library().lam_device_count()
 
 
+class HostArray:
+    """Pinned host buffer (lam_host_alloc) viewed as a numpy array: H2D copies
+    from it are DMA at full link speed and run asynchronously on the
+    pipeline's streams (LambdaFinalPacketData's cudaHostAlloc buffers,
+    packet_formats.hpp:131-142)."""
+
+    def __init__(self, shape, dtype):
+        self.L = load_library()
+        dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(shape)) * dtype.itemsize
+        self.ptr = self.L.lam_host_alloc(self.nbytes)
+        if not self.ptr:
+            raise LamError(LAM_ERR_HIP, f"lam_host_alloc({self.nbytes}) failed")
+        raw = (ctypes.c_uint8 * self.nbytes).from_address(self.ptr)
+        self.array = np.frombuffer(raw, dtype=dtype).reshape(shape)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            self.L.lam_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HostRing:
+    """Pinned input blocks recycled through the pipeline's release callback:
+    the buffer life cycle of the reference's ProcessorState
+    (include/spatial/spatial.hpp:718-770).  acquire() waits for a buffer the
+    pipeline has finished copying to the device; submit() enqueues it
+    (lam_execute with release_fn), so the producer fills one buffer while
+    the H2D copies and kernels of the others run."""
+
+    def __init__(self, pipeline, num_buffers):
+        self.p = pipeline
+        s = pipeline.shape
+        self.samples = [HostArray(s.ring_shape(), np.int8) for _ in range(num_buffers)]
+        self.scales = [HostArray(s.scale_shape(), np.int16) for _ in range(num_buffers)]
+        self.free = collections.deque(range(num_buffers))
+        self.cv = threading.Condition()
+        self.released = 0
+        self._cb = HOSTFN(self._release)
+
+    def _release(self, user):
+        with self.cv:
+            self.free.append(int(user or 0))
+            self.released += 1
+            self.cv.notify()
+
+    def acquire(self, timeout=120.0):
+        with self.cv:
+            if not self.cv.wait_for(lambda: len(self.free) > 0, timeout):
+                raise TimeoutError("no pinned buffer released")
+            return self.free.popleft()
+
+    def submit(self, i, **outputs):
+        self.p.execute(self.samples[i].array, self.scales[i].array, on_device=False,
+                       release_fn=self._cb, release_user=i, **outputs)
+
+    def close(self):
+        for a in self.samples + self.scales:
+            a.free()
+
+
 def _ptr(x):
     """Raw address of a numpy array (host), DeviceArray or torch tensor (device)."""
     if x is None:
         return None
+    if isinstance(x, HostArray):
+        return x.ptr
     if isinstance(x, np.ndarray):
         assert x.flags.c_contiguous
         return x.ctypes.data
@@ -296,7 +368,7 @@ class Pipeline:
 
     # -- per block
     def execute(self, samples, scales, beams_out=None, evals_out=None, evecs_out=None,
-                counts_out=None, on_device=None, spectra_out=None):
+                counts_out=None, on_device=None, spectra_out=None, release_fn=None, release_user=None):
         """Enqueue one block.  samples/scales: numpy (host) or torch (device)."""
         if on_device is None:
             on_device = not isinstance(samples, np.ndarray)
@@ -304,7 +376,8 @@ class Pipeline:
         io.samples = _ptr(samples)
         io.scales = _ptr(scales)
         io.inputs_on_device = int(on_device)
-        io.release_fn = _NULLFN
+        io.release_fn = _NULLFN if release_fn is None else release_fn
+        io.release_user = release_user
         io.beams_dst = _ptr(beams_out)
         io.beams_done_fn = _NULLFN
         io.eigenvalues_dst = _ptr(evals_out)

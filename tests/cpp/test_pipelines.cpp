@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "spatial/block_ring.hpp"
 #include "spatial/output.hpp"
 #include "spatial/packet_formats.hpp"
 #include "spatial/pipeline.hpp"
@@ -217,7 +218,7 @@ TEST(CorrBeamAutomaticDumpAndStateChecks) {
   r.run(const_sample, unit_scale, false);
   CHECK(r.output->vis_blocks_done == 0, "no dump after 1 block");
   r.run(const_sample, unit_scale, false);
-  CHECK(r.output->vis_blocks_done == 1, "dump after 2 blocks (%d)", r.output->vis_blocks_done);
+  CHECK(r.output->vis_blocks_done == 1, "dump after 2 blocks (%d)", r.output->vis_blocks_done.load());
   CHECK((*r.output->visibilities)[0][bl(1, 1)][0][0][0] == 128.0f, "2 blocks accumulated");
   CHECK(r.output->last_total_packets == 2, "total packets %d", r.output->last_total_packets);
   BeamWeightsT<Small> w = unity_weights<Small>();
@@ -361,6 +362,48 @@ TEST(AdaptivePermutationSizeError) {
   CHECK(spatial::half_to_float((*r.output->beam_data)[0][0][0][0][0]) == 8.0f, "identity perm");
 }
 
+// Overlapped ingest (ProcessorState -> pinned buffers -> async H2D,
+// spatial.hpp:718-770, 1182-1299): 12 blocks through 3 pinned buffers, the
+// producer refilling each buffer the moment the pipeline releases it.  A
+// release issued before that block's H2D copy finished would let the next
+// block's data into the copy and break the exact visibility sum.
+TEST(BlockRingOverlappedIngest) {
+  constexpr int kBlocks = 12;
+  auto output = std::make_shared<SingleHostMemoryOutput<Small>>();
+  BeamWeightsT<Small> weights = unity_weights<Small>();
+  LambdaCorrBeamOnlyGPUPipeline<Small> pipeline(3, &weights);
+  pipeline.set_output(output);
+  BlockRing<Small> ring(3);
+  ring.start(pipeline);
+  for (int b = 0; b < kBlocks; ++b) {
+    const size_t i = ring.acquire();
+    fill_block<Small>(ring.block(i),
+                      [b](size_t, size_t, int, int, int, int) { return std::complex<int8_t>{(int8_t)(b + 1), 0}; },
+                      unit_scale);
+    ring.submit(i, 100 * (uint64_t)b, 100 * (uint64_t)b + 99);
+  }
+  ring.stop();
+  pipeline.dump_visibilities();
+  pipeline.synchronize();
+  CHECK(ring.blocks_executed() == (size_t)kBlocks, "executed %zu", ring.blocks_executed());
+  CHECK(ring.blocks_released() == (size_t)kBlocks, "released %zu", ring.blocks_released());
+  CHECK(ring.missing_packets() == 0, "missing");
+  float expect = 0.0f;
+  for (int b = 0; b < kBlocks; ++b) expect += 8.0f * (float)((b + 1) * (b + 1));
+  auto &vis = *output->visibilities;
+  for (size_t bl_ = 0; bl_ < Small::NR_BASELINES_UNPADDED; ++bl_)
+    for (size_t p = 0; p < 2; ++p)
+      for (size_t q = 0; q < 2; ++q)
+        CHECK(vis[0][bl_][p][q][0] == expect && vis[0][bl_][p][q][1] == 0.0f, "vis %f", vis[0][bl_][p][q][0]);
+  // the landing area holds one whole block's beams (4 receivers x (b + 1))
+  auto &beam = *output->beam_data;
+  const float b0 = spatial::half_to_float(beam[0][0][0][0][0]);
+  CHECK(b0 >= 4.0f && b0 <= 4.0f * kBlocks && std::fmod(b0, 4.0f) == 0.0f, "beam %f", b0);
+  for (size_t p = 0; p < 2; ++p)
+    for (size_t t = 0; t < 8; ++t) CHECK(spatial::half_to_float(beam[0][p][0][t][0]) == b0, "whole block");
+  CHECK(output->beam_blocks_done == kBlocks, "beam callbacks %d", output->beam_blocks_done.load());
+}
+
 int main() {
   RUN(CorrBeamExactValues);
   RUN(CorrBeamZeroInputAndWeights);
@@ -371,6 +414,7 @@ int main() {
   RUN(AdaptiveK0TaggedLayout);
   RUN(AdaptiveShrinkAndDetection);
   RUN(AdaptivePermutationSizeError);
+  RUN(BlockRingOverlappedIngest);
   std::printf("%d checks, %d failures\n", g_checks, g_fail);
   return g_fail == 0 ? 0 : 1;
 }
