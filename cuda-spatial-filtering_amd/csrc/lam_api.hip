@@ -237,10 +237,19 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
   hipStream_t s = b.stream;
   const long long vis_count = (long long)ctx->C * ctx->NBL * 8;
   const bool atomic_acc = ctx->nbuf > 1;
-  if (ctx->mode == LAM_MODE_CORR_BEAM) {
+  if (ctx->mode == LAM_MODE_CORR_BEAM || ctx->mode == LAM_MODE_LAMBDA_GPU) {
     if (phase != 0) return LAM_OK;
     VoltParams p = base_volt(ctx, samples, scales);
-    p.window_base = 0;  // ring slots 0..NP-1 (spatial.cuh:254-264)
+    if (ctx->mode == LAM_MODE_CORR_BEAM) {
+      p.window_base = 0;  // ring slots 0..NP-1 (spatial.cuh:254-264)
+    } else {
+      // LambdaGPUPipeline: apply_delays + reorder_streams (lambda_gpu_pipeline.hpp:620-637),
+      // the window of the adaptive pipeline: slot 1 + per-FPGA delay (spatial.cuh:1431-1436)
+      p.window_base = ctx->NT;
+      p.delays = ctx->delays_set ? ctx->d_delays : nullptr;
+      p.recv_perm = ctx->perm_set ? ctx->d_recv_perm : nullptr;
+      p.pol_perm = ctx->perm_set ? ctx->d_pol_perm : nullptr;
+    }
     p.vis_part = b.d_vis_part;
     p.wop = ctx->d_wop;
     p.nbeams_out = ctx->B;
@@ -249,6 +258,11 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
     {
       Prof pr(ctx, s, LAM_K_CORRBEAM);
       HIPC(ctx, launch_volt(p, true, true, s));
+    }
+    if (ctx->mode == LAM_MODE_LAMBDA_GPU) {  // lambda_gpu_pipeline.hpp:494-501
+      Prof pr(ctx, s, LAM_K_POLEIG);
+      HIPC(ctx, launch_pol_eig(b.d_vis_part, p.nsplit, ctx->C, ctx->R, ctx->d_gains, ctx->d_phase,
+                               ctx->max_sweeps, ctx->tol, b.d_evals, b.d_evecs, nullptr, s));
     }
     HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, (int)ctx->NBL, ctx->d_vis_acc, atomic_acc, s));
     return LAM_OK;
@@ -471,7 +485,7 @@ int dalloc(lam_ctx *ctx, T **p, size_t bytes) {
 }
 
 int update_wop(lam_ctx *ctx) {
-  if (ctx->mode != LAM_MODE_CORR_BEAM) return LAM_OK;
+  if (ctx->mode == LAM_MODE_ADAPTIVE) return LAM_OK;
   hipStream_t s = ctx->bufs[0].stream;
   HIPC(ctx, launch_prep_weights(ctx->d_weights, ctx->d_gains, ctx->d_phase, ctx->C, ctx->B,
                                 ctx->R, ctx->NMT1, ctx->NK, ctx->wprec, ctx->d_wop, s));
@@ -690,7 +704,7 @@ const char *lam_last_error(const lam_ctx *ctx) { return ctx ? ctx->err.c_str() :
 int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
   if (!sh || !out) return LAM_ERR_ARG;
   *out = nullptr;
-  if (mode != LAM_MODE_CORR_BEAM && mode != LAM_MODE_ADAPTIVE) return LAM_ERR_ARG;
+  if (mode != LAM_MODE_CORR_BEAM && mode != LAM_MODE_ADAPTIVE && mode != LAM_MODE_LAMBDA_GPU) return LAM_ERR_ARG;
   if (sh->nr_polarizations != 2) return LAM_ERR_UNSUPPORTED;
   if (sh->nr_channels < 1 || sh->nr_fpga_sources < 1 || sh->nr_time_steps_per_packet < 1 ||
       sh->nr_receivers_per_packet < 1 || sh->nr_packets_for_correlation < 1 || sh->nr_beams < 1)
@@ -701,7 +715,7 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
   const long T = (long)sh->nr_packets_for_correlation * sh->nr_time_steps_per_packet;
   if (T > 65536) return LAM_ERR_UNSUPPORTED;  // int32 per-block accumulators stay exact
   const int Bout = mode == LAM_MODE_ADAPTIVE ? 2 * sh->nr_beams : sh->nr_beams;
-  if (Bout > 64 || (mode == LAM_MODE_CORR_BEAM && sh->nr_beams > 32)) return LAM_ERR_UNSUPPORTED;
+  if (Bout > 64 || (mode != LAM_MODE_ADAPTIVE && sh->nr_beams > 32)) return LAM_ERR_UNSUPPORTED;
 
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return LAM_ERR_NO_DEVICE;
@@ -753,6 +767,10 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
     if (mode == LAM_MODE_ADAPTIVE && ctx->NT > 10)
       chk(dalloc(ctx, &b.d_spectra, (size_t)ctx->C * 2 * ctx->Bout * (ctx->NT - 10) * sizeof(float)));
     chk(dalloc(ctx, &b.d_vis_part, (size_t)kMaxSplits * vis_elems * sizeof(int)));
+    if (mode == LAM_MODE_LAMBDA_GPU) {  // eigenpairs of the four pol pairs
+      chk(dalloc(ctx, &b.d_evals, (size_t)ctx->C * 4 * ctx->R * sizeof(float)));
+      chk(dalloc(ctx, &b.d_evecs, (size_t)ctx->C * 4 * ctx->R * ctx->R * 2 * sizeof(float)));
+    }
     if (mode == LAM_MODE_ADAPTIVE) {
       chk(dalloc(ctx, &b.d_wop2, (size_t)ctx->C * 2 * ctx->NMT2 * ctx->NK * 2 * 64 * sizeof(v8h)));
       chk(dalloc(ctx, &b.d_evals, (size_t)ctx->C * 2 * ctx->R * sizeof(float)));
@@ -1033,6 +1051,15 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
   if (io->beams_dst) {
     HIPC(ctx, hipMemcpyAsync(io->beams_dst, b.d_beams, ctx->beams_bytes, hipMemcpyDefault, s));
     if (io->beams_done_fn) HIPC(ctx, hipLaunchHostFunc(s, io->beams_done_fn, io->beams_done_user));
+  }
+  if (ctx->mode == LAM_MODE_LAMBDA_GPU && (io->eigenvalues_dst || io->eigenvectors_dst)) {
+    if (io->eigenvalues_dst)
+      HIPC(ctx, hipMemcpyAsync(io->eigenvalues_dst, b.d_evals, (size_t)ctx->C * 4 * ctx->R * sizeof(float),
+                               hipMemcpyDefault, s));
+    if (io->eigenvectors_dst)
+      HIPC(ctx, hipMemcpyAsync(io->eigenvectors_dst, b.d_evecs,
+                               (size_t)ctx->C * 4 * ctx->R * ctx->R * 2 * sizeof(float), hipMemcpyDefault, s));
+    if (io->eigen_done_fn) HIPC(ctx, hipLaunchHostFunc(s, io->eigen_done_fn, io->eigen_done_user));
   }
   if (ctx->mode == LAM_MODE_ADAPTIVE) {
     bool any = false;

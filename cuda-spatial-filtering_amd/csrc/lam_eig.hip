@@ -1154,6 +1154,84 @@ eigh_batched_kernel(const float2 *Ain, int n, int whiten, int max_sweeps, float 
   if (tid == 0 && sweeps_out) sweeps_out[m] = sw;
 }
 
+// ---- every pol-pair covariance of one block (LambdaGPUPipeline) ----
+// LambdaGPUPipeline decomposes each (channel, p, q) covariance of the block,
+// batch C*P*P (lambda_gpu_pipeline.hpp:438-439, 494-501): visCorrToDecomp and
+// unpack_triangular_baseline_batch (spatial.cuh:673-713) place V[bl(i,j)][p][q]
+// at A(i, j), i <= j, and cuSOLVER (fill UPPER) decomposes the Hermitian matrix
+// with that upper triangle.  Its samples carry the antenna gains
+// (scale_and_convert_to_half in LambdaPipelineIngest, pipeline/common.hpp:
+// 505-536), so here A(i, j) = g_ip conj(g_jq) e^{i(phi_i - phi_j)} V, the same
+// factor the visibility dump applies.  Outputs: eigenvalues ascending
+// [C][P][P][n]; eigenvector k at [C][P][P][k][0..n), scaled so that its
+// largest-magnitude component (the first on ties) is real and positive.
+__global__ void __launch_bounds__(kEigThreads)
+pol_eig_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *gains, const float2 *phase,
+               int max_sweeps, float tol, float *evals, float2 *evecs, int *sweeps_out) {
+  extern __shared__ __align__(16) char smem_raw[];
+  EigSmem S = carve(smem_raw, n, 1, false);
+  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
+  const int c = m >> 2, pq = m & 3, p = pq >> 1, q = pq & 1;
+  const int nbl = n * (n + 1) / 2;
+  for (int idx = tid; idx < nbl; idx += blockDim.x) {
+    int j = (int)((sqrtf(8.0f * (float)idx + 1.0f) - 1.0f) * 0.5f);
+    while (j * (j + 1) / 2 > idx) --j;
+    while ((j + 1) * (j + 2) / 2 <= idx) ++j;
+    const int i = idx - j * (j + 1) / 2;
+    long long re = 0, im = 0;
+    for (int k = 0; k < nsplit; ++k) {
+      const int2 v = reinterpret_cast<const int2 *>(vis_part)[(((long)k * C + c) * 4 + pq) * nbl + idx];
+      re += v.x;
+      im += v.y;
+    }
+    const float2 gi = gains[((long)c * 2 + p) * n + i], gj = gains[((long)c * 2 + q) * n + j];
+    const float2 pi = phase[(long)c * n + i], pj = phase[(long)c * n + j];
+    const float2 f = cmul(make_float2(gi.x * gj.x + gi.y * gj.y, gi.y * gj.x - gi.x * gj.y),
+                          make_float2(pi.x * pj.x + pi.y * pj.y, pi.y * pj.x - pi.x * pj.y));
+    S.A[j * ld + i] = cmul(f, make_float2((float)re, (float)im));
+  }
+  __syncthreads();
+  const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.rpq, S.rot, S.red);
+  sort_eigen(S.A, n, S.perm, S.ev);
+  float2 *ph = S.W;  // per output column: conj(v_max) / |v_max|
+  if (tid < n) {
+    const float2 *col = S.V + S.perm[tid] * ld;
+    int best = 0;
+    float bm = -1.0f;
+    for (int r = 0; r < n; ++r) {
+      const float mg = col[r].x * col[r].x + col[r].y * col[r].y;
+      if (mg > bm) {
+        bm = mg;
+        best = r;
+      }
+    }
+    const float2 v = col[best];
+    const float inv = bm > 0.0f ? rsqrtf(bm) : 0.0f;
+    ph[tid] = bm > 0.0f ? make_float2(v.x * inv, -v.y * inv) : make_float2(1.0f, 0.0f);
+  }
+  __syncthreads();
+  if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int k = e / n, r = e - k * n;
+    evecs[(long)m * n * n + e] = cmul(S.V[S.perm[k] * ld + r], ph[k]);
+  }
+  if (tid == 0 && sweeps_out) sweeps_out[m] = sw;
+}
+
+hipError_t launch_pol_eig(const int *vis_part, int nsplit, int C, int n, const float2 *gains,
+                          const float2 *phase, int max_sweeps, float tol, float *evals, float *evecs,
+                          int *sweeps, hipStream_t s) {
+  const size_t sm = eig_smem_bytes(n, 1, false);
+  if (sm > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(pol_eig_kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(pol_eig_kernel, dim3(C * 4), dim3(kEigThreads), sm, s, vis_part, nsplit, C, n, gains,
+                     phase, max_sweeps, tol, evals, reinterpret_cast<float2 *>(evecs), sweeps);
+  return hipGetLastError();
+}
+
 // Warm-start solve of whitened Hermitian matrices (test entry): proposal
 // [nmat][KW][n], outputs ascending eigenvalues / eigenvectors (columns) and
 // the leakage ratio of the K largest modes (accepted when <= tol).

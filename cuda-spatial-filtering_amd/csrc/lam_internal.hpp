@@ -150,6 +150,11 @@ hipError_t launch_features(const FeatParams &p, hipStream_t s);
 // Post-beam spectra: 64-point DFT magnitudes per packet, 54 kept bins, summed
 // over packets.  table: spectra_table_bytes() of device memory filled from
 // spectra_table_host().
+// Eigendecomposition of all four pol-pair covariances of one block
+// (LambdaGPUPipeline): evals [C][2][2][n], evecs [C][2][2][n][n][2].
+hipError_t launch_pol_eig(const int *vis_part, int nsplit, int C, int n, const float2 *gains,
+                          const float2 *phase, int max_sweeps, float tol, float *evals, float *evecs,
+                          int *sweeps, hipStream_t s);
 size_t spectra_table_bytes();
 void spectra_table_host(uint16_t *dst);
 hipError_t launch_spectra(const __half *beams, const void *table, int C, int nbeams, int T, int NT, float *out,

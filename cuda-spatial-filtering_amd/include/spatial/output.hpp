@@ -48,11 +48,12 @@ class SingleHostMemoryOutput : public Output {
   using Visibilities = float[T::NR_CHANNELS][T::NR_BASELINES_UNPADDED][T::NR_POLARIZATIONS]
                             [T::NR_POLARIZATIONS][2];
   using Arrivals = bool[T::NR_CHANNELS][T::NR_PACKETS_FOR_CORRELATION + 2][T::NR_FPGA_SOURCES];
-  // The adaptive pipeline writes eigenvalues [C][P][R] and eigenvectors
-  // [C][P][R][R] into these (lambda_adaptive_beamformed_spectra_pipeline.hpp:914-950).
-  using Eigenvalues = float[T::NR_CHANNELS][T::NR_POLARIZATIONS][T::NR_RECEIVERS];
-  using Eigenvectors =
-      std::complex<float>[T::NR_CHANNELS][T::NR_POLARIZATIONS][T::NR_RECEIVERS][T::NR_RECEIVERS];
+  // EigenvalueOutputType / EigenvectorOutputType (output.hpp:72-73): the
+  // LambdaGPU pipeline fills [C][P][P][R] and [C][P][P][R][R]; the adaptive
+  // pipeline writes its [C][P][R] and [C][P][R][R] at the start of the same
+  // areas (lambda_adaptive_beamformed_spectra_pipeline.hpp:914-950).
+  using Eigenvalues = typename T::EigenvalueOutputType;
+  using Eigenvectors = typename T::EigenvectorOutputType;
   using EigenmodeCounts = int32_t[T::NR_CHANNELS][T::NR_POLARIZATIONS];
   // FFTOutputType (lambda_adaptive_beamformed_spectra_pipeline.hpp:77-80): per
   // output beam, 64 - 2*5 fine-channel magnitudes summed over the block.

@@ -82,6 +82,12 @@ inline void fft_done_host_func(void *data) {  // pipeline/common.hpp:453-459
   delete ctx;
 }
 
+inline void eig_output_done_host_func(void *data) {  // pipeline/common.hpp:437-443
+  auto *ctx = static_cast<OutputDoneContext *>(data);
+  ctx->output->register_eigendecomposition_data_transfer_complete(ctx->block_index);
+  delete ctx;
+}
+
 inline void vis_done_host_func(void *data) {
   auto *ctx = static_cast<OutputDoneContext *>(data);
   ctx->output->register_visibilities_transfer_complete(ctx->block_index);
@@ -233,6 +239,128 @@ class LambdaCorrBeamOnlyGPUPipeline : public GPUPipeline {
  private:
   int num_buffers_;
   int nr_blocks_to_accumulate_;
+  BeamWeights *h_weights_;
+  BeamSteering<T> beam_steering_;
+  spatial::detail::Context ctx_;
+  int64_t visibilities_start_seq_num_ = -1;
+  int visibilities_missing_packets_ = 0;
+  int num_correlation_units_integrated_ = 0;
+  std::vector<float> scratch_;
+};
+
+// LambdaGPUPipeline (include/spatial/pipeline/lambda_gpu_pipeline.hpp): beams,
+// accumulated visibilities and the per-block eigendecomposition of all four
+// pol-pair covariances, on the delay-aligned window, with antenna gains,
+// sub-packet delays, stream permutation and fine delays applied.  The
+// bandpass FFT product (register_fft_block) is not produced.
+template <typename T>
+class LambdaGPUPipeline : public GPUPipeline {
+ public:
+  static constexpr size_t NR_BENCHMARKING_RUNS = 100;
+  using BeamWeights = BeamWeightsT<T>;
+
+  LambdaGPUPipeline(const int num_buffers, BeamWeightsT<T> *h_weights, BeamSteering<T> beam_steering = {})
+      : num_buffers_(std::min(num_buffers, 4)),
+        h_weights_(h_weights),
+        beam_steering_(beam_steering),
+        ctx_(T::shape(), LAM_MODE_LAMBDA_GPU, std::min(num_buffers, 4)) {
+    ctx_(lam_set_weights(ctx_.get(), reinterpret_cast<const uint16_t *>(h_weights_)));
+    typename T::PacketFinalDataType warmup_packet;
+    execute_pipeline(&warmup_packet, true);
+    ctx_(lam_sync(ctx_.get()));
+    visibilities_start_seq_num_ = -1;
+    visibilities_missing_packets_ = 0;
+    num_correlation_units_integrated_ = 0;
+  }
+
+  void set_subpacket_delays(int *delays_subpacket) override {
+    subpacket_delays_ = delays_subpacket;
+    ctx_(lam_set_subpacket_delays(ctx_.get(), delays_subpacket));
+  }
+  void set_antenna_gains(std::complex<float> *gains) override {
+    gains_ = gains;
+    ctx_(lam_set_gains(ctx_.get(), reinterpret_cast<const float *>(gains)));
+  }
+  void set_fine_delays(const float *delays_ns, double base_freq_hz, double channel_bw_hz,
+                       int min_freq_ch) override {
+    ctx_(lam_set_fine_delays(ctx_.get(), delays_ns, base_freq_hz, channel_bw_hz, min_freq_ch));
+  }
+  void set_stream_permutation(const std::vector<int> &recv_perm, const std::vector<int> &pol_perm) override {
+    constexpr size_t NR_PERM = T::NR_FPGA_SOURCES * T::NR_RECEIVERS_PER_PACKET * T::NR_POLARIZATIONS;
+    if (recv_perm.size() != NR_PERM || pol_perm.size() != NR_PERM)
+      throw std::runtime_error(
+          "set_stream_permutation: size must equal NR_FPGA_SOURCES * "
+          "NR_RECEIVERS_PER_PACKET * NR_POLARIZATIONS");
+    ctx_(lam_set_stream_permutation(ctx_.get(), recv_perm.data(), pol_perm.data(), NR_PERM));
+  }
+
+  void execute_pipeline(FinalPacketData *packet_data, const bool dummy_run = false) override {
+    beam_steering_.maybe_refresh();
+    const size_t start_seq_num = packet_data->start_seq_id;
+    const size_t end_seq_num = packet_data->end_seq_id;
+    if (visibilities_start_seq_num_ == -1)
+      visibilities_start_seq_num_ = static_cast<int64_t>(packet_data->start_seq_id);
+    visibilities_missing_packets_ += packet_data->get_num_missing_packets();
+    if (!dummy_run && state_ == nullptr)
+      throw std::logic_error("State has not been set on GPUPipeline object!");
+    lam_block_io io{};
+    io.samples = packet_data->get_samples_ptr();
+    io.scales = packet_data->get_scales_ptr();
+    io.release_fn = spatial::detail::release_buffer_host_func;
+    io.release_user = new spatial::detail::ReleaseContext{state_, packet_data->buffer_index, dummy_run};
+    if (output_ != nullptr && !dummy_run) {  // lambda_gpu_pipeline.hpp:534-580
+      const size_t block_num = output_->register_beam_data_block(start_seq_num, end_seq_num);
+      const size_t eig_block = output_->register_eigendecomposition_data_block(start_seq_num, end_seq_num);
+      if (block_num != std::numeric_limits<size_t>::max()) {
+        io.beams_dst = output_->get_beam_data_landing_pointer(block_num);
+        io.beams_done_fn = spatial::detail::beam_done_host_func;
+        io.beams_done_user = new spatial::detail::OutputDoneContext{output_, block_num};
+        std::memcpy(output_->get_arrivals_data_landing_pointer(block_num),
+                    packet_data->get_arrivals_ptr(), packet_data->get_arrivals_size());
+        output_->register_arrivals_transfer_complete(block_num);
+      }
+      if (eig_block != std::numeric_limits<size_t>::max()) {
+        io.eigenvalues_dst = static_cast<float *>(output_->get_eigenvalues_data_landing_pointer(eig_block));
+        io.eigenvectors_dst = static_cast<float *>(output_->get_eigenvectors_data_landing_pointer(eig_block));
+        io.eigen_done_fn = spatial::detail::eig_output_done_host_func;
+        io.eigen_done_user = new spatial::detail::OutputDoneContext{output_, eig_block};
+      }
+    }
+    ctx_(lam_execute(ctx_.get(), &io));
+    if (output_ != nullptr && !dummy_run) {
+      num_correlation_units_integrated_ += 1;
+      if (num_correlation_units_integrated_ >= static_cast<int>(T::NR_CORRELATED_BLOCKS_TO_ACCUMULATE))
+        dump_visibilities();
+    }
+    if (!dummy_run) benchmark_runs_done = (benchmark_runs_done + 1) % NR_BENCHMARKING_RUNS;
+  }
+
+  void dump_visibilities(const uint64_t end_seq_num = 0) override {
+    const int total = num_correlation_units_integrated_ * static_cast<int>(
+        T::NR_CHANNELS * T::NR_PACKETS_FOR_CORRELATION * T::NR_FPGA_SOURCES);
+    size_t block_num = std::numeric_limits<size_t>::max();
+    if (output_ != nullptr)
+      block_num = output_->register_visibilities_block(static_cast<size_t>(visibilities_start_seq_num_),
+                                                       end_seq_num, visibilities_missing_packets_, total);
+    visibilities_start_seq_num_ = -1;
+    visibilities_missing_packets_ = 0;
+    if (block_num != std::numeric_limits<size_t>::max()) {
+      ctx_(lam_dump_visibilities(
+          ctx_.get(), static_cast<float *>(output_->get_visibilities_landing_pointer(block_num)),
+          spatial::detail::vis_done_host_func, new spatial::detail::OutputDoneContext{output_, block_num}));
+    } else {
+      scratch_.resize(T::NR_CHANNELS * T::NR_BASELINES_UNPADDED * T::NR_POLARIZATIONS * T::NR_POLARIZATIONS * 2);
+      ctx_(lam_dump_visibilities(ctx_.get(), scratch_.data(), nullptr, nullptr));
+    }
+    num_correlation_units_integrated_ = 0;
+  }
+
+  void synchronize() { ctx_(lam_sync(ctx_.get())); }
+  lam_ctx *native_handle() const { return ctx_.get(); }
+  size_t benchmark_runs_done = 0;
+
+ private:
+  int num_buffers_;
   BeamWeights *h_weights_;
   BeamSteering<T> beam_steering_;
   spatial::detail::Context ctx_;

@@ -24,10 +24,10 @@ LIB_PATH = os.environ.get("LAM_LIBRARY", os.path.join(_HERE, "liblam.so"))
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "lam.h")
 
 LAM_OK, LAM_ERR_ARG, LAM_ERR_HIP, LAM_ERR_RANGE, LAM_ERR_STATE, LAM_ERR_UNSUPPORTED, LAM_ERR_NO_DEVICE = range(7)
-MODE_CORR_BEAM, MODE_ADAPTIVE = 0, 1
+MODE_CORR_BEAM, MODE_ADAPTIVE, MODE_LAMBDA_GPU = 0, 1, 2
 K_CORRBEAM, K_CORR, K_EIG, K_BEAM2B = range(4)
 KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp", "eig_fallback",
-                "subarray_eig", "spectra"]
+                "subarray_eig", "spectra", "pol_eig"]
 EIG_PLAIN, EIG_LEARNED, EIG_PREVIOUS = 0, 1, 2
 WPREC_F16_HILO, WPREC_F16, WPREC_BF16 = 0, 1, 2
 
@@ -545,6 +545,9 @@ class Pipeline:
                          counts_out=np.zeros((s.C, 2), np.int32))
             if self.spectra:
                 extra["spectra_out"] = self.spectra_array()
+        elif self.mode == MODE_LAMBDA_GPU:
+            extra = dict(evals_out=np.zeros((s.C, 2, 2, s.R), np.float32),
+                         evecs_out=np.zeros((s.C, 2, 2, s.R, s.R, 2), np.float32))
         self.execute(np.ascontiguousarray(ring, np.int8), np.ascontiguousarray(scales, np.int16),
                      beams_out=beams, **extra)
         self.sync()

@@ -24,6 +24,10 @@
  *   eigenvalues  float32  [C][P][R]     ascending       adaptive 41-45, 914-950
  *   eigenvectors float32  [C][P][R][R][2] column-major (eigenvector k at [k][0..R))
  *   counts       int32    [C][P]                        adaptive 709-715
+ *   pol-pair eigen (LAM_MODE_LAMBDA_GPU)                lambda_gpu_pipeline.hpp:44-49
+ *                eigenvalues  float32 [C][P][P][R] ascending
+ *                eigenvectors float32 [C][P][P][R][R][2], eigenvector k at [k][0..R),
+ *                largest-magnitude component real and positive
  *   spectra      float32  [C][P][2B][NT-10]             adaptive 808-848, FFTOutputType
  *                sum over the block's packets of |64-point FFT| of each beam
  *                packet, 5 edge bins dropped each side, DC centred
@@ -59,7 +63,13 @@ enum {
 /* Pipeline variants. */
 enum {
   LAM_MODE_CORR_BEAM = 0, /* LambdaCorrBeamOnlyGPUPipeline */
-  LAM_MODE_ADAPTIVE = 1   /* LambdaAdaptiveBeamformedSpectraPipeline (corr + eig + null + 2B beams) */
+  LAM_MODE_ADAPTIVE = 1,  /* LambdaAdaptiveBeamformedSpectraPipeline (corr + eig + null + 2B beams) */
+  /* LambdaGPUPipeline's correlate / beamform / eigen part (lambda_gpu_pipeline.hpp:445-608):
+   * B beams and accumulated visibilities as in corr-beam mode, on the delay-aligned
+   * window of the adaptive pipeline (slot 1 + per-FPGA delay, stream permutation),
+   * plus the eigendecomposition of all four pol-pair covariances of every block
+   * (layout "pol-pair eigen" below).  Its bandpass FFT is not computed. */
+  LAM_MODE_LAMBDA_GPU = 2
 };
 
 /* Runtime form of LambdaConfig (packet_formats.hpp:265-407). */
@@ -139,7 +149,7 @@ typedef struct lam_block_io {
   void *beams_dst;         /* host or device landing pointer for beams (NULL: skip) */
   lam_host_fn beams_done_fn;
   void *beams_done_user;
-  float *eigenvalues_dst;  /* adaptive only (NULL: skip) */
+  float *eigenvalues_dst;  /* adaptive or LAMBDA_GPU mode (NULL: skip) */
   float *eigenvectors_dst;
   int32_t *counts_dst;
   lam_host_fn eigen_done_fn;
@@ -184,7 +194,7 @@ int lam_set_graphs(lam_ctx *ctx, int enable);
 enum {
   LAM_K_CORRBEAM = 0, LAM_K_CORR = 1, LAM_K_EIG = 2, LAM_K_BEAM2B = 3,
   LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_EIGFB = 6, LAM_K_SUBARRAY = 7, LAM_K_SPECTRA = 8,
-  LAM_K_COUNT = 9
+  LAM_K_POLEIG = 9, LAM_K_COUNT = 10
 };
 int lam_set_profiling(lam_ctx *ctx, int enable);
 int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);

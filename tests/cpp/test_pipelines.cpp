@@ -277,7 +277,8 @@ TEST(AdaptiveNullRankOne) {
       CHECK(std::fabs(spatial::half_to_float(beam[0][p][1][t][1])) <= 0.25f, "mitigated im");
     }
   CHECK(r.mitigated_power(0) + r.mitigated_power(1) < 1.0, "power");
-  auto &ev = *r.output->eigenvalues;
+  // the adaptive pipeline's [C][P][R] eigenvalues at the start of the landing area
+  auto &ev = *reinterpret_cast<float(*)[Adaptive::NR_CHANNELS][2][Adaptive::NR_RECEIVERS]>(r.output->eigenvalues);
   for (int p = 0; p < 2; ++p)
     for (int k = 0; k < 4; ++k) {
       CHECK(ev[0][p][k] >= -1e-3f, "non-negative");
@@ -362,6 +363,49 @@ TEST(AdaptivePermutationSizeError) {
   CHECK(spatial::half_to_float((*r.output->beam_data)[0][0][0][0][0]) == 8.0f, "identity perm");
 }
 
+// LambdaGPUPipeline, tests/test_pipeline.cu:956-1052 (EigenvalueBasic): every
+// sample (2, -2), 4 receivers, 8 samples: each pol-pair covariance is 64 in
+// every entry, so its eigenvalues are 0, 0, 0, 256 and the principal
+// eigenvector is 0.5 in every component.  The other three eigenvectors span a
+// degenerate eigenspace (any basis of it is a valid answer), so only their
+// orthogonality to the principal one is checked.  Beams and visibilities as
+// in test_pipeline.cu's Ex1.
+TEST(LambdaGPUEigenvalueBasic) {
+  auto output = std::make_shared<SingleHostMemoryOutput<Small>>();
+  BeamWeightsT<Small> weights = unity_weights<Small>();
+  LambdaGPUPipeline<Small> pipeline(1, &weights);
+  FakeProcessorState state;
+  pipeline.set_state(&state);
+  pipeline.set_output(output);
+  int delays[1] = {0};
+  pipeline.set_subpacket_delays(delays);
+  typename Small::PacketFinalDataType pd;
+  fill_block<Small>(pd, const_sample, unit_scale);
+  pipeline.execute_pipeline(&pd);
+  pipeline.dump_visibilities();
+  pipeline.synchronize();
+  auto &ev = *output->eigenvalues;
+  auto &vec = *output->eigenvectors;
+  for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < 2; ++q) {
+      for (int n = 0; n < 4; ++n)
+        CHECK(std::fabs(ev[0][p][q][n] - (n == 3 ? 256.0f : 0.0f)) <= 1e-4f, "eig p=%d q=%d n=%d %f", p, q,
+              n, ev[0][p][q][n]);
+      for (int l = 0; l < 4; ++l) {
+        CHECK(std::fabs(vec[0][p][q][3][l].real() - 0.5f) <= 1e-4f, "principal %f", vec[0][p][q][3][l].real());
+        CHECK(std::fabs(vec[0][p][q][3][l].imag()) <= 1e-4f, "principal imag");
+      }
+      for (int n = 0; n < 3; ++n) {
+        std::complex<float> dot = 0.0f;
+        for (int l = 0; l < 4; ++l) dot += std::conj(vec[0][p][q][3][l]) * vec[0][p][q][n][l];
+        CHECK(std::abs(dot) <= 1e-4f, "orthogonal");
+      }
+    }
+  CHECK(output->eig_blocks_done == 1, "eigen callback");
+  for (size_t t = 0; t < 8; ++t) CHECK(spatial::half_to_float((*output->beam_data)[0][0][0][t][0]) == 8.0f, "beam");
+  CHECK((*output->visibilities)[0][0][0][0][0] == 64.0f, "vis");
+}
+
 // Overlapped ingest (ProcessorState -> pinned buffers -> async H2D,
 // spatial.hpp:718-770, 1182-1299): 12 blocks through 3 pinned buffers, the
 // producer refilling each buffer the moment the pipeline releases it.  A
@@ -414,6 +458,7 @@ int main() {
   RUN(AdaptiveK0TaggedLayout);
   RUN(AdaptiveShrinkAndDetection);
   RUN(AdaptivePermutationSizeError);
+  RUN(LambdaGPUEigenvalueBasic);
   RUN(BlockRingOverlappedIngest);
   std::printf("%d checks, %d failures\n", g_checks, g_fail);
   return g_fail == 0 ? 0 : 1;
