@@ -153,6 +153,39 @@ def run_host_fed(p, shape, ring, scales, args):
             "steps": n, "buffers": args.buffers}
 
 
+def run_lambda_gpu(shape, d_ring, d_scales, w16, args):
+    """LambdaGPUPipeline's correlate / beamform / eigen part on the same block:
+    B beams, accumulated visibilities, and the eigendecomposition of all four
+    pol-pair covariances of every block (4 x 384 = 1536 problems of 36 x 36)."""
+    n = max(10, args.steps // 3)
+    p = lam.Pipeline(shape, lam.MODE_LAMBDA_GPU, num_buffers=args.buffers)
+    p.set_weights(w16)
+    p.set_eig_params(args.sweeps, 1e-6)
+    p.set_tuning(args.spw)
+    for _ in range(5):
+        p.execute(d_ring, d_scales, on_device=True)
+    p.sync()
+    p.dump_visibilities_i64()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        p.execute(d_ring, d_scales, on_device=True)
+    p.sync()
+    t1 = time.perf_counter()
+    p.dump_visibilities_i64()
+    p.set_profiling(True)
+    p.kernel_times()
+    for _ in range(n):
+        p.execute(d_ring, d_scales, on_device=True)
+        p.sync()
+    kt = p.kernel_times()
+    p.close()
+    sec = (t1 - t0) / n
+    return {"ms_per_step": sec * 1e3, "real_time_factor": (shape.T / CHANNEL_RATE_HZ) / sec,
+            "ant_samples_per_s": shape.R * shape.C * shape.T / sec,
+            "kernels_us": {k: round(v[0] / max(1, v[1]) * 1e3, 1) for k, v in kt.items()},
+            "eigenproblems_per_block": shape.C * 4}
+
+
 def run_learned(args):
     """Learned warm start on 384 ch x 2 pol x 36 ant with drifting, switching
     interferers: blocks [0, train) run the plain solver and are captured as the
@@ -473,6 +506,7 @@ def main():
     ap.add_argument("--no-subarrays", action="store_true")
     ap.add_argument("--no-spectra", action="store_true")
     ap.add_argument("--no-host-fed", action="store_true")
+    ap.add_argument("--no-lambda-gpu", action="store_true")
     ap.add_argument("--spw", type=int, default=0, help="samples per voltage workgroup (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -591,6 +625,11 @@ def main():
             roofline["traffic"] = ent["bytes_per_launch"].get(dominant)
             roofline["traffic_splits"] = nsplit
 
+    lambda_gpu = None
+    if not args.no_lambda_gpu:
+        lambda_gpu = run_lambda_gpu(shape, d_ring, d_scales, w16, args)
+        log(f"lambda-gpu: {lambda_gpu}")
+
     learned = None
     if not args.no_learned:
         learned = run_learned(args)
@@ -639,6 +678,7 @@ def main():
         "precision_sweep": precision,
         "host_fed_pcie_inclusive": host_fed,
         "spectra_on": spectra,
+        "lambda_gpu_mode": lambda_gpu,
         "uneven_subarrays": subarrays,
         "cpu_baseline": cpu,
     }
