@@ -154,13 +154,15 @@ def run_host_fed(p, shape, ring, scales, args):
 
 
 def run_lambda_gpu(shape, d_ring, d_scales, w16, args):
-    """LambdaGPUPipeline's correlate / beamform / eigen part on the same block:
-    B beams, accumulated visibilities, and the eigendecomposition of all four
-    pol-pair covariances of every block (4 x 384 = 1536 problems of 36 x 36)."""
+    """LambdaGPUPipeline on the same block: B beams, accumulated visibilities,
+    the eigendecomposition of all four pol-pair covariances of every block
+    (4 x 384 = 1536 problems of 36 x 36) and the bandpass (8192-point FFT of
+    every beam row, |X|^2 averaged over runs of 128 bins)."""
     n = max(10, args.steps // 3)
     p = lam.Pipeline(shape, lam.MODE_LAMBDA_GPU, num_buffers=args.buffers)
     p.set_weights(w16)
     p.set_eig_params(args.sweeps, 1e-6)
+    p.set_bandpass(128)  # FFT_DOWNSAMPLE_FACTOR default of LambdaConfig
     p.set_tuning(args.spw)
     for _ in range(5):
         p.execute(d_ring, d_scales, on_device=True)

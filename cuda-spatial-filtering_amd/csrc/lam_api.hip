@@ -36,7 +36,7 @@ struct Buf {
   int8_t *d_samples = nullptr;  // staging for host inputs
   int16_t *d_scales = nullptr;
   __half *d_beams = nullptr;
-  float *d_spectra = nullptr;   // adaptive: post-beam spectra [C][2][2B][NT-10]
+  float *d_spectra = nullptr;   // adaptive spectra [C][2][2B][NT-10] or LambdaGPU bandpass [C][2][B][T/D]
   int *d_vis_part = nullptr;    // this block's partial visibilities [split][C][pq][NBL][2]
   v8h *d_wop2 = nullptr;        // adaptive: this block's 2B-beam weight operands
   float *d_evals = nullptr, *d_evecs = nullptr;
@@ -106,6 +106,7 @@ struct lam_ctx {
   int blocks_acc = 0;
   bool use_graphs = true, graphs_dirty = true, profiling = false;
   bool spectra = false;  // compute the post-beam spectra in the beam phase
+  int bandpass = 0;      // LambdaGPU bandpass: bins averaged per output value (0: off)
   int samples_per_wg = 0;
   std::vector<ProfEvent> prof;
   // eigen-stage method
@@ -263,6 +264,10 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
       Prof pr(ctx, s, LAM_K_POLEIG);
       HIPC(ctx, launch_pol_eig(b.d_vis_part, p.nsplit, ctx->C, ctx->R, ctx->d_gains, ctx->d_phase,
                                ctx->max_sweeps, ctx->tol, b.d_evals, b.d_evecs, nullptr, s));
+    }
+    if (ctx->mode == LAM_MODE_LAMBDA_GPU && ctx->bandpass) {  // lambda_gpu_pipeline.hpp:522-531
+      Prof pr(ctx, s, LAM_K_BANDPASS);
+      HIPC(ctx, launch_bandpass(b.d_beams, ctx->C * 2 * ctx->B, ctx->T, ctx->bandpass, b.d_spectra, s));
     }
     HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, (int)ctx->NBL, ctx->d_vis_acc, atomic_acc, s));
     return LAM_OK;
@@ -959,6 +964,27 @@ int lam_set_spectra(lam_ctx *ctx, int enable) {
   return LAM_OK;
 }
 
+int lam_set_bandpass(lam_ctx *ctx, int downsample_factor) {
+  if (!ctx || downsample_factor < 0) return LAM_ERR_ARG;
+  if (downsample_factor > 0 &&
+      (ctx->mode != LAM_MODE_LAMBDA_GPU || ctx->T > 8192 || (ctx->T & (ctx->T - 1)) || ctx->T % downsample_factor))
+    return fail(ctx, LAM_ERR_ARG,
+                "bandpass needs the LambdaGPU mode, a power-of-two block length <= 8192 and a factor dividing it");
+  if (int r = sync_all(ctx)) return r;
+  for (auto &b : ctx->bufs) {
+    hipFree(b.d_spectra);
+    b.d_spectra = nullptr;
+  }
+  ctx->bandpass = 0;
+  ctx->graphs_dirty = true;
+  if (downsample_factor == 0) return LAM_OK;
+  for (auto &b : ctx->bufs)
+    if (int r = dalloc(ctx, &b.d_spectra, (size_t)ctx->C * 2 * ctx->B * (ctx->T / downsample_factor) * sizeof(float)))
+      return r;
+  ctx->bandpass = downsample_factor;
+  return LAM_OK;
+}
+
 int lam_set_eig_params(lam_ctx *ctx, int max_sweeps, float tol) {
   if (!ctx || max_sweeps < 0 || !(tol >= 0.0f)) return LAM_ERR_ARG;
   ctx->max_sweeps = max_sweeps;
@@ -1031,7 +1057,8 @@ int lam_blocks_accumulated(const lam_ctx *ctx) { return ctx ? ctx->blocks_acc : 
 
 int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
   if (!ctx || !io || !io->samples || !io->scales) return LAM_ERR_ARG;
-  if (io->spectra_dst && !ctx->spectra) return fail(ctx, LAM_ERR_ARG, "spectra_dst given but spectra are off");
+  if (io->spectra_dst && !ctx->spectra && !ctx->bandpass)
+    return fail(ctx, LAM_ERR_ARG, "spectra_dst given but spectra are off");
   Buf &b = ctx->bufs[ctx->cur];
   hipStream_t s = b.stream;
   const void *smp = io->samples, *scl = io->scales;
@@ -1051,6 +1078,12 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
   if (io->beams_dst) {
     HIPC(ctx, hipMemcpyAsync(io->beams_dst, b.d_beams, ctx->beams_bytes, hipMemcpyDefault, s));
     if (io->beams_done_fn) HIPC(ctx, hipLaunchHostFunc(s, io->beams_done_fn, io->beams_done_user));
+  }
+  if (ctx->mode == LAM_MODE_LAMBDA_GPU && io->spectra_dst) {
+    HIPC(ctx, hipMemcpyAsync(io->spectra_dst, b.d_spectra,
+                             (size_t)ctx->C * 2 * ctx->B * (ctx->T / ctx->bandpass) * sizeof(float),
+                             hipMemcpyDefault, s));
+    if (io->spectra_done_fn) HIPC(ctx, hipLaunchHostFunc(s, io->spectra_done_fn, io->spectra_done_user));
   }
   if (ctx->mode == LAM_MODE_LAMBDA_GPU && (io->eigenvalues_dst || io->eigenvectors_dst)) {
     if (io->eigenvalues_dst)

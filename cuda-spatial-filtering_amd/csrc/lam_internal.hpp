@@ -150,6 +150,9 @@ hipError_t launch_features(const FeatParams &p, hipStream_t s);
 // Post-beam spectra: 64-point DFT magnitudes per packet, 54 kept bins, summed
 // over packets.  table: spectra_table_bytes() of device memory filled from
 // spectra_table_host().
+// LambdaGPU bandpass: N-point FFT per beam row (N = block length, power of
+// two <= 8192), mean |X|^2 over runs of D bins -> out [rows][N / D].
+hipError_t launch_bandpass(const __half *beams, int rows, int N, int D, float *out, hipStream_t s);
 // Eigendecomposition of all four pol-pair covariances of one block
 // (LambdaGPUPipeline): evals [C][2][2][n], evecs [C][2][2][n][n][2].
 hipError_t launch_pol_eig(const int *vis_part, int nsplit, int C, int n, const float2 *gains,

@@ -190,4 +190,182 @@ hipError_t launch_spectra(const __half *beams, const void *table, int C, int nbe
   return hipGetLastError();
 }
 
+
+
+// ============================================================ bandpass
+// LambdaGPUPipeline's bandpass product (lambda_gpu_pipeline.hpp:522-531,
+// 814-816; detect_and_downsample_fft, spatial.cuh:795-835): per (channel,
+// pol, beam) the forward FFT over the whole block (N = NP * NT samples), then
+// the mean of |X|^2 over each run of D consecutive bins (natural order; NaN
+// values are skipped) -> float [C][P][B][N / D].
+//
+// One workgroup per row, the row in LDS as FP32 complex (N <= 8192: 64 KB),
+// a mixed-radix Stockham FFT in place: each pass reads every butterfly's R
+// inputs into registers, then (after a barrier) writes the R-point DFTs to
+// their autosorted places, so the result is in natural order with no
+// bit-reversal pass.  Radix 16 while it fits, then 8, 4 or 2.  Each
+// butterfly's twiddles are powers of W_N^{k N/(Ns R)}, computed with sincospif
+// (a table read from global memory put its latency into every pass).
+namespace {
+
+// cos / sin (2 pi k / 16), k = 0..7: the twiddles inside a 16-point DFT.
+__device__ constexpr float kCos16[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
+                                        0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
+__device__ constexpr float kSin16[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
+                                        1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
+
+// t = W16^I * o for I = 0..7 (W16 = exp(-2 pi i / 16)); 1 and -i need no multiply.
+template <int I>
+__device__ __forceinline__ float2 w16_mul(float2 o) {
+  if constexpr (I == 0) {
+    return o;
+  } else if constexpr (I == 4) {
+    return make_float2(o.y, -o.x);
+  } else {
+    const float c = kCos16[I], s = kSin16[I];
+    return make_float2(c * o.x + s * o.y, c * o.y - s * o.x);
+  }
+}
+
+template <int R, int K>
+__device__ __forceinline__ void dft_combine(float2 *v, const float2 *e, const float2 *o) {
+  if constexpr (K < R / 2) {
+    const float2 t = w16_mul<K * (16 / R)>(o[K]);
+    v[K] = make_float2(e[K].x + t.x, e[K].y + t.y);
+    v[K + R / 2] = make_float2(e[K].x - t.x, e[K].y - t.y);
+    dft_combine<R, K + 1>(v, e, o);
+  }
+}
+
+// In-register forward DFT of R = 2^k <= 16 points, natural order in and out
+// (radix-2 decimation in time, fully unrolled).
+template <int R>
+__device__ __forceinline__ void dft_reg(float2 *v) {
+  if constexpr (R > 1) {
+    float2 e[R / 2], o[R / 2];
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) {
+      e[i] = v[2 * i];
+      o[i] = v[2 * i + 1];
+    }
+    dft_reg<R / 2>(e);
+    dft_reg<R / 2>(o);
+    dft_combine<R, 0>(v, e, o);
+  }
+}
+
+constexpr int kBpThreads = 512, kBpMaxN = 8192;
+
+// Sum over the 64 lanes of a wavefront (xor butterflies), all lanes get it.
+__device__ __forceinline__ float wave_sum_bp(float v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// LDS slot of element i: one pad slot per 16, so the strided writes of the
+// early passes spread over the banks.
+__device__ __forceinline__ int bp_slot(int i) { return i + (i >> 4); }
+
+// One Stockham pass of radix R over N points with completed sub-transform size
+// Ns: butterfly j reads x[j + r N/R], twiddles by W_{Ns R}^{(j mod Ns) r}, and
+// writes x[(j / Ns) Ns R + (j mod Ns) + r Ns].  The twiddles of a butterfly are
+// powers of one table entry (W_N^{k N/(Ns R)}), formed by repeated products.
+template <int R>
+__device__ __forceinline__ void stockham_pass(float2 *buf, int N, int Ns) {
+  constexpr int kG = (kBpMaxN / R + kBpThreads - 1) / kBpThreads;  // butterflies per thread at most
+  const int nj = N / R, stride = N / R, tstep = N / (Ns * R);
+  float2 v[kG][R];
+#pragma unroll
+  for (int g = 0; g < kG; ++g) {
+    const int j = threadIdx.x + g * kBpThreads;
+    if (j < nj) {
+      const int k = j % Ns;
+      float2 w1;  // W_N^{k tstep}: angle -2 pi k tstep / N, k tstep < N exact in FP32
+      sincospif(-2.0f * (float)(k * tstep) / (float)N, &w1.y, &w1.x);
+      float2 w = make_float2(1.0f, 0.0f);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const float2 x = buf[bp_slot(j + r * stride)];
+        v[g][r] = r == 0 ? x : make_float2(x.x * w.x - x.y * w.y, x.x * w.y + x.y * w.x);
+        w = make_float2(w.x * w1.x - w.y * w1.y, w.x * w1.y + w.y * w1.x);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int g = 0; g < kG; ++g) {
+    const int j = threadIdx.x + g * kBpThreads;
+    if (j < nj) {
+      dft_reg<R>(v[g]);
+      const int k = j % Ns, base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[bp_slot(base + r * Ns)] = v[g][r];
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(kBpThreads)
+bandpass_kernel(const __half2 *__restrict__ beams, int N, int D,
+                float *__restrict__ out) {
+  extern __shared__ __align__(16) char bp_smem[];
+  float2 *buf = reinterpret_cast<float2 *>(bp_smem);
+  const int M = N / D;
+  float *sums = reinterpret_cast<float *>(buf + bp_slot(N));
+  int *cnt = reinterpret_cast<int *>(sums + M);
+  const long row = blockIdx.x;
+  const __half2 *x = beams + row * N;
+  for (int i = threadIdx.x; i < N; i += kBpThreads) buf[bp_slot(i)] = __half22float2(x[i]);
+  for (int m = threadIdx.x; m < M; m += kBpThreads) {
+    sums[m] = 0.0f;
+    cnt[m] = 0;
+  }
+  __syncthreads();
+  int Ns = 1;
+  while (N / Ns >= 16) {
+    stockham_pass<16>(buf, N, Ns);
+    Ns *= 16;
+  }
+  if (N / Ns == 8) stockham_pass<8>(buf, N, Ns);
+  else if (N / Ns == 4) stockham_pass<4>(buf, N, Ns);
+  else if (N / Ns == 2) stockham_pass<2>(buf, N, Ns);
+  // mean of |X|^2 over runs of D bins, NaN skipped
+  const int lane = threadIdx.x & 63;
+  for (int i0 = threadIdx.x - lane; i0 < N; i0 += kBpThreads) {
+    const int i = i0 + lane;  // a wave covers 64 consecutive bins
+    const float2 X = i < N ? buf[bp_slot(i)] : make_float2(0.0f, 0.0f);
+    const float p = X.x * X.x + X.y * X.y;
+    const bool ok = i < N && !isnan(p);
+    if (D % 64 == 0) {  // the wave's 64 bins share one run: one atomic per wave
+      const float ws = wave_sum_bp(ok ? p : 0.0f);
+      const int wc = __popcll(__ballot(ok));
+      if (lane == 0 && i0 < N) {
+        atomicAdd(&sums[i0 / D], ws);
+        atomicAdd(&cnt[i0 / D], wc);
+      }
+    } else if (ok) {
+      atomicAdd(&sums[i / D], p);
+      atomicAdd(&cnt[i / D], 1);
+    }
+  }
+  __syncthreads();
+  for (int m = threadIdx.x; m < M; m += kBpThreads) out[row * M + m] = cnt[m] > 0 ? sums[m] / (float)cnt[m] : 0.0f;
+}
+
+hipError_t launch_bandpass(const __half *beams, int rows, int N, int D, float *out, hipStream_t s) {
+  if (N < 2 || N > kBpMaxN || (N & (N - 1)) || D < 1 || N % D) return hipErrorInvalidValue;
+  const size_t sm = (size_t)(N + N / 16) * sizeof(float2) + (size_t)(N / D) * (sizeof(float) + sizeof(int));
+  if (sm > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(bandpass_kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(bandpass_kernel, dim3(rows), dim3(kBpThreads), sm, s, reinterpret_cast<const __half2 *>(beams),
+                     N, D, out);
+  return hipGetLastError();
+}
+
 }  // namespace lam

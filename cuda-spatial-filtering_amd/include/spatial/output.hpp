@@ -3,6 +3,7 @@
 // (output.hpp:60-164) with pinned buffers from lam_host_alloc.  The buffered
 // writer machinery (BufferedOutput, HDF5/PSRDADA writers) is not part of this path.
 #pragma once
+#include <algorithm>
 #include <atomic>
 #include <cstddef>
 #include <cstdint>
@@ -68,6 +69,7 @@ class SingleHostMemoryOutput : public Output {
   Eigenvectors *eigenvectors;
   EigenmodeCounts *nulled_eigenmode_counts;
   FFTOutput *fft_output;
+  typename T::FFTOutputType &bandpass() { return *reinterpret_cast<typename T::FFTOutputType *>(fft_output); }
 
   size_t register_beam_data_block(const size_t, const size_t) override { return 1; }
   size_t register_visibilities_block(const size_t, const size_t, const int missing,
@@ -108,7 +110,11 @@ class SingleHostMemoryOutput : public Output {
     eigenvalues = alloc<Eigenvalues>();
     eigenvectors = alloc<Eigenvectors>();
     nulled_eigenmode_counts = alloc<EigenmodeCounts>();
-    fft_output = alloc<FFTOutput>();
+    // one landing area for either product: the adaptive spectra (FFTOutput)
+    // or the LambdaGPU bandpass (T::FFTOutputType, view with bandpass())
+    fft_output = static_cast<FFTOutput *>(
+        lam_host_alloc(std::max(sizeof(FFTOutput), sizeof(typename T::FFTOutputType))));
+    if (!fft_output) throw std::runtime_error("lam_host_alloc failed");
   }
   ~SingleHostMemoryOutput() override {
     lam_host_free(beam_data);

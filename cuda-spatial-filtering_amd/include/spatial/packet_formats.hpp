@@ -106,6 +106,7 @@ struct LambdaConfig {
   static constexpr size_t COMPLEX = 2;
   static constexpr size_t NR_TIME_STEPS_FOR_CORRELATION =
       NR_PACKETS_FOR_CORRELATION * NR_TIME_STEPS_PER_PACKET;
+  static constexpr int FFT_DOWNSAMPLE_FACTOR = FFT_DOWNSAMPLE_FACTOR_T;
   static_assert(NR_RECEIVERS == NR_FPGA_SOURCES * NR_RECEIVERS_PER_PACKET,
                 "NR_RECEIVERS must equal NR_FPGA_SOURCES * NR_RECEIVERS_PER_PACKET");
 
@@ -122,6 +123,10 @@ struct LambdaConfig {
   using BeamOutputType = spatial::half_t[NR_CHANNELS][NR_POLARIZATIONS][NR_BEAMS]
                                         [NR_TIME_STEPS_FOR_CORRELATION][COMPLEX];
   using ArrivalsOutputType = bool[NR_CHANNELS][NR_PACKETS_FOR_CORRELATION + 2][NR_FPGA_SOURCES];
+  // LambdaGPU bandpass: mean |FFT|^2 per run of FFT_DOWNSAMPLE_FACTOR bins
+  // (packet_formats.hpp:376-379)
+  using FFTOutputType = float[NR_CHANNELS][NR_POLARIZATIONS][NR_BEAMS]
+                             [NR_TIME_STEPS_FOR_CORRELATION / FFT_DOWNSAMPLE_FACTOR_T];
   using VisibilitiesOutputType =
       float[NR_CHANNELS][NR_BASELINES_UNPADDED][NR_POLARIZATIONS][NR_POLARIZATIONS][COMPLEX];
   using EigenvalueOutputType = float[NR_CHANNELS][NR_POLARIZATIONS][NR_POLARIZATIONS][NR_RECEIVERS];

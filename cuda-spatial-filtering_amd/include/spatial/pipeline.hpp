@@ -251,8 +251,8 @@ class LambdaCorrBeamOnlyGPUPipeline : public GPUPipeline {
 // LambdaGPUPipeline (include/spatial/pipeline/lambda_gpu_pipeline.hpp): beams,
 // accumulated visibilities and the per-block eigendecomposition of all four
 // pol-pair covariances, on the delay-aligned window, with antenna gains,
-// sub-packet delays, stream permutation and fine delays applied.  The
-// bandpass FFT product (register_fft_block) is not produced.
+// sub-packet delays, stream permutation and fine delays applied, and the
+// bandpass FFT product (register_fft_block).
 template <typename T>
 class LambdaGPUPipeline : public GPUPipeline {
  public:
@@ -265,6 +265,12 @@ class LambdaGPUPipeline : public GPUPipeline {
         beam_steering_(beam_steering),
         ctx_(T::shape(), LAM_MODE_LAMBDA_GPU, std::min(num_buffers, 4)) {
     ctx_(lam_set_weights(ctx_.get(), reinterpret_cast<const uint16_t *>(h_weights_)));
+    // bandpass FFT over the block (lambda_gpu_pipeline.hpp:814-816, 522-531)
+    constexpr size_t N = T::NR_TIME_STEPS_FOR_CORRELATION;
+    static_assert(N <= 8192 && (N & (N - 1)) == 0 && N % T::FFT_DOWNSAMPLE_FACTOR == 0,
+                  "the LambdaGPU bandpass needs a power-of-two block length <= 8192 that "
+                  "FFT_DOWNSAMPLE_FACTOR divides");
+    ctx_(lam_set_bandpass(ctx_.get(), T::FFT_DOWNSAMPLE_FACTOR));
     typename T::PacketFinalDataType warmup_packet;
     execute_pipeline(&warmup_packet, true);
     ctx_(lam_sync(ctx_.get()));
@@ -318,6 +324,12 @@ class LambdaGPUPipeline : public GPUPipeline {
         std::memcpy(output_->get_arrivals_data_landing_pointer(block_num),
                     packet_data->get_arrivals_ptr(), packet_data->get_arrivals_size());
         output_->register_arrivals_transfer_complete(block_num);
+      }
+      const size_t fft_block = output_->register_fft_block(start_seq_num, end_seq_num);
+      if (fft_block != std::numeric_limits<size_t>::max()) {
+        io.spectra_dst = static_cast<float *>(output_->get_fft_landing_pointer(fft_block));
+        io.spectra_done_fn = spatial::detail::fft_done_host_func;
+        io.spectra_done_user = new spatial::detail::OutputDoneContext{output_, fft_block};
       }
       if (eig_block != std::numeric_limits<size_t>::max()) {
         io.eigenvalues_dst = static_cast<float *>(output_->get_eigenvalues_data_landing_pointer(eig_block));

@@ -27,7 +27,7 @@ LAM_OK, LAM_ERR_ARG, LAM_ERR_HIP, LAM_ERR_RANGE, LAM_ERR_STATE, LAM_ERR_UNSUPPOR
 MODE_CORR_BEAM, MODE_ADAPTIVE, MODE_LAMBDA_GPU = 0, 1, 2
 K_CORRBEAM, K_CORR, K_EIG, K_BEAM2B = range(4)
 KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp", "eig_fallback",
-                "subarray_eig", "spectra", "pol_eig"]
+                "subarray_eig", "spectra", "pol_eig", "bandpass"]
 EIG_PLAIN, EIG_LEARNED, EIG_PREVIOUS = 0, 1, 2
 WPREC_F16_HILO, WPREC_F16, WPREC_BF16 = 0, 1, 2
 
@@ -89,6 +89,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_set_fixed_k.argtypes = [vp, vp]
     L.lam_set_nulling.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_int]
     L.lam_set_spectra.argtypes = [vp, ctypes.c_int]
+    L.lam_set_bandpass.argtypes = [vp, ctypes.c_int]
     L.lam_set_eig_params.argtypes = [vp, ctypes.c_int, ctypes.c_float]
     L.lam_set_graphs.argtypes = [vp, ctypes.c_int]
     L.lam_set_profiling.argtypes = [vp, ctypes.c_int]
@@ -293,6 +294,7 @@ class Pipeline:
         self.h = h
         self._keep = []
         self.spectra = False
+        self.bandpass = 0
 
     # -- plumbing
     def _chk(self, rc):
@@ -350,8 +352,15 @@ class Pipeline:
         self._chk(self.L.lam_set_spectra(self.h, int(bool(enable))))
         self.spectra = bool(enable)
 
+    def set_bandpass(self, downsample_factor):
+        """LambdaGPU bandpass: mean |FFT|^2 over runs of downsample_factor bins (0: off)."""
+        self._chk(self.L.lam_set_bandpass(self.h, int(downsample_factor)))
+        self.bandpass = int(downsample_factor)
+
     def spectra_array(self):
         s = self.shape
+        if self.mode == MODE_LAMBDA_GPU:
+            return np.zeros((s.C, 2, s.B, s.T // self.bandpass), np.float32)
         return np.zeros((s.C, 2, 2 * s.B, s.NT - 10), np.float32)
 
     def set_eig_params(self, max_sweeps=12, tol=1e-6):
@@ -548,6 +557,8 @@ class Pipeline:
         elif self.mode == MODE_LAMBDA_GPU:
             extra = dict(evals_out=np.zeros((s.C, 2, 2, s.R), np.float32),
                          evecs_out=np.zeros((s.C, 2, 2, s.R, s.R, 2), np.float32))
+            if self.bandpass:
+                extra["spectra_out"] = self.spectra_array()
         self.execute(np.ascontiguousarray(ring, np.int8), np.ascontiguousarray(scales, np.int16),
                      beams_out=beams, **extra)
         self.sync()

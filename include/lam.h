@@ -133,6 +133,16 @@ int lam_set_nulling(lam_ctx *ctx, int detect, float delta, int shrink);
  * by default (it reads the whole beam output once more per block).
  * LAM_ERR_ARG in corr-beam mode or for another NT. */
 int lam_set_spectra(lam_ctx *ctx, int enable);
+
+/* Bandpass product of the LambdaGPU mode (lambda_gpu_pipeline.hpp:522-531,
+ * 814-816; detect_and_downsample_fft, spatial.cuh:795-835): per (channel,
+ * pol, beam) the forward FFT over the whole block (NP * NT points), then the
+ * mean of |X|^2 over each run of `downsample_factor` consecutive bins (natural
+ * FFT order, NaN skipped) -> float32 [C][P][B][NP*NT / downsample_factor],
+ * returned through lam_block_io.spectra_dst.  0 turns it off (the default).
+ * LAM_ERR_ARG outside the LambdaGPU mode, for a block length that is not a
+ * power of two up to 8192, or for a factor that does not divide it. */
+int lam_set_bandpass(lam_ctx *ctx, int downsample_factor);
 /* Eigensolver controls: maximum Jacobi sweeps and convergence tolerance on
  * the relative off-diagonal Frobenius norm. */
 int lam_set_eig_params(lam_ctx *ctx, int max_sweeps, float tol);
@@ -154,7 +164,8 @@ typedef struct lam_block_io {
   int32_t *counts_dst;
   lam_host_fn eigen_done_fn;
   void *eigen_done_user;
-  float *spectra_dst;      /* adaptive with lam_set_spectra(ctx, 1) only (NULL: skip) */
+  float *spectra_dst;      /* adaptive spectra (lam_set_spectra) or LambdaGPU bandpass
+                            * (lam_set_bandpass); NULL: skip */
   lam_host_fn spectra_done_fn;
   void *spectra_done_user;
 } lam_block_io;
@@ -194,7 +205,7 @@ int lam_set_graphs(lam_ctx *ctx, int enable);
 enum {
   LAM_K_CORRBEAM = 0, LAM_K_CORR = 1, LAM_K_EIG = 2, LAM_K_BEAM2B = 3,
   LAM_K_FEAT = 4, LAM_K_MLP = 5, LAM_K_EIGFB = 6, LAM_K_SUBARRAY = 7, LAM_K_SPECTRA = 8,
-  LAM_K_POLEIG = 9, LAM_K_COUNT = 10
+  LAM_K_POLEIG = 9, LAM_K_BANDPASS = 10, LAM_K_COUNT = 11
 };
 int lam_set_profiling(lam_ctx *ctx, int enable);
 int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);

@@ -388,6 +388,31 @@ void orc_beam_spectra(int C, int P, int E, int np, int nfft, int drop,
   }
 }
 
+void orc_bandpass(int rows, int n, int d, const double *x, double *out) {
+  for (long row = 0; row < rows; ++row) {
+    const double *xr = x + 2 * row * (long)n;
+    for (int m = 0; m < n / d; ++m) {
+      double sum = 0.0;
+      int cnt = 0;
+      for (int k = m * d; k < (m + 1) * d; ++k) {
+        double re = 0.0, im = 0.0;
+        for (int t = 0; t < n; ++t) {
+          const double a = -2.0 * M_PI * (double)((long)k * t % n) / n;
+          const double c = cos(a), s = sin(a);
+          re += xr[2 * t] * c - xr[2 * t + 1] * s;
+          im += xr[2 * t] * s + xr[2 * t + 1] * c;
+        }
+        const double p = re * re + im * im;
+        if (!isnan(p)) {
+          sum += p;
+          ++cnt;
+        }
+      }
+      out[row * (n / d) + m] = cnt ? sum / cnt : 0.0;
+    }
+  }
+}
+
 /* ------------------------------------------------------------ block driver */
 int orc_rfi_null_cp(int R, int P, int B, const int64_t *vis_c, int p,
                     const double *w_cp, int fixed_k, double delta, int shrink,

@@ -142,6 +142,14 @@ int orc_adaptive_block(const orc_shape *s, const int32_t *y, const double *w,
 void orc_beam_spectra(int C, int P, int E, int np, int nfft, int drop,
                       const double *beams, double *out);
 
+/* Bandpass product of LambdaGPUPipeline (lambda_gpu_pipeline.hpp:522-531,
+ * 814-816; detect_and_downsample_fft, spatial.cuh:795-835): per row the
+ * unnormalised forward DFT over all n samples, then the mean of |X[k]|^2 over
+ * each run of d consecutive bins (natural order), NaN values skipped (0 when
+ * a run has none).  x: [rows][n][2]; out: [rows][n / d].  Direct O(n^2) DFT in
+ * FP64 with exact phase indices (k t mod n). */
+void orc_bandpass(int rows, int n, int d, const double *x, double *out);
+
 /* Number of OpenMP threads the oracle would use (1 when built without it). */
 int orc_num_threads(void);
 

@@ -98,6 +98,7 @@ def _declare(L):
     L.orc_adaptive_block.restype = ctypes.c_int
     L.orc_num_threads.restype = ctypes.c_int
     L.orc_beam_spectra.argtypes = [ctypes.c_int] * 6 + [_p, _p]
+    L.orc_bandpass.argtypes = [ctypes.c_int] * 3 + [_p, _p]
 
 
 def _ptr(a):
@@ -278,6 +279,18 @@ def beam_spectra(beams, nfft=64, drop=5):
     out = np.empty((C, P, E, nfft - 2 * drop), np.float64)
     lib().orc_beam_spectra(C, P, E, T // nfft, nfft, drop, _ptr(b), _ptr(out))
     return out
+
+
+def bandpass(beams, d):
+    """beams complex [..., n] -> float64 [..., n / d]: mean |DFT|^2 over runs of d bins."""
+    beams = np.asarray(beams)
+    lead, n = beams.shape[:-1], beams.shape[-1]
+    assert n % d == 0
+    rows = int(np.prod(lead)) if lead else 1
+    b = _c(np.stack([beams.real, beams.imag], -1).reshape(rows, n, 2), np.float64)
+    out = np.empty((rows, n // d), np.float64)
+    lib().orc_bandpass(rows, n, d, _ptr(b), _ptr(out))
+    return out.reshape(*lead, n // d)
 
 
 def num_threads() -> int:

@@ -48,6 +48,7 @@ using TwoPacket = LambdaConfig<1, 1, 8, 4, 2, 4, 2, 1, 32, 32, 10000>;
 using MultiFPGA = LambdaConfig<1, 3, 8, 6, 2, 2, 1, 1, 32, 32, 10000>;
 using AutoDump = LambdaConfig<1, 1, 8, 4, 2, 4, 1, 1, 32, 32, 2>;
 using Adaptive = LambdaConfig<1, 1, 64, 4, 2, 4, 1, 1, 32, 32, 1>;
+using SmallGPU = LambdaConfig<1, 1, 8, 4, 2, 4, 1, 1, 32, 32, 10000, false, 2>;  // bandpass of 2-bin runs
 using Adaptive2 = LambdaConfig<2, 1, 64, 4, 2, 4, 1, 1, 32, 32, 1>;
 
 struct FakeProcessorState : ProcessorStateBase {
@@ -371,16 +372,16 @@ TEST(AdaptivePermutationSizeError) {
 // orthogonality to the principal one is checked.  Beams and visibilities as
 // in test_pipeline.cu's Ex1.
 TEST(LambdaGPUEigenvalueBasic) {
-  auto output = std::make_shared<SingleHostMemoryOutput<Small>>();
-  BeamWeightsT<Small> weights = unity_weights<Small>();
-  LambdaGPUPipeline<Small> pipeline(1, &weights);
+  auto output = std::make_shared<SingleHostMemoryOutput<SmallGPU>>();
+  BeamWeightsT<SmallGPU> weights = unity_weights<SmallGPU>();
+  LambdaGPUPipeline<SmallGPU> pipeline(1, &weights);
   FakeProcessorState state;
   pipeline.set_state(&state);
   pipeline.set_output(output);
   int delays[1] = {0};
   pipeline.set_subpacket_delays(delays);
-  typename Small::PacketFinalDataType pd;
-  fill_block<Small>(pd, const_sample, unit_scale);
+  typename SmallGPU::PacketFinalDataType pd;
+  fill_block<SmallGPU>(pd, const_sample, unit_scale);
   pipeline.execute_pipeline(&pd);
   pipeline.dump_visibilities();
   pipeline.synchronize();
@@ -402,6 +403,13 @@ TEST(LambdaGPUEigenvalueBasic) {
       }
     }
   CHECK(output->eig_blocks_done == 1, "eigen callback");
+  // bandpass: the constant beam 8 - 8i is all DC, |X_0|^2 = (8 * |8 - 8i|)^2 = 8192,
+  // averaged with bin 1 (0) into output bin 0
+  CHECK(output->fft_blocks_done == 1, "fft callback");
+  for (int p = 0; p < 2; ++p)
+    for (int m = 0; m < 4; ++m)
+      CHECK(std::fabs(output->bandpass()[0][p][0][m] - (m == 0 ? 4096.0f : 0.0f)) <= 1e-3f,
+            "bandpass m=%d %f", m, output->bandpass()[0][p][0][m]);
   for (size_t t = 0; t < 8; ++t) CHECK(spatial::half_to_float((*output->beam_data)[0][0][0][t][0]) == 8.0f, "beam");
   CHECK((*output->visibilities)[0][0][0][0][0] == 64.0f, "vis");
 }

@@ -113,3 +113,62 @@ def test_scene_vs_oracle(gains):
                 k = np.argmax(np.abs(vg[c, p_, q]), axis=1)
                 top = vg[c, p_, q][np.arange(n), k]
                 assert np.all(top.real > 0) and np.max(np.abs(top.imag)) < 1e-6
+
+
+def test_bandpass_vs_oracle():
+    """Bandpass (detect_and_downsample_fft after a block-length FFT): the GPU's
+    values of its own fp16 beams within 1e-5 of each row's largest bin, and
+    within 2e-3 of the same product of the FP64 oracle's beams (power is the
+    square of the beams, so their fp16 rounding counts twice)."""
+    shape = syn.BlockShape(C=2, F=1, NT=64, R=8, P=2, RPP=8, NP=8, B=4)
+    scene = syn.make_scene(shape, seed=91, n_tones=2)
+    ring, scales = syn.make_block(shape, scene, seed=92)
+    w16, w = syn.random_weights(shape, 93)
+    p = lam.Pipeline(shape, lam.MODE_LAMBDA_GPU)
+    p.set_weights(w16)
+    p.set_bandpass(16)
+    out = p.run_host(ring, scales)
+    p.close()
+    bp = out["spectra_out"].astype(np.float64)
+    own = orc.bandpass(out["beams"].astype(np.complex128), 16)
+    assert np.max(np.abs(bp - own) / np.max(own, axis=-1, keepdims=True)) < 1e-5
+    s = _o(shape)
+    y = orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)
+    fp64 = orc.bandpass(orc.beamform(s, y, w), 16)
+    assert rel_err(bp, fp64) < 2e-3
+
+
+def test_bandpass_8192_points_and_tone():
+    """N = 8192 (three radix-16 passes and a radix-2 pass): against numpy's
+    FFT of the GPU's beams binned by 128, and a tone at FFT bin 1000 on every
+    receiver lands in output bin 1000 // 128 = 7 with (N * 4 * 40)^2 / 128."""
+    shape = syn.BlockShape(C=1, F=1, NT=64, R=4, P=2, RPP=4, NP=128, B=2)
+
+    def tone(c, f, pk, t, r, pol):
+        a = 2 * np.pi * 1000 * (pk * 64 + t) / 8192
+        return (int(round(40 * np.cos(a))), int(round(40 * np.sin(a))))
+    ring, scales = syn.ring_from_fn(shape, tone)
+    p = lam.Pipeline(shape, lam.MODE_LAMBDA_GPU)
+    p.set_weights(syn.unity_weights(shape)[0])
+    p.set_bandpass(128)
+    out = p.run_host(ring, scales)
+    p.close()
+    bp = out["spectra_out"].astype(np.float64)
+    X = np.fft.fft(out["beams"].astype(np.complex128), axis=-1)
+    ref = (np.abs(X) ** 2).reshape(1, 2, 2, 64, 128).mean(-1)
+    assert np.max(np.abs(bp - ref) / np.max(ref, axis=-1, keepdims=True)) < 1e-5
+    assert np.all(np.argmax(bp, axis=-1) == 7)
+    peak = (8192 * 4 * 40.0) ** 2 / 128
+    assert np.all(np.abs(bp[..., 7] - peak) < 0.02 * peak)
+
+
+def test_bandpass_argument_errors():
+    shape = syn.BlockShape(C=1, F=1, NT=48, R=4, P=2, RPP=4, NP=1, B=1)  # 48 samples: not a power of two
+    p = lam.Pipeline(shape, lam.MODE_LAMBDA_GPU)
+    with pytest.raises(lam.LamError):
+        p.set_bandpass(16)
+    p.close()
+    p = lam.Pipeline(syn.BlockShape(C=1, F=1, NT=64, R=4, P=2, RPP=4, NP=1, B=1), lam.MODE_CORR_BEAM)
+    with pytest.raises(lam.LamError):
+        p.set_bandpass(16)
+    p.close()

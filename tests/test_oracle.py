@@ -389,3 +389,26 @@ def test_beam_spectra_packets_sum_and_library_fft():
     assert np.allclose(a, out, rtol=1e-12)
     # |X| is positively homogeneous and phase-blind
     assert np.allclose(orc.beam_spectra(2.5 * np.exp(0.7j) * x), 2.5 * out, rtol=1e-12)
+
+
+# ------------------------------------------------------ LambdaGPU bandpass
+def test_bandpass_tone_and_library_fft():
+    """A complex exponential at bin k0 puts (n amp)^2 into that one bin, so
+    the run holding it reads (n amp)^2 / d and every other run 0; random rows
+    match numpy's FFT binned the same way; a NaN sample leaves no valid value
+    in any run (0, as detect_and_downsample_fft writes)."""
+    n, d = 256, 16
+    t = np.arange(n)
+    for k0 in (0, 5, 100, 255):
+        x = 1.5 * np.exp(2j * np.pi * k0 * t / n)
+        out = orc.bandpass(x[None, :], d)[0]
+        exp = np.zeros(n // d)
+        exp[k0 // d] = (n * 1.5) ** 2 / d
+        assert np.allclose(out, exp, atol=1e-6 * exp.max()), k0
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    ref = (np.abs(np.fft.fft(x, axis=-1)) ** 2).reshape(3, n // d, d).mean(-1)
+    assert np.allclose(orc.bandpass(x, d), ref, rtol=1e-10)
+    y = x[:1].copy()
+    y[0, 7] = np.nan
+    assert np.all(orc.bandpass(y, d) == 0.0)
