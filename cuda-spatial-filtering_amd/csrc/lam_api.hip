@@ -107,6 +107,9 @@ struct lam_ctx {
   bool use_graphs = true, graphs_dirty = true, profiling = false;
   bool spectra = false;  // compute the post-beam spectra in the beam phase
   int bandpass = 0;      // LambdaGPU bandpass: bins averaged per output value (0: off)
+  // projection mode: signal eigenvectors K, blocks per average, blocks since the last dump
+  int proj_k = 1, proj_avg = 1, proj_runs = 0;
+  float *d_proj_acc = nullptr, *d_proj_evals = nullptr, *d_proj_evecs = nullptr;
   int samples_per_wg = 0;
   std::vector<ProfEvent> prof;
   // eigen-stage method
@@ -238,6 +241,20 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
   hipStream_t s = b.stream;
   const long long vis_count = (long long)ctx->C * ctx->NBL * 8;
   const bool atomic_acc = ctx->nbuf > 1;
+  if (ctx->mode == LAM_MODE_PROJECTION) {
+    if (phase != 0) return LAM_OK;
+    VoltParams p = base_volt(ctx, samples, scales);
+    p.window_base = 0;  // no alignment step: ring slots 0..NP-1 as packetToPadding reads them
+    p.vis_part = b.d_vis_part;
+    {
+      Prof pr(ctx, s, LAM_K_CORR);
+      HIPC(ctx, launch_volt(p, true, false, s));
+    }
+    Prof pr(ctx, s, LAM_K_POLEIG);  // lambda_projection_pipeline.hpp:380-431
+    HIPC(ctx, launch_proj_block(b.d_vis_part, p.nsplit, ctx->C, ctx->R, ctx->d_gains, ctx->d_phase, ctx->proj_k,
+                                ctx->max_sweeps, ctx->tol, ctx->d_proj_acc, s));
+    return LAM_OK;
+  }
   if (ctx->mode == LAM_MODE_CORR_BEAM || ctx->mode == LAM_MODE_LAMBDA_GPU) {
     if (phase != 0) return LAM_OK;
     VoltParams p = base_volt(ctx, samples, scales);
@@ -709,7 +726,9 @@ const char *lam_last_error(const lam_ctx *ctx) { return ctx ? ctx->err.c_str() :
 int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
   if (!sh || !out) return LAM_ERR_ARG;
   *out = nullptr;
-  if (mode != LAM_MODE_CORR_BEAM && mode != LAM_MODE_ADAPTIVE && mode != LAM_MODE_LAMBDA_GPU) return LAM_ERR_ARG;
+  if (mode != LAM_MODE_CORR_BEAM && mode != LAM_MODE_ADAPTIVE && mode != LAM_MODE_LAMBDA_GPU &&
+      mode != LAM_MODE_PROJECTION)
+    return LAM_ERR_ARG;
   if (sh->nr_polarizations != 2) return LAM_ERR_UNSUPPORTED;
   if (sh->nr_channels < 1 || sh->nr_fpga_sources < 1 || sh->nr_time_steps_per_packet < 1 ||
       sh->nr_receivers_per_packet < 1 || sh->nr_packets_for_correlation < 1 || sh->nr_beams < 1)
@@ -796,6 +815,12 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
   chk(dalloc(ctx, &ctx->d_pol_perm, (size_t)ctx->R * 2 * sizeof(int)));
   chk(dalloc(ctx, &ctx->d_wop, (size_t)ctx->C * 2 * ctx->NMT1 * ctx->NK * 2 * 64 * sizeof(v8h)));
   chk(dalloc(ctx, &ctx->d_fixed_k, (size_t)ctx->C * 2 * sizeof(int)));
+  if (mode == LAM_MODE_PROJECTION) {
+    const size_t nprob = (size_t)ctx->C * 4;
+    chk(dalloc(ctx, &ctx->d_proj_acc, nprob * ctx->R * ctx->R * 2 * sizeof(float)));
+    chk(dalloc(ctx, &ctx->d_proj_evals, nprob * ctx->R * sizeof(float)));
+    chk(dalloc(ctx, &ctx->d_proj_evecs, nprob * ctx->R * ctx->R * 2 * sizeof(float)));
+  }
   if (mode == LAM_MODE_ADAPTIVE && ctx->NT == 64 && rc == LAM_OK) {
     std::vector<uint16_t> tab(spectra_table_bytes() / 2);
     spectra_table_host(tab.data());
@@ -863,6 +888,9 @@ int lam_destroy(lam_ctx *ctx) {
   hipFree(ctx->d_wop);
   hipFree(ctx->d_fixed_k);
   hipFree(ctx->d_spec_table);
+  hipFree(ctx->d_proj_acc);
+  hipFree(ctx->d_proj_evals);
+  hipFree(ctx->d_proj_evecs);
   hipFree(ctx->d_err);
   hipFree(ctx->d_dump_f32);
   hipFree(ctx->d_dump_i64);
@@ -1120,6 +1148,7 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
     }
   }
   HIPC(ctx, hipEventRecord(b.done, s));
+  if (ctx->mode == LAM_MODE_PROJECTION) ++ctx->proj_runs;
   ctx->last_buf = ctx->cur;
   ctx->blocks_acc += 1;
   ctx->cur = (ctx->cur + 1) % ctx->nbuf;
@@ -1137,6 +1166,40 @@ static int dump_common(lam_ctx *ctx, float *f32, int64_t *i64, lam_host_fn fn, v
   if (i64) HIPC(ctx, hipMemcpyAsync(i64, ctx->d_dump_i64, elems * sizeof(long long), hipMemcpyDefault, s));
   if (fn) HIPC(ctx, hipLaunchHostFunc(s, fn, user));
   ctx->blocks_acc = 0;
+  return LAM_OK;
+}
+
+int lam_set_projection(lam_ctx *ctx, int nr_signal_eigenvectors, int nr_runs_to_average) {
+  if (!ctx) return LAM_ERR_ARG;
+  if (ctx->mode != LAM_MODE_PROJECTION) return fail(ctx, LAM_ERR_STATE, "projection settings need the projection mode");
+  if (nr_signal_eigenvectors < 1 || nr_signal_eigenvectors > ctx->R || nr_runs_to_average < 1)
+    return fail(ctx, LAM_ERR_ARG, "need 1 <= nr_signal_eigenvectors <= %d and nr_runs_to_average >= 1", ctx->R);
+  if (int r = sync_all(ctx)) return r;
+  ctx->proj_k = nr_signal_eigenvectors;
+  ctx->proj_avg = nr_runs_to_average;
+  ctx->graphs_dirty = true;
+  return LAM_OK;
+}
+
+int lam_projection_runs(const lam_ctx *ctx) { return ctx ? ctx->proj_runs : 0; }
+
+int lam_dump_projection(lam_ctx *ctx, float *evals_dst, float *evecs_dst, lam_host_fn fn, void *user) {
+  if (!ctx) return LAM_ERR_ARG;
+  if (ctx->mode != LAM_MODE_PROJECTION) return fail(ctx, LAM_ERR_STATE, "no projection accumulator in this mode");
+  if (ctx->proj_runs == 0) return fail(ctx, LAM_ERR_STATE, "no block integrated since the last projection dump");
+  hipStream_t s = ctx->bufs[0].stream;
+  for (auto &b : ctx->bufs) HIPC(ctx, hipStreamWaitEvent(s, b.done, 0));
+  const size_t nprob = (size_t)ctx->C * 4;
+  HIPC(ctx, launch_proj_dump(ctx->d_proj_acc, ctx->C, ctx->R, ctx->proj_runs, ctx->max_sweeps, ctx->tol,
+                             ctx->d_proj_evals, ctx->d_proj_evecs, s));
+  HIPC(ctx, hipEventRecord(ctx->reset_done, s));  // later blocks add into the cleared accumulator
+  if (evals_dst)
+    HIPC(ctx, hipMemcpyAsync(evals_dst, ctx->d_proj_evals, nprob * ctx->R * sizeof(float), hipMemcpyDefault, s));
+  if (evecs_dst)
+    HIPC(ctx, hipMemcpyAsync(evecs_dst, ctx->d_proj_evecs, nprob * ctx->R * ctx->R * 2 * sizeof(float),
+                             hipMemcpyDefault, s));
+  if (fn) HIPC(ctx, hipLaunchHostFunc(s, fn, user));
+  ctx->proj_runs = 0;
   return LAM_OK;
 }
 

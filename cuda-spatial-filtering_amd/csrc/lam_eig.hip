@@ -1218,6 +1218,125 @@ pol_eig_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *gain
   if (tid == 0 && sweeps_out) sweeps_out[m] = sw;
 }
 
+// ---- projection averaging (LambdaProjectionPipeline) ----
+// Per block (lambda_projection_pipeline.hpp:330-431): the same four pol-pair
+// covariances as pol_eig_kernel, eigenvectors ascending, then the projector
+// onto the K largest, P = U_K U_K^H (cublasCherk, fill UPPER: only i <= j is
+// formed), added into the accumulator [C*4][n][n] (column-major, upper).
+// Atomic adds: blocks of different buffers may run side by side.
+__global__ void __launch_bounds__(kEigThreads)
+proj_block_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *gains, const float2 *phase,
+                  int K, int max_sweeps, float tol, float2 *acc) {
+  extern __shared__ __align__(16) char smem_raw[];
+  EigSmem S = carve(smem_raw, n, 1, false);
+  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
+  const int c = m >> 2, pq = m & 3, p = pq >> 1, q = pq & 1;
+  const int nbl = n * (n + 1) / 2;
+  for (int idx = tid; idx < nbl; idx += blockDim.x) {
+    int j = (int)((sqrtf(8.0f * (float)idx + 1.0f) - 1.0f) * 0.5f);
+    while (j * (j + 1) / 2 > idx) --j;
+    while ((j + 1) * (j + 2) / 2 <= idx) ++j;
+    const int i = idx - j * (j + 1) / 2;
+    long long re = 0, im = 0;
+    for (int k = 0; k < nsplit; ++k) {
+      const int2 v = reinterpret_cast<const int2 *>(vis_part)[(((long)k * C + c) * 4 + pq) * nbl + idx];
+      re += v.x;
+      im += v.y;
+    }
+    const float2 gi = gains[((long)c * 2 + p) * n + i], gj = gains[((long)c * 2 + q) * n + j];
+    const float2 pi = phase[(long)c * n + i], pj = phase[(long)c * n + j];
+    const float2 f = cmul(make_float2(gi.x * gj.x + gi.y * gj.y, gi.y * gj.x - gi.x * gj.y),
+                          make_float2(pi.x * pj.x + pi.y * pj.y, pi.y * pj.x - pi.x * pj.y));
+    S.A[j * ld + i] = cmul(f, make_float2((float)re, (float)im));
+  }
+  __syncthreads();
+  jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.rpq, S.rot, S.red);
+  sort_eigen(S.A, n, S.perm, S.ev);
+  float2 *out = acc + (long)m * n * n;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int j = e / n, i = e - j * n;
+    if (i > j) continue;
+    float2 sum = make_float2(0.f, 0.f);
+    for (int k = n - K; k < n; ++k) {  // U(i, k) conj(U(j, k))
+      const float2 *col = S.V + S.perm[k] * ld;
+      const float2 t = cmulc(col[i], col[j]);
+      sum.x += t.x;
+      sum.y += t.y;
+    }
+    atomicAdd(&out[e].x, sum.x);
+    atomicAdd(&out[e].y, sum.y);
+  }
+}
+
+// Averaged projector of each (channel, p, q): A = acc / runs (upper triangle),
+// its eigenpairs ascending with each eigenvector's largest-magnitude component
+// real and positive, and the accumulator cleared (dump_projection,
+// lambda_projection_pipeline.hpp:571-645).
+__global__ void __launch_bounds__(kEigThreads)
+proj_dump_kernel(float2 *acc, int n, float inv_runs, int max_sweeps, float tol, float *evals, float2 *evecs) {
+  extern __shared__ __align__(16) char smem_raw[];
+  EigSmem S = carve(smem_raw, n, 1, false);
+  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
+  float2 *a = acc + (long)m * n * n;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int j = e / n, i = e - j * n;
+    const float2 v = a[e];
+    a[e] = make_float2(0.f, 0.f);
+    if (i <= j) S.A[j * ld + i] = make_float2(v.x * inv_runs, v.y * inv_runs);
+  }
+  __syncthreads();
+  jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.rpq, S.rot, S.red);
+  sort_eigen(S.A, n, S.perm, S.ev);
+  float2 *ph = S.W;
+  if (tid < n) {
+    const float2 *col = S.V + S.perm[tid] * ld;
+    int best = 0;
+    float bm = -1.0f;
+    for (int r = 0; r < n; ++r) {
+      const float mg = col[r].x * col[r].x + col[r].y * col[r].y;
+      if (mg > bm) {
+        bm = mg;
+        best = r;
+      }
+    }
+    const float2 v = col[best];
+    const float inv = bm > 0.0f ? rsqrtf(bm) : 0.0f;
+    ph[tid] = bm > 0.0f ? make_float2(v.x * inv, -v.y * inv) : make_float2(1.0f, 0.0f);
+  }
+  __syncthreads();
+  if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int k = e / n, r = e - k * n;
+    evecs[(long)m * n * n + e] = cmul(S.V[S.perm[k] * ld + r], ph[k]);
+  }
+}
+
+hipError_t launch_proj_block(const int *vis_part, int nsplit, int C, int n, const float2 *gains,
+                             const float2 *phase, int K, int max_sweeps, float tol, float *acc, hipStream_t s) {
+  const size_t sm = eig_smem_bytes(n, 1, false);
+  if (sm > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(proj_block_kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(proj_block_kernel, dim3(C * 4), dim3(kEigThreads), sm, s, vis_part, nsplit, C, n, gains,
+                     phase, K, max_sweeps, tol, reinterpret_cast<float2 *>(acc));
+  return hipGetLastError();
+}
+
+hipError_t launch_proj_dump(float *acc, int C, int n, int runs, int max_sweeps, float tol, float *evals,
+                            float *evecs, hipStream_t s) {
+  const size_t sm = eig_smem_bytes(n, 1, false);
+  if (sm > 65536) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(proj_dump_kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(proj_dump_kernel, dim3(C * 4), dim3(kEigThreads), sm, s, reinterpret_cast<float2 *>(acc), n,
+                     1.0f / (float)runs, max_sweeps, tol, evals, reinterpret_cast<float2 *>(evecs));
+  return hipGetLastError();
+}
+
 hipError_t launch_pol_eig(const int *vis_part, int nsplit, int C, int n, const float2 *gains,
                           const float2 *phase, int max_sweeps, float tol, float *evals, float *evecs,
                           int *sweeps, hipStream_t s) {

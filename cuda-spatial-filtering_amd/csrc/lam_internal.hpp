@@ -150,6 +150,14 @@ hipError_t launch_features(const FeatParams &p, hipStream_t s);
 // Post-beam spectra: 64-point DFT magnitudes per packet, 54 kept bins, summed
 // over packets.  table: spectra_table_bytes() of device memory filled from
 // spectra_table_host().
+// Projection averaging (LambdaProjectionPipeline): per block add the
+// projector onto the K largest eigenvectors of every pol-pair covariance into
+// acc [C*4][n][n][2] (upper, column-major); the dump decomposes acc / runs into
+// evals [C][2][2][n] / evecs [C][2][2][n][n][2] and clears acc.
+hipError_t launch_proj_block(const int *vis_part, int nsplit, int C, int n, const float2 *gains,
+                             const float2 *phase, int K, int max_sweeps, float tol, float *acc, hipStream_t s);
+hipError_t launch_proj_dump(float *acc, int C, int n, int runs, int max_sweeps, float tol, float *evals,
+                            float *evecs, hipStream_t s);
 // LambdaGPU bandpass: N-point FFT per beam row (N = block length, power of
 // two <= 8192), mean |X|^2 over runs of D bins -> out [rows][N / D].
 hipError_t launch_bandpass(const __half *beams, int rows, int N, int D, float *out, hipStream_t s);

@@ -382,6 +382,71 @@ class LambdaGPUPipeline : public GPUPipeline {
   std::vector<float> scratch_;
 };
 
+// LambdaProjectionPipeline (include/spatial/pipeline/lambda_projection_pipeline.hpp):
+// per block the projector onto the NR_SIGNAL_EIGENVECTORS largest eigenvectors of
+// every (channel, p, q) covariance is accumulated; every NR_RUNS_TO_AVERAGE blocks
+// (and on dump_visibilities) the average is decomposed and its eigenpairs go to
+// the Output's eigendecomposition landing areas.
+template <typename T, int NR_SIGNAL_EIGENVECTORS, int NR_RUNS_TO_AVERAGE>
+class LambdaProjectionPipeline : public GPUPipeline {
+  static_assert(NR_SIGNAL_EIGENVECTORS >= 1 && NR_SIGNAL_EIGENVECTORS <= static_cast<int>(T::NR_RECEIVERS),
+                "NR_SIGNAL_EIGENVECTORS must be in [1, NR_RECEIVERS]");
+  static_assert(NR_RUNS_TO_AVERAGE >= 1, "NR_RUNS_TO_AVERAGE must be >= 1");
+
+ public:
+  explicit LambdaProjectionPipeline(const int num_buffers)
+      : ctx_(T::shape(), LAM_MODE_PROJECTION, std::min(num_buffers, 4)) {
+    ctx_(lam_set_projection(ctx_.get(), NR_SIGNAL_EIGENVECTORS, NR_RUNS_TO_AVERAGE));
+    typename T::PacketFinalDataType warmup_packet;
+    execute_pipeline(&warmup_packet, true);
+    // the warm-up block is not part of any average
+    ctx_(lam_dump_projection(ctx_.get(), nullptr, nullptr, nullptr, nullptr));
+    ctx_(lam_sync(ctx_.get()));
+  }
+
+  void set_antenna_gains(std::complex<float> *gains) override {
+    gains_ = gains;
+    ctx_(lam_set_gains(ctx_.get(), reinterpret_cast<const float *>(gains)));
+  }
+
+  void execute_pipeline(FinalPacketData *packet_data, const bool dummy_run = false) override {
+    if (!dummy_run && state_ == nullptr)
+      throw std::logic_error("State has not been set on GPUPipeline object!");
+    lam_block_io io{};
+    io.samples = packet_data->get_samples_ptr();
+    io.scales = packet_data->get_scales_ptr();
+    io.release_fn = spatial::detail::release_buffer_host_func;
+    io.release_user = new spatial::detail::ReleaseContext{state_, packet_data->buffer_index, dummy_run};
+    ctx_(lam_execute(ctx_.get(), &io));
+    if (!dummy_run && lam_projection_runs(ctx_.get()) >= NR_RUNS_TO_AVERAGE)
+      dump_projection(packet_data->start_seq_id, packet_data->end_seq_id);
+  }
+
+  // Flushes whatever has accumulated (lambda_projection_pipeline.hpp:452-460).
+  void dump_visibilities(const uint64_t end_seq_num = 0) override {
+    if (lam_projection_runs(ctx_.get()) > 0) dump_projection(0, end_seq_num);
+  }
+
+  void synchronize() { ctx_(lam_sync(ctx_.get())); }
+  lam_ctx *native_handle() const { return ctx_.get(); }
+
+ private:
+  void dump_projection(const uint64_t start_seq_num, const uint64_t end_seq_num) {
+    size_t block = std::numeric_limits<size_t>::max();
+    if (output_ != nullptr) block = output_->register_eigendecomposition_data_block(start_seq_num, end_seq_num);
+    if (block != std::numeric_limits<size_t>::max()) {
+      ctx_(lam_dump_projection(ctx_.get(), static_cast<float *>(output_->get_eigenvalues_data_landing_pointer(block)),
+                               static_cast<float *>(output_->get_eigenvectors_data_landing_pointer(block)),
+                               spatial::detail::eig_output_done_host_func,
+                               new spatial::detail::OutputDoneContext{output_, block}));
+    } else {
+      ctx_(lam_dump_projection(ctx_.get(), nullptr, nullptr, nullptr, nullptr));
+    }
+  }
+
+  spatial::detail::Context ctx_;
+};
+
 template <typename T>
 class LambdaAdaptiveBeamformedSpectraPipeline : public GPUPipeline {
  public:

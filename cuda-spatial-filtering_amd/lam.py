@@ -24,7 +24,7 @@ LIB_PATH = os.environ.get("LAM_LIBRARY", os.path.join(_HERE, "liblam.so"))
 HEADER = os.path.join(os.path.dirname(_HERE), "include", "lam.h")
 
 LAM_OK, LAM_ERR_ARG, LAM_ERR_HIP, LAM_ERR_RANGE, LAM_ERR_STATE, LAM_ERR_UNSUPPORTED, LAM_ERR_NO_DEVICE = range(7)
-MODE_CORR_BEAM, MODE_ADAPTIVE, MODE_LAMBDA_GPU = 0, 1, 2
+MODE_CORR_BEAM, MODE_ADAPTIVE, MODE_LAMBDA_GPU, MODE_PROJECTION = 0, 1, 2, 3
 K_CORRBEAM, K_CORR, K_EIG, K_BEAM2B = range(4)
 KERNEL_NAMES = ["corr_beam", "corr", "eig_null", "beam_2b", "features", "mlp", "eig_fallback",
                 "subarray_eig", "spectra", "pol_eig", "bandpass"]
@@ -90,6 +90,9 @@ def load_library(path: str = LIB_PATH):
     L.lam_set_nulling.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_int]
     L.lam_set_spectra.argtypes = [vp, ctypes.c_int]
     L.lam_set_bandpass.argtypes = [vp, ctypes.c_int]
+    L.lam_set_projection.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.lam_projection_runs.argtypes = [vp]
+    L.lam_dump_projection.argtypes = [vp, vp, vp, HOSTFN, vp]
     L.lam_set_eig_params.argtypes = [vp, ctypes.c_int, ctypes.c_float]
     L.lam_set_graphs.argtypes = [vp, ctypes.c_int]
     L.lam_set_profiling.argtypes = [vp, ctypes.c_int]
@@ -352,6 +355,22 @@ class Pipeline:
         self._chk(self.L.lam_set_spectra(self.h, int(bool(enable))))
         self.spectra = bool(enable)
 
+    def set_projection(self, nr_signal_eigenvectors, nr_runs_to_average):
+        self._chk(self.L.lam_set_projection(self.h, int(nr_signal_eigenvectors), int(nr_runs_to_average)))
+
+    def projection_runs(self):
+        return int(self.L.lam_projection_runs(self.h))
+
+    def dump_projection(self):
+        """Average of the accumulated projectors, decomposed: (evals [C][2][2][R],
+        evecs complex [C][2][2][R][R], eigenvector k at [..., k, :])."""
+        s = self.shape
+        ev = np.empty((s.C, 2, 2, s.R), np.float32)
+        vec = np.empty((s.C, 2, 2, s.R, s.R, 2), np.float32)
+        self._chk(self.L.lam_dump_projection(self.h, ev.ctypes.data, vec.ctypes.data, _NULLFN, None))
+        self.sync()
+        return ev, vec[..., 0] + 1j * vec[..., 1]
+
     def set_bandpass(self, downsample_factor):
         """LambdaGPU bandpass: mean |FFT|^2 over runs of downsample_factor bins (0: off)."""
         self._chk(self.L.lam_set_bandpass(self.h, int(downsample_factor)))
@@ -545,6 +564,10 @@ class Pipeline:
 
     def run_host(self, ring, scales):
         """Convenience: one block from host arrays, returns fp16 beams (+eigen)."""
+        if self.mode == MODE_PROJECTION:  # no per-block product: only the accumulator
+            self.execute(np.ascontiguousarray(ring, np.int8), np.ascontiguousarray(scales, np.int16))
+            self.sync()
+            return {}
         beams = self.beams_array()
         s = self.shape
         extra = {}

@@ -69,7 +69,12 @@ enum {
    * window of the adaptive pipeline (slot 1 + per-FPGA delay, stream permutation),
    * plus the eigendecomposition of all four pol-pair covariances of every block
    * (layout "pol-pair eigen" below).  Its bandpass FFT is not computed. */
-  LAM_MODE_LAMBDA_GPU = 2
+  LAM_MODE_LAMBDA_GPU = 2,
+  /* LambdaProjectionPipeline (lambda_projection_pipeline.hpp:307-645): per block the
+   * four pol-pair covariances (ring slots 0..NP-1, gains applied) are decomposed and
+   * the projector onto their K largest eigenvectors is accumulated; a projection
+   * dump decomposes the average.  No beams and no visibility product. */
+  LAM_MODE_PROJECTION = 3
 };
 
 /* Runtime form of LambdaConfig (packet_formats.hpp:265-407). */
@@ -184,6 +189,22 @@ int lam_dump_visibilities(lam_ctx *ctx, float *dst, lam_host_fn done_fn, void *u
 int lam_dump_visibilities_i64(lam_ctx *ctx, int64_t *dst, lam_host_fn done_fn, void *user);
 /* Blocks accumulated since the last dump. */
 int lam_blocks_accumulated(const lam_ctx *ctx);
+
+/* Projection mode: K = nr_signal_eigenvectors (1..R) and the number of blocks
+ * averaged per dump (the NR_SIGNAL_EIGENVECTORS / NR_RUNS_TO_AVERAGE template
+ * arguments; defaults 1, 1).  lam_projection_runs: blocks accumulated since the
+ * last dump; a caller dumps when it reaches nr_runs_to_average, as
+ * execute_pipeline does (lambda_projection_pipeline.hpp:432-438). */
+int lam_set_projection(lam_ctx *ctx, int nr_signal_eigenvectors, int nr_runs_to_average);
+int lam_projection_runs(const lam_ctx *ctx);
+/* Enqueue dump_projection (lambda_projection_pipeline.hpp:571-645): the average of
+ * the accumulated projectors U_K U_K^H (upper triangle, as cublasCherk fills it) is
+ * decomposed; eigenvalues [C][P][P][R] ascending and eigenvectors
+ * [C][P][P][R][R][2] (eigenvector k at [k][0..R), largest-magnitude component
+ * real and positive) land in evals_dst / evecs_dst (host or device, either may
+ * be NULL); the accumulator is cleared.  LAM_ERR_STATE when no block was
+ * accumulated since the last dump. */
+int lam_dump_projection(lam_ctx *ctx, float *evals_dst, float *evecs_dst, lam_host_fn done_fn, void *user);
 
 /* Wait for all enqueued work; reports LAM_ERR_RANGE if any sample scaled
  * out of int8 range since the last call. */

@@ -414,6 +414,36 @@ TEST(LambdaGPUEigenvalueBasic) {
   CHECK((*output->visibilities)[0][0][0][0][0] == 64.0f, "vis");
 }
 
+// LambdaProjectionPipeline: every sample (2, -2) makes each pol-pair
+// covariance 64 everywhere, its top eigenvector 0.5 * ones, so each block's
+// projector (K = 1) is 0.25 everywhere; after NR_RUNS_TO_AVERAGE = 2 blocks the
+// average is dumped: eigenvalues 0, 0, 0, 1 and principal eigenvector 0.5.
+TEST(ProjectionAverageBasic) {
+  auto output = std::make_shared<SingleHostMemoryOutput<Small>>();
+  LambdaProjectionPipeline<Small, 1, 2> pipeline(2);
+  FakeProcessorState state;
+  pipeline.set_state(&state);
+  pipeline.set_output(output);
+  for (int b = 0; b < 2; ++b) {
+    typename Small::PacketFinalDataType pd;
+    fill_block<Small>(pd, const_sample, unit_scale);
+    pipeline.execute_pipeline(&pd);
+    pipeline.synchronize();
+  }
+  CHECK(output->eig_blocks_done == 1, "one projection dump after two blocks (%d)", output->eig_blocks_done.load());
+  auto &ev = *output->eigenvalues;
+  auto &vec = *output->eigenvectors;
+  for (int p = 0; p < 2; ++p)
+    for (int q = 0; q < 2; ++q) {
+      for (int n = 0; n < 4; ++n)
+        CHECK(std::fabs(ev[0][p][q][n] - (n == 3 ? 1.0f : 0.0f)) <= 1e-4f, "eig %d %f", n, ev[0][p][q][n]);
+      for (int l = 0; l < 4; ++l)
+        CHECK(std::fabs(vec[0][p][q][3][l].real() - 0.5f) <= 1e-4f && std::fabs(vec[0][p][q][3][l].imag()) <= 1e-4f,
+              "principal");
+    }
+  CHECK(state.released.size() == 2, "released twice");
+}
+
 // Overlapped ingest (ProcessorState -> pinned buffers -> async H2D,
 // spatial.hpp:718-770, 1182-1299): 12 blocks through 3 pinned buffers, the
 // producer refilling each buffer the moment the pipeline releases it.  A
@@ -467,6 +497,7 @@ int main() {
   RUN(AdaptiveShrinkAndDetection);
   RUN(AdaptivePermutationSizeError);
   RUN(LambdaGPUEigenvalueBasic);
+  RUN(ProjectionAverageBasic);
   RUN(BlockRingOverlappedIngest);
   std::printf("%d checks, %d failures\n", g_checks, g_fail);
   return g_fail == 0 ? 0 : 1;

@@ -172,3 +172,62 @@ def test_bandpass_argument_errors():
     with pytest.raises(lam.LamError):
         p.set_bandpass(16)
     p.close()
+
+
+def _herm_upper(A):
+    """Hermitian matrix with A's upper triangle (real diagonal), as cuSOLVER's
+    fill-UPPER mode reads it."""
+    return np.triu(A, 1) + np.triu(A, 1).conj().T + np.diag(A.diagonal().real)
+
+
+def test_projection_average_vs_numpy():
+    """LambdaProjectionPipeline: three blocks, K = 2, averaged every 3 blocks.
+    The reference of each (channel, p, q) is built with numpy: per block the
+    projector onto the two largest eigenvectors of the covariance (upper
+    triangle), averaged, decomposed.  Checked where every block's K-th and
+    (K+1)-th eigenvalues are separated by more than 1 % of the largest (else the
+    projector itself is ill-defined); the diagonal pol pairs always are."""
+    shape = syn.BlockShape(C=2, F=1, NT=64, R=12, P=2, RPP=12, NP=2, B=1)
+    scene = syn.make_scene(shape, seed=101, n_tones=2, inr_db=(20, 30))
+    blocks = [syn.make_block(shape, scene, seed=110 + b, time_offset=b * shape.T) for b in range(3)]
+    p = lam.Pipeline(shape, lam.MODE_PROJECTION, num_buffers=2)
+    p.set_projection(2, 3)
+    p.set_eig_params(20, 1e-7)
+    for ring, scales in blocks:
+        p.run_host(ring, scales)
+    assert p.projection_runs() == 3
+    ev, vec = p.dump_projection()
+    assert p.projection_runs() == 0
+    with pytest.raises(lam.LamError):
+        p.dump_projection()
+    p.close()
+    s = _o(shape)
+    n, K = shape.R, 2
+    Pacc = np.zeros((shape.C, 2, 2, n, n), complex)
+    sep_ok = np.ones((shape.C, 2, 2), bool)
+    for ring, scales in blocks:
+        raw = orc.correlate(s, orc.ingest(s, ring, scales, orc.WINDOW_CORR_BEAM))
+        for c in range(shape.C):
+            for p_ in range(2):
+                for q in range(2):
+                    A = np.zeros((n, n), complex)
+                    for j in range(n):
+                        for i in range(j + 1):
+                            A[i, j] = raw[c, bl(i, j), p_, q, 0] + 1j * raw[c, bl(i, j), p_, q, 1]
+                    lam_b, U = np.linalg.eigh(_herm_upper(A))
+                    sep_ok[c, p_, q] &= (lam_b[-K] - lam_b[-K - 1]) > 1e-2 * np.max(np.abs(lam_b))
+                    Pacc[c, p_, q] += U[:, -K:] @ U[:, -K:].conj().T
+    checked = 0
+    for c in range(shape.C):
+        for p_ in range(2):
+            for q in range(2):
+                if p_ == q:
+                    assert sep_ok[c, p_, q]
+                if not sep_ok[c, p_, q]:
+                    continue
+                lam_r, U_r = np.linalg.eigh(_herm_upper(Pacc[c, p_, q] / 3))
+                assert np.max(np.abs(ev[c, p_, q] - lam_r)) < 1e-4, (c, p_, q)
+                Ug = vec[c, p_, q, -K:].T
+                assert np.max(np.abs(Ug @ Ug.conj().T - U_r[:, -K:] @ U_r[:, -K:].conj().T)) < 1e-3, (c, p_, q)
+                checked += 1
+    assert checked >= 4
