@@ -188,6 +188,44 @@ def run_lambda_gpu(shape, d_ring, d_scales, w16, args):
             "eigenproblems_per_block": shape.C * 4}
 
 
+def run_projection(shape, d_ring, d_scales, args, k=1, runs=4):
+    """LambdaProjectionPipeline on the same block: per block the projector onto
+    the K largest eigenvectors of all four pol-pair covariances is accumulated
+    (1536 problems), and every `runs` blocks the average is decomposed (the
+    dump is inside the timed loop at that cadence)."""
+    n = max(12, (args.steps // 3) // runs * runs)
+    p = lam.Pipeline(shape, lam.MODE_PROJECTION, num_buffers=args.buffers)
+    p.set_projection(k, runs)
+    p.set_eig_params(args.sweeps, 1e-6)
+    p.set_tuning(args.spw)
+    for _ in range(runs):
+        p.execute(d_ring, d_scales, on_device=True)
+    p.dump_projection()
+    ev = lam.HostArray((shape.C, 2, 2, shape.R), np.float32)
+    vec = lam.HostArray((shape.C, 2, 2, shape.R, shape.R, 2), np.float32)
+    t0 = time.perf_counter()
+    for i in range(n):
+        p.execute(d_ring, d_scales, on_device=True)
+        if (i + 1) % runs == 0:
+            p.dump_projection(wait=False, out=(ev.array, vec.array))
+    p.sync()
+    t1 = time.perf_counter()
+    p.set_profiling(True)
+    p.kernel_times()
+    for _ in range(runs):
+        p.execute(d_ring, d_scales, on_device=True)
+        p.sync()
+    kt = p.kernel_times()
+    p.dump_projection()
+    p.close()
+    ev.free()
+    vec.free()
+    sec = (t1 - t0) / n
+    return {"ms_per_step": sec * 1e3, "real_time_factor": (shape.T / CHANNEL_RATE_HZ) / sec,
+            "nr_signal_eigenvectors": k, "nr_runs_to_average": runs,
+            "kernels_us": {k2: round(v[0] / max(1, v[1]) * 1e3, 1) for k2, v in kt.items()}}
+
+
 def run_learned(args):
     """Learned warm start on 384 ch x 2 pol x 36 ant with drifting, switching
     interferers: blocks [0, train) run the plain solver and are captured as the
@@ -627,10 +665,12 @@ def main():
             roofline["traffic"] = ent["bytes_per_launch"].get(dominant)
             roofline["traffic_splits"] = nsplit
 
-    lambda_gpu = None
+    lambda_gpu = projection = None
     if not args.no_lambda_gpu:
         lambda_gpu = run_lambda_gpu(shape, d_ring, d_scales, w16, args)
         log(f"lambda-gpu: {lambda_gpu}")
+        projection = run_projection(shape, d_ring, d_scales, args)
+        log(f"projection: {projection}")
 
     learned = None
     if not args.no_learned:
@@ -681,6 +721,7 @@ def main():
         "host_fed_pcie_inclusive": host_fed,
         "spectra_on": spectra,
         "lambda_gpu_mode": lambda_gpu,
+        "projection_mode": projection,
         "uneven_subarrays": subarrays,
         "cpu_baseline": cpu,
     }

@@ -361,14 +361,24 @@ class Pipeline:
     def projection_runs(self):
         return int(self.L.lam_projection_runs(self.h))
 
-    def dump_projection(self):
+    def dump_projection(self, wait=True, out=None):
         """Average of the accumulated projectors, decomposed: (evals [C][2][2][R],
-        evecs complex [C][2][2][R][R], eigenvector k at [..., k, :])."""
+        evecs complex [C][2][2][R][R], eigenvector k at [..., k, :]).  wait=False
+        only enqueues the dump (the arrays fill once the stream gets there) and
+        returns the raw float32 landing arrays; out = (evals, evecs) arrays of
+        those shapes (pinned HostArray views keep the copies asynchronous)."""
         s = self.shape
-        ev = np.empty((s.C, 2, 2, s.R), np.float32)
-        vec = np.empty((s.C, 2, 2, s.R, s.R, 2), np.float32)
+        if out is None:
+            ev = np.empty((s.C, 2, 2, s.R), np.float32)
+            vec = np.empty((s.C, 2, 2, s.R, s.R, 2), np.float32)
+        else:
+            ev, vec = out
         self._chk(self.L.lam_dump_projection(self.h, ev.ctypes.data, vec.ctypes.data, _NULLFN, None))
+        if not wait:
+            self._keep.append((ev, vec))
+            return ev, vec
         self.sync()
+        self._keep.clear()
         return ev, vec[..., 0] + 1j * vec[..., 1]
 
     def set_bandpass(self, downsample_factor):
