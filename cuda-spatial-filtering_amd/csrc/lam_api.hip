@@ -109,6 +109,7 @@ struct lam_ctx {
   int bandpass = 0;      // LambdaGPU bandpass: bins averaged per output value (0: off)
   // projection mode: signal eigenvectors K, blocks per average, blocks since the last dump
   int proj_k = 1, proj_avg = 1, proj_runs = 0;
+  int whiten = 1;  // adaptive: whiten the covariance before the eigen solve
   float *d_proj_acc = nullptr, *d_proj_evals = nullptr, *d_proj_evecs = nullptr;
   int samples_per_wg = 0;
   std::vector<ProfEvent> prof;
@@ -354,6 +355,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
     e.detect = ctx->detect;
     e.delta = ctx->delta;
     e.shrink = ctx->shrink;
+    e.whiten = ctx->whiten;
     e.max_sweeps = ctx->max_sweeps;
     e.tol = ctx->tol;
     e.NMT = ctx->NMT2;
@@ -992,6 +994,17 @@ int lam_set_spectra(lam_ctx *ctx, int enable) {
   return LAM_OK;
 }
 
+int lam_set_whitening(lam_ctx *ctx, int enable) {
+  if (!ctx) return LAM_ERR_ARG;
+  if (ctx->mode != LAM_MODE_ADAPTIVE) return fail(ctx, LAM_ERR_STATE, "whitening applies to the adaptive mode");
+  if (!enable && ctx->eig_method != LAM_EIG_PLAIN)
+    return fail(ctx, LAM_ERR_STATE, "the warm starts work on the whitened covariance: set the plain solver first");
+  if (int r = sync_all(ctx)) return r;
+  ctx->whiten = enable ? 1 : 0;
+  ctx->graphs_dirty = true;
+  return LAM_OK;
+}
+
 int lam_set_bandpass(lam_ctx *ctx, int downsample_factor) {
   if (!ctx || downsample_factor < 0) return LAM_ERR_ARG;
   if (downsample_factor > 0 &&
@@ -1435,6 +1448,8 @@ int lam_set_eig_method(lam_ctx *ctx, int method, int refine_sweeps, int power, f
       !(residual_tol >= 0.0f))
     return fail(ctx, LAM_ERR_ARG, "lam_set_eig_method: bad argument");
   if (ctx->mode != LAM_MODE_ADAPTIVE) return fail(ctx, LAM_ERR_STATE, "eigen method needs the adaptive mode");
+  if (method != LAM_EIG_PLAIN && !ctx->whiten)
+    return fail(ctx, LAM_ERR_STATE, "the warm starts work on the whitened covariance: whitening is off");
   if (method != LAM_EIG_PLAIN)
     if (int r = ensure_learn(ctx, 4, 256, 1, false)) return r;
   if (int r = sync_all(ctx)) return r;

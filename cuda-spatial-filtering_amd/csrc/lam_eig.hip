@@ -781,16 +781,20 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
     S.A[i * ld + j] = make_float2(a.x, -a.y);
   }
   __syncthreads();
-  // 2. whitening (spatial.cuh:1065-1093)
-  if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f));
+  // 2. whitening (spatial.cuh:1065-1093); off: the raw covariance, as the
+  //    pulsar-fold pipeline's RFI branch decomposes it (d = 1 makes the
+  //    unwhitening below a no-op)
+  if (tid < n) S.d[tid] = P.whiten ? sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f)) : 1.0f;
   __syncthreads();
-  for (int i = tid; i < n * n; i += nthr) {
-    const int col = i / n, row = i - col * n;
-    const float den = S.d[row] * S.d[col];
-    S.A[col * ld + row].x /= den;
-    S.A[col * ld + row].y /= den;
+  if (P.whiten) {
+    for (int i = tid; i < n * n; i += nthr) {
+      const int col = i / n, row = i - col * n;
+      const float den = S.d[row] * S.d[col];
+      S.A[col * ld + row].x /= den;
+      S.A[col * ld + row].y /= den;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   // 3. eigen decomposition: plain cyclic Jacobi, or a warm start refined by a
   //    few block sweeps and accepted only if the top-K residual is within tolerance
   const long cp = (long)c * 2 + p;

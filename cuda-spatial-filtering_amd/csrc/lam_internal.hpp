@@ -100,6 +100,7 @@ struct EigParams {
   float *orth;             // out: [C*2] FP64 check max |U_K^H U_K - I| of the removed (whitened) basis
   int plain_threads;       // workgroup width of the plain solver (256, 512, 1024)
   int wprec;               // precision of the [W; W'] beam operands (kWprec*)
+  int whiten;              // 1: whiten the covariance first (adaptive); 0: raw (pulsar-fold RFI branch)
 };
 
 // Eigen stage of subarrays (uneven sizes), one persistent launch.

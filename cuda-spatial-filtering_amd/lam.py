@@ -90,6 +90,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_set_nulling.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_int]
     L.lam_set_spectra.argtypes = [vp, ctypes.c_int]
     L.lam_set_bandpass.argtypes = [vp, ctypes.c_int]
+    L.lam_set_whitening.argtypes = [vp, ctypes.c_int]
     L.lam_set_projection.argtypes = [vp, ctypes.c_int, ctypes.c_int]
     L.lam_projection_runs.argtypes = [vp]
     L.lam_dump_projection.argtypes = [vp, vp, vp, HOSTFN, vp]
@@ -354,6 +355,10 @@ class Pipeline:
         """Post-beam spectra (adaptive mode, 64 time steps per packet)."""
         self._chk(self.L.lam_set_spectra(self.h, int(bool(enable))))
         self.spectra = bool(enable)
+
+    def set_whitening(self, enable=True):
+        """Off: the pulsar-fold RFI branch (eigenvectors of the raw covariance)."""
+        self._chk(self.L.lam_set_whitening(self.h, int(bool(enable))))
 
     def set_projection(self, nr_signal_eigenvectors, nr_runs_to_average):
         self._chk(self.L.lam_set_projection(self.h, int(nr_signal_eigenvectors), int(nr_runs_to_average)))

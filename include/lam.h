@@ -129,6 +129,14 @@ int lam_set_fixed_k(lam_ctx *ctx, const int32_t *k);
 /* Null/shrink and detection switches (adaptive ctor 1014-1020, 687-757). */
 int lam_set_nulling(lam_ctx *ctx, int detect, float delta, int shrink);
 
+/* Whitening of each covariance before the eigen solve (adaptive mode; on by
+ * default, spatial.cuh:1065-1093).  0 is the pulsar-fold pipeline's RFI branch
+ * (lambda_pulsar_fold_pipeline.hpp:481-640): the raw XX / YY covariance is
+ * decomposed, P = I - U_K U_K^H with its K largest eigenvectors, W' = W P, and
+ * the eigen outputs are the raw covariance's.  Plain solver only: LAM_ERR_STATE
+ * with a warm start set (either order). */
+int lam_set_whitening(lam_ctx *ctx, int enable);
+
 /* Post-beam spectra of the adaptive pipeline (register_fft_block,
  * lambda_adaptive_beamformed_spectra_pipeline.hpp:808-848, 1173-1178): when
  * enabled, each block also computes, per (channel, pol, output beam), the

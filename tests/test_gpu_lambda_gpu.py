@@ -231,3 +231,41 @@ def test_projection_average_vs_numpy():
                 assert np.max(np.abs(Ug @ Ug.conj().T - U_r[:, -K:] @ U_r[:, -K:].conj().T)) < 1e-3, (c, p_, q)
                 checked += 1
     assert checked >= 4
+
+
+def test_pulsar_fold_rfi_branch_unwhitened():
+    """Pulsar-fold RFI branch (lambda_pulsar_fold_pipeline.hpp:481-640): the
+    adaptive path with whitening off.  Eigenvalues of the raw XX / YY
+    covariance against LAPACK, and the mitigated beams against W (I - U_K U_K^H)
+    applied to the oracle's samples, U_K the two largest LAPACK eigenvectors."""
+    shape = syn.BlockShape(C=3, F=1, NT=64, R=16, P=2, RPP=16, NP=4, B=2)
+    scene = syn.make_scene(shape, seed=121, n_tones=2, inr_db=(15, 30))
+    ring, scales = syn.make_block(shape, scene, seed=122)
+    w16, w = syn.random_weights(shape, 123)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+    p.set_weights(w16)
+    p.set_fixed_k(np.full((shape.C, 2), 2))
+    p.set_whitening(False)
+    with pytest.raises(lam.LamError):
+        p.set_eig_method(lam.EIG_PREVIOUS)
+    p.set_eig_params(20, 1e-7)
+    out = p.run_host(ring, scales)
+    p.close()
+    s = _o(shape)
+    y = orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)
+    raw = orc.correlate(s, y)
+    n, K = shape.R, 2
+    w2 = np.zeros_like(w)
+    for c in range(shape.C):
+        for p_ in range(2):
+            A = np.zeros((n, n), complex)
+            for j in range(n):
+                for i in range(j + 1):
+                    A[i, j] = raw[c, bl(i, j), p_, p_, 0] + 1j * raw[c, bl(i, j), p_, p_, 1]
+            H = np.triu(A, 1) + np.triu(A, 1).conj().T + np.diag(A.diagonal().real)
+            lam_r, U = np.linalg.eigh(H)
+            assert np.max(np.abs(out["evals_out"][c, p_] - lam_r)) <= 1e-4 * lam_r[-1], (c, p_)
+            Pm = np.eye(n) - U[:, -K:] @ U[:, -K:].conj().T
+            w2[c, p_] = w[c, p_] @ Pm
+    assert rel_err(out["beams"][:, :, :2], orc.beamform(s, y, w)) < 1e-3
+    assert rel_err(out["beams"][:, :, 2:], orc.beamform(s, y, w2)) < 1e-3
