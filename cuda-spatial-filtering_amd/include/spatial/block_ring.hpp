@@ -56,9 +56,12 @@ class BlockRing : public ProcessorStateBase {
 
   // Producer: wait until a buffer has been released by the pipeline and
   // return its index; the caller owns it until submit().
+  // Throws once the feeder has stopped on a pipeline error and no buffer is
+  // free (nothing would ever release one).
   size_t acquire() {
     std::unique_lock<std::mutex> lock(mu_);
-    cv_free_.wait(lock, [this] { return !free_.empty(); });
+    cv_free_.wait(lock, [this] { return !free_.empty() || failed_; });
+    if (free_.empty()) throw std::runtime_error("BlockRing: the pipeline failed; no buffer will be released");
     const size_t i = free_.front();
     free_.pop_front();
     return i;
@@ -75,6 +78,11 @@ class BlockRing : public ProcessorStateBase {
     {
       std::lock_guard<std::mutex> lock(mu_);
       if (stopping_) throw std::logic_error("BlockRing::submit after stop");
+      if (failed_) {  // nothing executes any more: the buffer goes straight back
+        free_.push_back(i);
+        cv_free_.notify_one();
+        throw std::runtime_error("BlockRing: the pipeline failed; block not executed");
+      }
       missing_packets_ += missing;
       ready_.push_back(i);
     }
@@ -89,8 +97,10 @@ class BlockRing : public ProcessorStateBase {
     feeder_ = std::thread([this, &pipeline] { feed(pipeline); });
   }
 
-  // Execute every submitted block, then join the feeder.  Rethrows an
-  // exception raised by execute_pipeline.
+  // Execute every submitted block, join the feeder and wait until the
+  // pipeline has released every block it executed (a GPUPipeline releases
+  // each non-dummy block after its H2D copy), so that no release arrives
+  // after the ring is gone.  Rethrows an exception raised by execute_pipeline.
   void stop() {
     {
       std::lock_guard<std::mutex> lock(mu_);
@@ -98,6 +108,10 @@ class BlockRing : public ProcessorStateBase {
     }
     cv_ready_.notify_all();
     if (feeder_.joinable()) feeder_.join();
+    {
+      std::unique_lock<std::mutex> lock(mu_);
+      cv_free_.wait(lock, [this] { return released_ >= executed_; });
+    }
     if (error_) {
       auto e = error_;
       error_ = nullptr;
@@ -109,12 +123,12 @@ class BlockRing : public ProcessorStateBase {
   void release_buffer(const size_t buffer_index) override {
     Block &b = block(buffer_index);
     std::memset(b.get_arrivals_ptr(), 0, b.get_arrivals_size());
-    {
-      std::lock_guard<std::mutex> lock(mu_);
-      free_.push_back(buffer_index);
-      ++released_;
-    }
-    cv_free_.notify_one();
+    // notify under the lock: stop() may return (and the ring be destroyed) as
+    // soon as it can see this release, which it cannot before the unlock
+    std::lock_guard<std::mutex> lock(mu_);
+    free_.push_back(buffer_index);
+    ++released_;
+    cv_free_.notify_all();
   }
 
   size_t blocks_executed() const {
@@ -149,6 +163,7 @@ class BlockRing : public ProcessorStateBase {
         {
           std::lock_guard<std::mutex> lock(mu_);
           error_ = std::current_exception();
+          failed_ = true;
           free_.push_back(i);
           for (size_t j : ready_) free_.push_back(j);
           ready_.clear();
@@ -166,7 +181,7 @@ class BlockRing : public ProcessorStateBase {
   std::condition_variable cv_free_, cv_ready_;
   std::deque<size_t> free_, ready_;
   std::thread feeder_;
-  bool stopping_ = false;
+  bool stopping_ = false, failed_ = false;
   size_t executed_ = 0, released_ = 0;
   long missing_packets_ = 0;
   std::exception_ptr error_;
