@@ -999,6 +999,8 @@ int lam_set_whitening(lam_ctx *ctx, int enable) {
   if (ctx->mode != LAM_MODE_ADAPTIVE) return fail(ctx, LAM_ERR_STATE, "whitening applies to the adaptive mode");
   if (!enable && ctx->eig_method != LAM_EIG_PLAIN)
     return fail(ctx, LAM_ERR_STATE, "the warm starts work on the whitened covariance: set the plain solver first");
+  if (!enable && ctx->sub_G > 0)
+    return fail(ctx, LAM_ERR_STATE, "the subarray eigen stage always whitens: clear the subarrays first");
   if (int r = sync_all(ctx)) return r;
   ctx->whiten = enable ? 1 : 0;
   ctx->graphs_dirty = true;
@@ -1007,6 +1009,8 @@ int lam_set_whitening(lam_ctx *ctx, int enable) {
 
 int lam_set_bandpass(lam_ctx *ctx, int downsample_factor) {
   if (!ctx || downsample_factor < 0) return LAM_ERR_ARG;
+  if (ctx->mode != LAM_MODE_LAMBDA_GPU)  // (the adaptive spectra share the landing buffers)
+    return downsample_factor == 0 ? LAM_OK : fail(ctx, LAM_ERR_ARG, "bandpass needs the LambdaGPU mode");
   if (downsample_factor > 0 &&
       (ctx->mode != LAM_MODE_LAMBDA_GPU || ctx->T > 8192 || (ctx->T & (ctx->T - 1)) || ctx->T % downsample_factor))
     return fail(ctx, LAM_ERR_ARG,
@@ -1371,8 +1375,11 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
   if (rc == LAM_OK) {
     hipStream_t st = ctx->bufs[0].stream;
     if (launch_zero(ctx->d_vis_acc, (size_t)ctx->C * ctx->NBL * 8 * sizeof(long long), st) != hipSuccess ||
+        (ctx->d_proj_acc &&
+         launch_zero(ctx->d_proj_acc, (size_t)ctx->C * 4 * ctx->R * ctx->R * 2 * sizeof(float), st) != hipSuccess) ||
         hipStreamSynchronize(st) != hipSuccess)
       rc = fail(ctx, LAM_ERR_HIP, "autotune: clearing the accumulator failed");
+    ctx->proj_runs = 0;
   }
   if (result) {
     result[0] = ctx->samples_per_wg;
@@ -1385,6 +1392,8 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
 int lam_set_subarrays(lam_ctx *ctx, int ngroups, const uint8_t *mask, const int32_t *k) {
   if (!ctx || ngroups < 0 || (ngroups > 0 && (!mask || !k))) return LAM_ERR_ARG;
   if (ctx->mode != LAM_MODE_ADAPTIVE) return fail(ctx, LAM_ERR_STATE, "subarrays need the adaptive mode");
+  if (ngroups > 0 && !ctx->whiten)
+    return fail(ctx, LAM_ERR_STATE, "the subarray eigen stage whitens: turn whitening back on first");
   if (int r = sync_all(ctx)) return r;
   free_subarrays(ctx);
   ctx->graphs_dirty = true;

@@ -133,8 +133,9 @@ int lam_set_nulling(lam_ctx *ctx, int detect, float delta, int shrink);
  * default, spatial.cuh:1065-1093).  0 is the pulsar-fold pipeline's RFI branch
  * (lambda_pulsar_fold_pipeline.hpp:481-640): the raw XX / YY covariance is
  * decomposed, P = I - U_K U_K^H with its K largest eigenvectors, W' = W P, and
- * the eigen outputs are the raw covariance's.  Plain solver only: LAM_ERR_STATE
- * with a warm start set (either order). */
+ * the eigen outputs are the raw covariance's.  Plain solver only, and no
+ * subarrays (their eigen stage always whitens): LAM_ERR_STATE otherwise, in
+ * either order. */
 int lam_set_whitening(lam_ctx *ctx, int enable);
 
 /* Post-beam spectra of the adaptive pipeline (register_fft_block,
@@ -152,9 +153,10 @@ int lam_set_spectra(lam_ctx *ctx, int enable);
  * pol, beam) the forward FFT over the whole block (NP * NT points), then the
  * mean of |X|^2 over each run of `downsample_factor` consecutive bins (natural
  * FFT order, NaN skipped) -> float32 [C][P][B][NP*NT / downsample_factor],
- * returned through lam_block_io.spectra_dst.  0 turns it off (the default).
- * LAM_ERR_ARG outside the LambdaGPU mode, for a block length that is not a
- * power of two up to 8192, or for a factor that does not divide it. */
+ * returned through lam_block_io.spectra_dst.  0 turns it off (the default; a
+ * no-op in the other modes).  LAM_ERR_ARG for a factor > 0 outside the
+ * LambdaGPU mode, for a block length that is not a power of two up to 8192,
+ * or for a factor that does not divide it. */
 int lam_set_bandpass(lam_ctx *ctx, int downsample_factor);
 /* Eigensolver controls: maximum Jacobi sweeps and convergence tolerance on
  * the relative off-diagonal Frobenius norm. */

@@ -172,6 +172,17 @@ def test_bandpass_argument_errors():
     with pytest.raises(lam.LamError):
         p.set_bandpass(16)
     p.close()
+    # 0 is a no-op outside the LambdaGPU mode: the adaptive spectra keep their buffers
+    shape = syn.BlockShape(C=1, F=1, NT=64, R=4, P=2, RPP=4, NP=1, B=1)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+    p.set_weights(syn.unity_weights(shape)[0])
+    p.set_fixed_k(np.zeros((1, 2)))
+    p.set_spectra(True)
+    p.set_bandpass(0)
+    ring, scales = syn.ring_from_fn(shape, lambda *a: (2, -2))
+    out = p.run_host(ring, scales)
+    p.close()
+    assert abs(out["spectra_out"][0, 0, 0, 27] - 64 * 4 * np.sqrt(8.0)) < 1e-2
 
 
 def _herm_upper(A):
