@@ -380,7 +380,8 @@ class Pipeline:
             ev, vec = out
         self._chk(self.L.lam_dump_projection(self.h, ev.ctypes.data, vec.ctypes.data, _NULLFN, None))
         if not wait:
-            self._keep.append((ev, vec))
+            if out is None:  # our own arrays must outlive the copy; the caller keeps theirs
+                self._keep.append((ev, vec))
             return ev, vec
         self.sync()
         self._keep.clear()
