@@ -203,26 +203,34 @@ hipError_t launch_spectra(const __half *beams, const void *table, int C, int nbe
 // a mixed-radix Stockham FFT in place: each pass reads every butterfly's R
 // inputs into registers, then (after a barrier) writes the R-point DFTs to
 // their autosorted places, so the result is in natural order with no
-// bit-reversal pass.  Radix 16 while it fits, then 8, 4 or 2.  Each
+// bit-reversal pass.  N = 2^r 32^q: one 2^r-point pass, then radix-32
+// passes (8192 = 8 x 32 x 32: three passes, bank-conflict free with one pad
+// slot per 32 elements; radix 16 + 16 + 16 + 2 took four passes).  Each
 // butterfly's twiddles are powers of W_N^{k N/(Ns R)}, computed with sincospif
 // (a table read from global memory put its latency into every pass).
 namespace {
 
-// cos / sin (2 pi k / 16), k = 0..7: the twiddles inside a 16-point DFT.
-__device__ constexpr float kCos16[8] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
-                                        0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f};
-__device__ constexpr float kSin16[8] = {0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f,
-                                        1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f};
+// cos / sin (2 pi k / 32), k = 0..15: the twiddles inside DFTs of up to 32 points.
+__device__ constexpr float kCos32[16] = {
+    1.0f, 0.98078528040323043f, 0.92387953251128674f, 0.83146961230254524f,
+    0.70710678118654752f, 0.55557023301960218f, 0.38268343236508977f, 0.19509032201612826f,
+    0.0f, -0.19509032201612826f, -0.38268343236508977f, -0.55557023301960218f,
+    -0.70710678118654752f, -0.83146961230254524f, -0.92387953251128674f, -0.98078528040323043f};
+__device__ constexpr float kSin32[16] = {
+    0.0f, 0.19509032201612826f, 0.38268343236508977f, 0.55557023301960218f,
+    0.70710678118654752f, 0.83146961230254524f, 0.92387953251128674f, 0.98078528040323043f,
+    1.0f, 0.98078528040323043f, 0.92387953251128674f, 0.83146961230254524f,
+    0.70710678118654752f, 0.55557023301960218f, 0.38268343236508977f, 0.19509032201612826f};
 
-// t = W16^I * o for I = 0..7 (W16 = exp(-2 pi i / 16)); 1 and -i need no multiply.
+// t = W32^I * o for I = 0..15 (W32 = exp(-2 pi i / 32)); 1 and -i need no multiply.
 template <int I>
-__device__ __forceinline__ float2 w16_mul(float2 o) {
+__device__ __forceinline__ float2 w32_mul(float2 o) {
   if constexpr (I == 0) {
     return o;
-  } else if constexpr (I == 4) {
+  } else if constexpr (I == 8) {
     return make_float2(o.y, -o.x);
   } else {
-    const float c = kCos16[I], s = kSin16[I];
+    const float c = kCos32[I], s = kSin32[I];
     return make_float2(c * o.x + s * o.y, c * o.y - s * o.x);
   }
 }
@@ -230,14 +238,14 @@ __device__ __forceinline__ float2 w16_mul(float2 o) {
 template <int R, int K>
 __device__ __forceinline__ void dft_combine(float2 *v, const float2 *e, const float2 *o) {
   if constexpr (K < R / 2) {
-    const float2 t = w16_mul<K * (16 / R)>(o[K]);
+    const float2 t = w32_mul<K * (32 / R)>(o[K]);
     v[K] = make_float2(e[K].x + t.x, e[K].y + t.y);
     v[K + R / 2] = make_float2(e[K].x - t.x, e[K].y - t.y);
     dft_combine<R, K + 1>(v, e, o);
   }
 }
 
-// In-register forward DFT of R = 2^k <= 16 points, natural order in and out
+// In-register forward DFT of R = 2^k <= 32 points, natural order in and out
 // (radix-2 decimation in time, fully unrolled).
 template <int R>
 __device__ __forceinline__ void dft_reg(float2 *v) {
@@ -263,9 +271,9 @@ __device__ __forceinline__ float wave_sum_bp(float v) {
   return v;
 }
 
-// LDS slot of element i: one pad slot per 16, so the strided writes of the
+// LDS slot of element i: one pad slot per 32, so the strided writes of the
 // early passes spread over the banks.
-__device__ __forceinline__ int bp_slot(int i) { return i + (i >> 4); }
+__device__ __forceinline__ int bp_slot(int i) { return i + (i >> 5); }
 
 // One Stockham pass of radix R over N points with completed sub-transform size
 // Ns: butterfly j reads x[j + r N/R], twiddles by W_{Ns R}^{(j mod Ns) r}, and
@@ -324,14 +332,21 @@ bandpass_kernel(const __half2 *__restrict__ beams, int N, int D,
     cnt[m] = 0;
   }
   __syncthreads();
+  // N = 2^m = 2^r 32^q: the 2^r-point pass first (it has the most butterflies
+  // and the least register pressure), then q radix-32 passes
+  int m = 0;
+  while ((1 << m) < N) ++m;
+  const int r = m % 5;
   int Ns = 1;
-  while (N / Ns >= 16) {
-    stockham_pass<16>(buf, N, Ns);
-    Ns *= 16;
+  if (r == 1) stockham_pass<2>(buf, N, Ns);
+  else if (r == 2) stockham_pass<4>(buf, N, Ns);
+  else if (r == 3) stockham_pass<8>(buf, N, Ns);
+  else if (r == 4) stockham_pass<16>(buf, N, Ns);
+  Ns <<= r;
+  while (Ns < N) {
+    stockham_pass<32>(buf, N, Ns);
+    Ns *= 32;
   }
-  if (N / Ns == 8) stockham_pass<8>(buf, N, Ns);
-  else if (N / Ns == 4) stockham_pass<4>(buf, N, Ns);
-  else if (N / Ns == 2) stockham_pass<2>(buf, N, Ns);
   // mean of |X|^2 over runs of D bins, NaN skipped
   const int lane = threadIdx.x & 63;
   for (int i0 = threadIdx.x - lane; i0 < N; i0 += kBpThreads) {
@@ -357,7 +372,7 @@ bandpass_kernel(const __half2 *__restrict__ beams, int N, int D,
 
 hipError_t launch_bandpass(const __half *beams, int rows, int N, int D, float *out, hipStream_t s) {
   if (N < 2 || N > kBpMaxN || (N & (N - 1)) || D < 1 || N % D) return hipErrorInvalidValue;
-  const size_t sm = (size_t)(N + N / 16) * sizeof(float2) + (size_t)(N / D) * (sizeof(float) + sizeof(int));
+  const size_t sm = (size_t)(N + N / 32) * sizeof(float2) + (size_t)(N / D) * (sizeof(float) + sizeof(int));
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(bandpass_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
