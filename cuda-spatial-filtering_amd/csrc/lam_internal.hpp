@@ -176,12 +176,11 @@ struct Skip {
   int ld = 0, off = 0, len = 0;
   float scale = 0.0f;
 };
-// Inference forward of the 2-layer MLP: layer 1 split over K into nparts FP32
-// partials (Zp: nparts * M * H floats), layer 2 with bias, ReLU and skip
-// (hidden widths other than 128, 256 or 512 go through Hbf: M * H BF16).
+// Inference forward of the 2-layer MLP: hidden layer relu(X W1^T + b1) as BF16
+// in Hbf (M * H), output Y = Hbf W2^T + b2 + skip (M * N FP32).
 hipError_t launch_mlp_infer(const uint16_t *X, int M, int K, const uint16_t *W1, const float *b1, int H,
-                            const uint16_t *W2, const float *b2, int N, int nparts, float *Zp, uint16_t *Hbf,
-                            float *Y, const Skip &sk, hipStream_t s);
+                            const uint16_t *W2, const float *b2, int N, uint16_t *Hbf, float *Y, const Skip &sk,
+                            hipStream_t s);
 hipError_t launch_dense_fwd(const uint16_t *X, int M, int K, const uint16_t *Wt, const float *b,
                             int N, float *Z, uint16_t *Hbf, const Skip &sk, hipStream_t s);
 hipError_t launch_gemm_bwd(bool at, const float *A, int lda, const uint16_t *B, int ldb, float *C,

@@ -179,7 +179,9 @@ hipError_t launch_spectra(const __half *beams, const void *table, int C, int nbe
                           hipStream_t s) {
   if (NT != kSpecFFT || T % kSpecFFT || !table) return hipErrorInvalidValue;
   int dev = 0, ncu = 256, per_cu = 0;
-  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+    ncu = 256;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, spectra_kernel, 64 * kSpecWaves, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
