@@ -1308,6 +1308,14 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
   }
   ctx->profiling = true;
   int rc = LAM_OK;
+  // With several buffers consecutive blocks overlap, and the eigen stage
+  // shares the GPU with the voltage kernels of neighbouring blocks: the
+  // choices are then judged by streaming time per block, over at least 16
+  // blocks so that the overlap reaches its steady state (measured on 384
+  // channels: 1024 samples per workgroup gives the shortest serial kernels
+  // but the longest streaming block, 0.71 against 0.61 ms at 2048-3072).
+  const bool stream = ctx->nbuf > 1;
+  const int stream_iters = std::max(iters, 16);
   // 1. time samples per voltage workgroup: more splits help the voltage
   //    kernels but multiply the partial visibilities that the eigen stage,
   //    the features and the accumulator read, so the whole sequence is timed
@@ -1319,10 +1327,11 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
       if (cand > Tr) continue;
       ctx->samples_per_wg = cand;
       double t = 0.0;
-      if ((rc = time_sequence(ctx, samples, scales, nblocks, iters,
-                              {LAM_K_CORRBEAM, LAM_K_CORR, LAM_K_BEAM2B, LAM_K_EIG, LAM_K_FEAT,
-                               LAM_K_MLP, LAM_K_SUBARRAY},
-                              t)))
+      if ((rc = stream ? time_stream(ctx, samples, scales, nblocks, stream_iters, t)
+                       : time_sequence(ctx, samples, scales, nblocks, iters,
+                                       {LAM_K_CORRBEAM, LAM_K_CORR, LAM_K_BEAM2B, LAM_K_EIG, LAM_K_FEAT,
+                                        LAM_K_MLP, LAM_K_SUBARRAY},
+                                       t)))
         break;
       if (t < best_t) {
         best_t = t;
@@ -1337,13 +1346,10 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
     ctx->eig_method = LAM_EIG_PLAIN;
     int best = ctx->eig_plain_threads;
     double best_t = 1e30;
-    // with several buffers the eigen stage shares the GPU with the voltage
-    // kernels of neighbouring blocks: judge by streaming time per block
-    const bool stream = ctx->nbuf > 1;
     for (int th : {256, 512, 1024}) {  // 64 and 128 are slower at every size measured
       ctx->eig_plain_threads = th;
       double t = 0.0;
-      if ((rc = stream ? time_stream(ctx, samples, scales, nblocks, iters, t)
+      if ((rc = stream ? time_stream(ctx, samples, scales, nblocks, stream_iters, t)
                        : time_sequence(ctx, samples, scales, nblocks, iters, {LAM_K_EIG}, t)))
         break;
       if (t < best_t) {

@@ -264,8 +264,9 @@ int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads);
  *   - time samples per voltage workgroup: automatic, 1024, 2048, 3072, 4096,
  *     8192, judged on the whole block sequence (more splits speed the voltage
  *     kernels but multiply the partial visibilities the eigen stage reads);
- *   - plain Jacobi workgroup width: 256, 512, 1024 (adaptive mode), judged on
- *     streaming time per block when the context has several buffers;
+ *   - plain Jacobi workgroup width: 256, 512, 1024 (adaptive mode);
+ *   with several buffers both are judged on streaming time per block (blocks
+ *   overlapping, at least 16 timed passes), with one buffer on kernel time;
  *   - warm-start refinement steps 1..3 when LAM_EIG_LEARNED / PREVIOUS is set
  *     (rejected problems fall back to the plain solver inside the timing, so a
  *     too-short refinement loses on time).
