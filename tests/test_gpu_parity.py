@@ -514,17 +514,19 @@ def test_adaptive_delays_and_permutation():
     assert rel_err(out["beams"][:, :, 2:], ref) < BEAM_TOL
 
 
-def test_autotune_leaves_results_exact():
+@pytest.mark.parametrize("num_buffers", [1, 3])
+def test_autotune_leaves_results_exact(num_buffers):
     """The autotuner times launch shapes on the caller's blocks; afterwards the
     pipeline's visibilities hold only the blocks executed after it (bit-exact)
     and the beams stay within tolerance.  36 receivers with a partial last
-    chunk and scales 0/1/2 exercise the packed-row loader."""
+    chunk and scales 0/1/2 exercise the packed-row loader.  With three buffers
+    the split size is chosen by streaming time (overlapping blocks)."""
     shape = syn.BlockShape(C=6, F=1, NT=64, R=36, P=2, RPP=36, NP=5, B=4)
     rng = np.random.default_rng(11)
     ring = rng.integers(-60, 61, shape.ring_shape(), dtype=np.int16).astype(np.int8)
     scales = rng.choice(np.array([0, 1, 1, 2], np.int16), shape.scale_shape())
     w16, w = syn.random_weights(shape, seed=4)
-    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM)
+    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM, num_buffers=num_buffers)
     p.set_weights(w16)
     d_ring = lam.DeviceArray.from_numpy(ring)
     d_scales = lam.DeviceArray.from_numpy(scales)
