@@ -157,20 +157,6 @@ int sync_all(lam_ctx *ctx) {
   return LAM_OK;
 }
 
-void pick_split(lam_ctx *ctx, int &spw, int &nsplit) {
-  const int T = ctx->T;
-  spw = ctx->samples_per_wg;
-  if (spw <= 0) {
-    const int target = std::max(1, (1024 + ctx->C - 1) / ctx->C);
-    spw = (T + target - 1) / target;
-  }
-  spw = ((spw + kChunk - 1) / kChunk) * kChunk;
-  const int min_spw = (((T + kMaxSplits - 1) / kMaxSplits + kChunk - 1) / kChunk) * kChunk;
-  if (spw < min_spw) spw = min_spw;
-  if (spw > ((T + kChunk - 1) / kChunk) * kChunk) spw = ((T + kChunk - 1) / kChunk) * kChunk;
-  nsplit = (T + spw - 1) / spw;
-}
-
 VoltParams base_volt(lam_ctx *ctx, const void *samples, const void *scales) {
   VoltParams p{};
   p.samples = static_cast<const int8_t *>(samples);
@@ -185,7 +171,7 @@ VoltParams base_volt(lam_ctx *ctx, const void *samples, const void *scales) {
   p.nt_log2 = -1;
   for (int l = 0; l < 30; ++l)
     if ((1 << l) == ctx->NT) p.nt_log2 = l;
-  pick_split(ctx, p.samples_per_wg, p.nsplit);
+  pick_split(ctx->T, ctx->C, ctx->samples_per_wg, p.samples_per_wg, p.nsplit);
   p.xcd_map = (ctx->C % 8 == 0) ? 1 : 0;
   p.err = ctx->d_err;
   p.NK = ctx->NK;
@@ -743,7 +729,9 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
   if (sh->nr_receivers < 1 || sh->nr_receivers > kMaxReceivers) return LAM_ERR_UNSUPPORTED;
   if (num_buffers < 1 || num_buffers > 4) return LAM_ERR_ARG;
   const long T = (long)sh->nr_packets_for_correlation * sh->nr_time_steps_per_packet;
-  if (T > 65536) return LAM_ERR_UNSUPPORTED;  // int32 per-block accumulators stay exact
+  // at most kMaxSplits splits of at most kMaxSamplesPerSplit samples each, so
+  // every int32 partial sum stays exact
+  if (T > 65536) return LAM_ERR_UNSUPPORTED;
   const int Bout = mode == LAM_MODE_ADAPTIVE ? 2 * sh->nr_beams : sh->nr_beams;
   if (Bout > 64 || (mode != LAM_MODE_ADAPTIVE && sh->nr_beams > 32)) return LAM_ERR_UNSUPPORTED;
 

@@ -35,6 +35,28 @@ __device__ __forceinline__ void weight_operands(float v, int wprec, _Float16 &hi
 constexpr int kChunk = 128;
 constexpr int kMaxReceivers = 64;  // 4 antenna groups of 16
 constexpr int kMaxSplits = 16;     // time splits per channel (partial visibility buffers)
+// Samples per workgroup at most: an int32 partial sum of one split stays exact
+// (|Re a Re b + Im a Im b| <= 2 * 128^2 per sample, 2^15 * 2^15 = 2^30 < 2^31).
+constexpr int kMaxSamplesPerSplit = 32768;
+
+// Time samples per voltage workgroup (a multiple of kChunk) and the number of
+// splits per channel for T samples and C channels.  requested <= 0 picks about
+// 1024 workgroups in total; the result keeps nsplit <= kMaxSplits and every
+// split within kMaxSamplesPerSplit.
+inline void pick_split(int T, int C, int requested, int &spw, int &nsplit) {
+  const int Tr = ((T + kChunk - 1) / kChunk) * kChunk;
+  spw = requested;
+  if (spw <= 0) {
+    const int target = (1024 + C - 1) / C > 1 ? (1024 + C - 1) / C : 1;
+    spw = (T + target - 1) / target;
+  }
+  spw = ((spw + kChunk - 1) / kChunk) * kChunk;
+  const int min_spw = (((T + kMaxSplits - 1) / kMaxSplits + kChunk - 1) / kChunk) * kChunk;
+  if (spw < min_spw) spw = min_spw;
+  if (spw > Tr) spw = Tr;
+  if (spw > kMaxSamplesPerSplit) spw = kMaxSamplesPerSplit;
+  nsplit = (T + spw - 1) / spw;
+}
 #ifndef LAM_EIG_THREADS
 #define LAM_EIG_THREADS 256
 #endif

@@ -409,6 +409,21 @@ int lam_eigh_batched(const float *d_A, int n, int nmat, int whiten, int max_swee
 int lam_eigh_warm(const float *d_A, const float *d_prop, int n, int nmat, int warm_vectors, int k,
                   int steps, float tol, float *d_evals, float *d_evecs, float *d_ratio, void *stream);
 
+/* Correlator of one device-resident block without a pipeline context: the
+ * corr-beam pipeline's TCC launch plus trim and accumulate
+ * (lambda_corr_beam_only_gpu_pipeline.hpp:314-316, src/spatial.cu:136-170).
+ * d_samples / d_scales are a ring block and its scales (layouts above) for
+ * `shape`; window 0 reads ring slots 0..NP-1 (spatial.cuh:254-264), window 1
+ * the adaptive window from slot 1 (zero delays, spatial.cuh:1431-1436).  The
+ * block's exact integer visibilities are ADDED into d_vis, int64
+ * [C][R(R+1)/2][P][P][2] (raw sums, no gains), so a caller accumulates blocks
+ * by calling again and clears d_vis itself.  d_flags (device int32, may be
+ * NULL) is OR-ed with 1 when a scaled sample left the int8 range.  Allocates
+ * its per-split partial sums (C * R(R+1)/2 * 32 B per time split) on every
+ * call and returns after the stream has finished the block. */
+int lam_correlate(const lam_shape *shape, int window, const void *d_samples, const void *d_scales,
+                  int64_t *d_vis, int32_t *d_flags, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
