@@ -8,7 +8,9 @@
 // are the only memory besides the caller's inputs and outputs.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/lam.h"
 #include "lam_internal.hpp"
@@ -90,4 +92,72 @@ extern "C" int lam_correlate(const lam_shape *shape, int window, const void *d_s
   const hipError_t es = hipStreamSynchronize(s);
   (void)hipFree(buf);
   return (e == hipSuccess && es == hipSuccess) ? LAM_OK : LAM_ERR_HIP;
+}
+
+// ---- allocation-free variant ----
+// The per-split partial sums live in a file-scope device array, one copy per
+// GPU, held until the process ends.  Its first 256 bytes take the range flag
+// when the caller passes none; the partials follow.  Every call on a GPU uses
+// the same array, so calls that overlap on one GPU (on different streams or
+// host threads) race; calls ordered on one stream do not.
+namespace {
+
+constexpr size_t kScratchBytes = size_t(32) << 20;  // 32 MiB per GPU
+constexpr size_t kScratchFlagBytes = 256;
+static __device__ __align__(256) int scratch[kScratchBytes / sizeof(int)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` on each device
+
+// Device address of this device's `scratch` (looked up once per device, with
+// that device current; the array's host-side name is not a device address).
+int scratch_address(int dev, char **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return LAM_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = static_cast<char *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return LAM_OK;
+}
+
+}  // namespace
+
+extern "C" int lam_correlate_noalloc(const lam_shape *shape, int window, const void *d_samples,
+                                     const void *d_scales, int64_t *d_vis, int32_t *d_flags, void *stream) {
+  if (int r = check_corr_shape(shape)) return r;
+  if ((window != 0 && window != 1) || !d_samples || !d_scales || !d_vis) return LAM_ERR_ARG;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return LAM_ERR_NO_DEVICE;
+  if (dev < 0 || dev >= kMaxDevices) return LAM_ERR_UNSUPPORTED;
+  char *base = nullptr;
+  if (int r = scratch_address(dev, &base)) return r;
+  const int C = shape->nr_channels, R = shape->nr_receivers;
+  const int T = shape->nr_packets_for_correlation * shape->nr_time_steps_per_packet;
+  const size_t per_split = (size_t)C * R * (R + 1) / 2 * 8 * sizeof(int);
+  const size_t avail = kScratchBytes - kScratchFlagBytes;
+  int spw = 0, nsplit = 0;
+  pick_split(T, C, 0, spw, nsplit);
+  if ((size_t)nsplit * per_split > avail) {
+    // fewer, longer splits; give up when even the longest allowed split does not fit
+    const size_t max_split = avail / per_split;
+    if (max_split >= 1) pick_split(T, C, (int)((T + max_split - 1) / max_split), spw, nsplit);
+    if (max_split < 1 || (size_t)nsplit * per_split > avail) {
+      std::fprintf(stderr,
+                   "lam_correlate_noalloc: %d channels x %d receivers x %d samples need %zu bytes of partial "
+                   "sums, more than the %zu-byte static scratch\n",
+                   C, R, T, (size_t)nsplit * per_split, avail);
+      return LAM_ERR_UNSUPPORTED;
+    }
+  }
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const VoltParams p = corr_params(shape, window, d_samples, d_scales, spw, nsplit);
+  int *err = d_flags ? reinterpret_cast<int *>(d_flags) : reinterpret_cast<int *>(base);
+  const hipError_t e = enqueue_correlate(p, reinterpret_cast<int *>(base + kScratchFlagBytes), err, d_vis, s);
+  return e == hipSuccess ? LAM_OK : LAM_ERR_HIP;
 }

@@ -135,6 +135,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_learn_params.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
     L.lam_learn_grad.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, vp]
     L.lam_correlate.argtypes = [ctypes.POINTER(LamShape), ctypes.c_int, vp, vp, vp, vp, vp]
+    L.lam_correlate_noalloc.argtypes = [ctypes.POINTER(LamShape), ctypes.c_int, vp, vp, vp, vp, vp]
     L.lam_synth_block.argtypes = [ctypes.POINTER(LamShape), ctypes.c_uint64, i32, ctypes.c_float, i32,
                                   vp, vp, vp, vp, vp, vp]
     _lib = L
@@ -646,3 +647,15 @@ def correlate(shape, window, samples_dev, scales_dev, vis_dev, flags_dev=None, s
                          _ptr(flags_dev), stream)
     if rc != LAM_OK:
         raise LamError(rc, "lam_correlate failed")
+
+
+def correlate_noalloc(shape, window, samples_dev, scales_dev, vis_dev, flags_dev=None, stream=None):
+    """lam_correlate_noalloc: as correlate(), with the library's static scratch
+    (no device allocation; enqueues on `stream` and returns)."""
+    L = load_library()
+    sh = LamShape(shape.C, shape.F, shape.NT, shape.R, shape.P, shape.RPP, shape.NP, shape.B,
+                  shape.R_pad, shape.blocks_to_accumulate)
+    rc = L.lam_correlate_noalloc(ctypes.byref(sh), int(window), _ptr(samples_dev), _ptr(scales_dev),
+                                 _ptr(vis_dev), _ptr(flags_dev), stream)
+    if rc != LAM_OK:
+        raise LamError(rc, "lam_correlate_noalloc failed")

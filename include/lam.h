@@ -424,6 +424,21 @@ int lam_eigh_warm(const float *d_A, const float *d_prop, int n, int nmat, int wa
 int lam_correlate(const lam_shape *shape, int window, const void *d_samples, const void *d_scales,
                   int64_t *d_vis, int32_t *d_flags, void *stream);
 
+/* lam_correlate without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as lam_correlate, but the per-split partial sums live in a static
+ * 32 MiB device array per GPU (held until the process ends), the call only
+ * enqueues on `stream` and returns without synchronising, and it calls no
+ * hipMalloc / hipFree.  Fewer, longer time splits are used when the automatic
+ * choice would not fit; a block needing more than 32 MiB for one split per
+ * channel (C * R(R+1)/2 * 32 B: e.g. more than 504 channels at 64 receivers)
+ * is rejected with LAM_ERR_UNSUPPORTED and a message on stderr.  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU (different
+ * streams or threads) race; calls ordered on one stream are safe. */
+int lam_correlate_noalloc(const lam_shape *shape, int window, const void *d_samples, const void *d_scales,
+                          int64_t *d_vis, int32_t *d_flags, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
