@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <initializer_list>
 #include <numeric>
 #include <random>
@@ -46,6 +47,12 @@ struct Buf {
   const void *graph_samples = nullptr, *graph_scales = nullptr;
   hipEvent_t done = nullptr;
   hipEvent_t eig_done = nullptr;  // the eigen stage of this buffer's latest block
+  // range flag of this buffer's blocks: cleared before each block, copied to
+  // pinned host memory after it, read once the block's `done` event has passed
+  int *d_err = nullptr;
+  int *h_err = nullptr;
+  bool err_pending = false;
+  long long err_block = 0;  // index of the block whose flag is pending
 };
 
 
@@ -97,7 +104,6 @@ struct lam_ctx {
   void *d_spec_table = nullptr;  // DFT operand fragments of the spectra kernel
   int detect = 0, shrink = 0, max_sweeps = 12;
   float delta = 3.0f, tol = 1e-6f;
-  int *d_err = nullptr;
   float *d_dump_f32 = nullptr;
   long long *d_dump_i64 = nullptr;
   hipEvent_t reset_done = nullptr;
@@ -125,6 +131,9 @@ struct lam_ctx {
   int *d_sub_counts = nullptr, *d_sub_sweeps = nullptr, *d_sub_sizes = nullptr;
   Learn lr;
   int last_buf = -1;
+  long long blocks_run = 0;     // lam_execute calls that ran a block
+  long long graph_replays = 0;  // hipGraphLaunch calls
+  bool graph_warned = false;
   double prof_ms[LAM_K_COUNT] = {0};
   int prof_n[LAM_K_COUNT] = {0};
   std::string err;
@@ -157,7 +166,7 @@ int sync_all(lam_ctx *ctx) {
   return LAM_OK;
 }
 
-VoltParams base_volt(lam_ctx *ctx, const void *samples, const void *scales) {
+VoltParams base_volt(lam_ctx *ctx, const Buf &b, const void *samples, const void *scales) {
   VoltParams p{};
   p.samples = static_cast<const int8_t *>(samples);
   p.scales = static_cast<const int16_t *>(scales);
@@ -173,7 +182,7 @@ VoltParams base_volt(lam_ctx *ctx, const void *samples, const void *scales) {
     if ((1 << l) == ctx->NT) p.nt_log2 = l;
   pick_split(ctx->T, ctx->C, ctx->samples_per_wg, p.samples_per_wg, p.nsplit);
   p.xcd_map = (ctx->C % 8 == 0) ? 1 : 0;
-  p.err = ctx->d_err;
+  p.err = b.d_err;
   p.NK = ctx->NK;
   p.wprec = ctx->wprec;
   return p;
@@ -235,7 +244,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
   const bool atomic_acc = ctx->nbuf > 1;
   if (ctx->mode == LAM_MODE_PROJECTION) {
     if (phase != 0) return LAM_OK;
-    VoltParams p = base_volt(ctx, samples, scales);
+    VoltParams p = base_volt(ctx, b, samples, scales);
     p.window_base = 0;  // no alignment step: ring slots 0..NP-1 as packetToPadding reads them
     p.vis_part = b.d_vis_part;
     {
@@ -249,7 +258,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
   }
   if (ctx->mode == LAM_MODE_CORR_BEAM || ctx->mode == LAM_MODE_LAMBDA_GPU) {
     if (phase != 0) return LAM_OK;
-    VoltParams p = base_volt(ctx, samples, scales);
+    VoltParams p = base_volt(ctx, b, samples, scales);
     if (ctx->mode == LAM_MODE_CORR_BEAM) {
       p.window_base = 0;  // ring slots 0..NP-1 (spatial.cuh:254-264)
     } else {
@@ -282,7 +291,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
     return LAM_OK;
   }
   // adaptive: correlate -> eig/null -> 2B beams
-  VoltParams p = base_volt(ctx, samples, scales);
+  VoltParams p = base_volt(ctx, b, samples, scales);
   p.window_base = ctx->NT;  // slot 1 + delay (spatial.cuh:1431-1436)
   p.delays = ctx->delays_set ? ctx->d_delays : nullptr;
   p.recv_perm = ctx->perm_set ? ctx->d_recv_perm : nullptr;
@@ -430,6 +439,19 @@ void destroy_graphs(Buf &b) {
     }
 }
 
+// A failed capture or instantiation turns graphs off for the context and the
+// block runs eagerly, as the reference falls back after a failed capture
+// (lambda_corr_beam_only_gpu_pipeline.hpp:561-586); the warning is printed
+// once per context and lam_graph_replays() stops counting.
+void graph_fallback(lam_ctx *ctx, const char *what, hipError_t e) {
+  (void)hipGetLastError();
+  ctx->use_graphs = false;
+  if (!ctx->graph_warned) {
+    std::fprintf(stderr, "lam: hipGraph %s failed (%s); the pipeline runs eagerly\n", what, hipGetErrorString(e));
+    ctx->graph_warned = true;
+  }
+}
+
 // One phase, replayed from a graph captured on first use when graphs are on.
 int run_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales, int phase, bool graphs) {
   if (!graphs) return enqueue_phase(ctx, b, samples, scales, phase);
@@ -441,23 +463,20 @@ int run_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales, int
     const hipError_t ec = hipStreamEndCapture(b.stream, &g);
     if (rc != LAM_OK || ec != hipSuccess || !g) {
       if (g) (void)hipGraphDestroy(g);
-      (void)hipGetLastError();
-      // capture failure falls back to eager launches
-      // (lambda_corr_beam_only_gpu_pipeline.hpp:346-383, 561-586)
-      ctx->use_graphs = false;
+      graph_fallback(ctx, "capture", ec != hipSuccess ? ec : hipErrorUnknown);
       return enqueue_phase(ctx, b, samples, scales, phase);
     }
     hipGraphExec_t ex = nullptr;
     const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ei != hipSuccess || !ex) {
-      (void)hipGetLastError();
-      ctx->use_graphs = false;
+      graph_fallback(ctx, "instantiation", ei != hipSuccess ? ei : hipErrorUnknown);
       return enqueue_phase(ctx, b, samples, scales, phase);
     }
     gx = ex;
   }
   HIPC(ctx, hipGraphLaunch(gx, b.stream));
+  ctx->graph_replays += 1;
   return LAM_OK;
 }
 
@@ -707,6 +726,72 @@ int time_stream(lam_ctx *ctx, const void *const *smp, const void *const *scl, in
   return rc;
 }
 
+// Choice among `cands` by time per block.  The candidates are measured in
+// `rounds` interleaved rounds (the order rotated every round) so that slow
+// drifts of the GPU clock favour none of them; each is judged by its median.
+// The incumbent (the setting in force before tuning) is kept unless the best
+// median beats it by more than the incumbent's own spread over the rounds.
+int pick_by_time(const std::vector<int> &cands, int incumbent, int rounds, const std::function<void(int)> &apply,
+                 const std::function<int(double &)> &measure, int &chosen) {
+  const size_t n = cands.size();
+  std::vector<std::vector<double>> t(n);
+  for (int r = 0; r < rounds; ++r)
+    for (size_t j = 0; j < n; ++j) {
+      const size_t i = (j + (size_t)r) % n;
+      apply(cands[i]);
+      double v = 0.0;
+      if (int rc = measure(v)) return rc;
+      t[i].push_back(v);
+    }
+  auto median = [](std::vector<double> v) {
+    std::sort(v.begin(), v.end());
+    const size_t m = v.size() / 2;
+    return v.size() % 2 ? v[m] : 0.5 * (v[m - 1] + v[m]);
+  };
+  size_t best = 0, inc = n;
+  for (size_t i = 0; i < n; ++i) {
+    if (median(t[i]) < median(t[best])) best = i;
+    if (cands[i] == incumbent) inc = i;
+  }
+  chosen = cands[best];
+  if (inc < n && inc != best) {
+    const double spread = *std::max_element(t[inc].begin(), t[inc].end()) -
+                          *std::min_element(t[inc].begin(), t[inc].end());
+    if (median(t[best]) >= median(t[inc]) - spread) chosen = incumbent;
+  }
+  apply(chosen);
+  return LAM_OK;
+}
+
+// Reports a range error of a completed block: each block's flag is copied to
+// pinned host memory at its end; the flag is read once the block's `done`
+// event has passed (wait = true: after synchronising every stream).
+int check_range(lam_ctx *ctx, bool wait) {
+  long long bad = -1;
+  for (auto &b : ctx->bufs) {
+    if (!b.err_pending) continue;
+    if (!wait) {
+      const hipError_t q = hipEventQuery(b.done);
+      if (q == hipErrorNotReady) {
+        (void)hipGetLastError();
+        continue;
+      }
+      if (q != hipSuccess) return fail(ctx, LAM_ERR_HIP, "hipEventQuery: %s", hipGetErrorString(q));
+    }
+    b.err_pending = false;
+    if (*b.h_err) {
+      *b.h_err = 0;
+      if (bad < 0 || b.err_block < bad) bad = b.err_block;
+    }
+  }
+  if (bad >= 0)
+    return fail(ctx, LAM_ERR_RANGE,
+                "block %lld: a scaled sample left the int8 range of the exact correlator; its visibilities "
+                "and beams are wrong",
+                bad);
+  return LAM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -766,7 +851,10 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
   ctx->samples_bytes = (size_t)ctx->C * (ctx->NP + 2) * ctx->F * ctx->NT * ctx->RPP * 4;
   ctx->scales_bytes = (size_t)ctx->C * (ctx->NP + 2) * ctx->R * 2 * 2;
   ctx->beams_bytes = (size_t)ctx->C * 2 * ctx->Bout * ctx->T * 4;
-  if (const char *e = std::getenv("LAM_DISABLE_GRAPH")) ctx->use_graphs = !(e[0] == '1');
+  // SPATIAL_DISABLE_CUDA_GRAPH set (any value) forces eager launches, as in the
+  // reference (lambda_corr_beam_only_gpu_pipeline.hpp:561); LAM_DISABLE_GRAPH=1 too
+  if (std::getenv("SPATIAL_DISABLE_CUDA_GRAPH") != nullptr) ctx->use_graphs = false;
+  if (const char *e = std::getenv("LAM_DISABLE_GRAPH")) ctx->use_graphs = ctx->use_graphs && e[0] != '1';
 
   int rc = LAM_OK;
   auto chk = [&](int r) {
@@ -785,6 +873,11 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
     if (mode == LAM_MODE_ADAPTIVE && ctx->NT > 10)
       chk(dalloc(ctx, &b.d_spectra, (size_t)ctx->C * 2 * ctx->Bout * (ctx->NT - 10) * sizeof(float)));
     chk(dalloc(ctx, &b.d_vis_part, (size_t)kMaxSplits * vis_elems * sizeof(int)));
+    chk(dalloc(ctx, &b.d_err, sizeof(int)));
+    if (hipHostMalloc(reinterpret_cast<void **>(&b.h_err), sizeof(int), hipHostMallocDefault) != hipSuccess)
+      rc = LAM_ERR_HIP;
+    else
+      *b.h_err = 0;
     if (mode == LAM_MODE_LAMBDA_GPU) {  // eigenpairs of the four pol pairs
       chk(dalloc(ctx, &b.d_evals, (size_t)ctx->C * 4 * ctx->R * sizeof(float)));
       chk(dalloc(ctx, &b.d_evecs, (size_t)ctx->C * 4 * ctx->R * ctx->R * 2 * sizeof(float)));
@@ -822,7 +915,6 @@ int lam_create(const lam_shape *sh, int mode, int num_buffers, lam_ctx **out) {
         hipMemcpy(ctx->d_spec_table, tab.data(), spectra_table_bytes(), hipMemcpyHostToDevice) != hipSuccess)
       rc = LAM_ERR_HIP;
   }
-  chk(dalloc(ctx, &ctx->d_err, sizeof(int)));
   chk(dalloc(ctx, &ctx->d_dump_f32, vis_elems * sizeof(float)));
   chk(dalloc(ctx, &ctx->d_dump_i64, vis_elems * sizeof(long long)));
   if (hipEventCreateWithFlags(&ctx->reset_done, hipEventDisableTiming) != hipSuccess) rc = LAM_ERR_HIP;
@@ -862,6 +954,8 @@ int lam_destroy(lam_ctx *ctx) {
     (void)hipFree(b.d_beams);
     (void)hipFree(b.d_spectra);
     (void)hipFree(b.d_vis_part);
+    (void)hipFree(b.d_err);
+    if (b.h_err) (void)hipHostFree(b.h_err);
     (void)hipFree(b.d_wop2);
     (void)hipFree(b.d_evals);
     (void)hipFree(b.d_evecs);
@@ -885,7 +979,6 @@ int lam_destroy(lam_ctx *ctx) {
   (void)hipFree(ctx->d_proj_acc);
   (void)hipFree(ctx->d_proj_evals);
   (void)hipFree(ctx->d_proj_evecs);
-  (void)hipFree(ctx->d_err);
   (void)hipFree(ctx->d_dump_f32);
   (void)hipFree(ctx->d_dump_i64);
   free_learn(ctx->lr);
@@ -1092,8 +1185,7 @@ void *lam_next_stream(lam_ctx *ctx) { return ctx ? (void *)ctx->bufs[ctx->cur].s
 
 int lam_blocks_accumulated(const lam_ctx *ctx) { return ctx ? ctx->blocks_acc : 0; }
 
-int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
-  if (!ctx || !io || !io->samples || !io->scales) return LAM_ERR_ARG;
+static int execute_block(lam_ctx *ctx, const lam_block_io *io, bool &release_queued) {
   if (io->spectra_dst && !ctx->spectra && !ctx->bandpass)
     return fail(ctx, LAM_ERR_ARG, "spectra_dst given but spectra are off");
   Buf &b = ctx->bufs[ctx->cur];
@@ -1103,15 +1195,21 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
     // H2D of the block, then release the host buffer (pipeline/common.hpp:517-528)
     HIPC(ctx, hipMemcpyAsync(b.d_samples, io->samples, ctx->samples_bytes, hipMemcpyHostToDevice, s));
     HIPC(ctx, hipMemcpyAsync(b.d_scales, io->scales, ctx->scales_bytes, hipMemcpyHostToDevice, s));
-    if (io->release_fn) HIPC(ctx, hipLaunchHostFunc(s, io->release_fn, io->release_user));
+    if (io->release_fn) {
+      HIPC(ctx, hipLaunchHostFunc(s, io->release_fn, io->release_user));
+      release_queued = true;
+    }
     smp = b.d_samples;
     scl = b.d_scales;
   }
   // never accumulate into a visibility buffer that a dump has not reset yet
   HIPC(ctx, hipStreamWaitEvent(s, ctx->reset_done, 0));
+  HIPC(ctx, hipMemsetAsync(b.d_err, 0, sizeof(int), s));
   if (int r = run_block(ctx, b, smp, scl)) return r;
-  if (io->inputs_on_device && io->release_fn)
+  if (io->inputs_on_device && io->release_fn) {
     HIPC(ctx, hipLaunchHostFunc(s, io->release_fn, io->release_user));
+    release_queued = true;
+  }
   if (io->beams_dst) {
     HIPC(ctx, hipMemcpyAsync(io->beams_dst, b.d_beams, ctx->beams_bytes, hipMemcpyDefault, s));
     if (io->beams_done_fn) HIPC(ctx, hipLaunchHostFunc(s, io->beams_done_fn, io->beams_done_user));
@@ -1156,7 +1254,10 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
       if (io->spectra_done_fn) HIPC(ctx, hipLaunchHostFunc(s, io->spectra_done_fn, io->spectra_done_user));
     }
   }
+  HIPC(ctx, hipMemcpyAsync(b.h_err, b.d_err, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPC(ctx, hipEventRecord(b.done, s));
+  b.err_pending = true;
+  b.err_block = ctx->blocks_run++;
   if (ctx->mode == LAM_MODE_PROJECTION) ++ctx->proj_runs;
   ctx->last_buf = ctx->cur;
   ctx->blocks_acc += 1;
@@ -1164,7 +1265,28 @@ int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
   return LAM_OK;
 }
 
+int lam_execute(lam_ctx *ctx, const lam_block_io *io) {
+  if (!ctx || !io) return LAM_ERR_ARG;
+  bool release_queued = false;
+  int rc = LAM_OK;
+  if (!io->samples || !io->scales) rc = fail(ctx, LAM_ERR_ARG, "execute: samples and scales are required");
+  // a range error of an earlier, completed block fails this call
+  if (rc == LAM_OK) rc = check_range(ctx, false);
+  if (rc == LAM_OK) rc = execute_block(ctx, io, release_queued);
+  if (rc != LAM_OK && io->release_fn && !release_queued) {
+    // the host input is released exactly once whether or not the block ran:
+    // once nothing on the stream reads it any more
+    Buf &b = ctx->bufs[ctx->cur];
+    (void)hipStreamSynchronize(b.stream);
+    (void)hipGetLastError();
+    io->release_fn(io->release_user);
+  }
+  return rc;
+}
+
 static int dump_common(lam_ctx *ctx, float *f32, int64_t *i64, lam_host_fn fn, void *user) {
+  // a block with a range error reached this accumulator: fail the dump
+  if (int r = check_range(ctx, false)) return r;
   hipStream_t s = ctx->bufs[0].stream;
   for (auto &b : ctx->bufs) HIPC(ctx, hipStreamWaitEvent(s, b.done, 0));
   const size_t elems = (size_t)ctx->C * ctx->NBL * 8;
@@ -1196,6 +1318,7 @@ int lam_dump_projection(lam_ctx *ctx, float *evals_dst, float *evecs_dst, lam_ho
   if (!ctx) return LAM_ERR_ARG;
   if (ctx->mode != LAM_MODE_PROJECTION) return fail(ctx, LAM_ERR_STATE, "no projection accumulator in this mode");
   if (ctx->proj_runs == 0) return fail(ctx, LAM_ERR_STATE, "no block integrated since the last projection dump");
+  if (int r = check_range(ctx, false)) return r;
   hipStream_t s = ctx->bufs[0].stream;
   for (auto &b : ctx->bufs) HIPC(ctx, hipStreamWaitEvent(s, b.done, 0));
   const size_t nprob = (size_t)ctx->C * 4;
@@ -1225,15 +1348,10 @@ int lam_dump_visibilities_i64(lam_ctx *ctx, int64_t *dst, lam_host_fn fn, void *
 int lam_sync(lam_ctx *ctx) {
   if (!ctx) return LAM_ERR_ARG;
   if (int r = sync_all(ctx)) return r;
-  int flag = 0;
-  HIPC(ctx, hipMemcpy(&flag, ctx->d_err, sizeof(int), hipMemcpyDeviceToHost));
-  if (flag) {
-    HIPC(ctx, hipMemset(ctx->d_err, 0, sizeof(int)));
-    return fail(ctx, LAM_ERR_RANGE,
-                "a scaled sample left the int8 range of the exact correlator (flag %d)", flag);
-  }
-  return LAM_OK;
+  return check_range(ctx, true);
 }
+
+long long lam_graph_replays(const lam_ctx *ctx) { return ctx ? ctx->graph_replays : -1; }
 
 void *lam_device_alloc(size_t bytes) {
   void *p = nullptr;
@@ -1295,6 +1413,10 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
     ctx->prof_n[k] += d1[k];
   }
   ctx->profiling = true;
+  // tuning blocks are no training examples: capture is suspended meanwhile
+  Learn &L = ctx->lr;
+  const int cap_max_was = L.cap_max;
+  L.cap_max = L.cap_count;
   int rc = LAM_OK;
   // With several buffers consecutive blocks overlap, and the eigen stage
   // shares the GPU with the voltage kernels of neighbouring blocks: the
@@ -1304,48 +1426,45 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
   // but the longest streaming block, 0.71 against 0.61 ms at 2048-3072).
   const bool stream = ctx->nbuf > 1;
   const int stream_iters = std::max(iters, 16);
+  constexpr int kRounds = 3;
+  // warm-up: after idle the GPU runs the first ~50 blocks slower
+  {
+    double t = 0.0;
+    rc = time_stream(ctx, samples, scales, nblocks, std::max(1, (48 + nblocks - 1) / nblocks), t);
+  }
   // 1. time samples per voltage workgroup: more splits help the voltage
   //    kernels but multiply the partial visibilities that the eigen stage,
   //    the features and the accumulator read, so the whole sequence is timed
-  {
+  if (rc == LAM_OK) {
     const int Tr = (ctx->T + kChunk - 1) / kChunk * kChunk;
-    int best = ctx->samples_per_wg;
-    double best_t = 1e30;
-    for (int cand : {0, 1024, 2048, 3072, 4096, 8192}) {
-      if (cand > Tr) continue;
-      ctx->samples_per_wg = cand;
-      double t = 0.0;
-      if ((rc = stream ? time_stream(ctx, samples, scales, nblocks, stream_iters, t)
-                       : time_sequence(ctx, samples, scales, nblocks, iters,
-                                       {LAM_K_CORRBEAM, LAM_K_CORR, LAM_K_BEAM2B, LAM_K_EIG, LAM_K_FEAT,
-                                        LAM_K_MLP, LAM_K_SUBARRAY},
-                                       t)))
-        break;
-      if (t < best_t) {
-        best_t = t;
-        best = cand;
-      }
-    }
-    ctx->samples_per_wg = best;
+    std::vector<int> cands;
+    for (int cand : {0, 1024, 2048, 3072, 4096, 8192})
+      if (cand <= Tr) cands.push_back(cand);
+    int chosen = ctx->samples_per_wg;
+    rc = pick_by_time(
+        cands, ctx->samples_per_wg, kRounds, [&](int v) { ctx->samples_per_wg = v; },
+        [&](double &t) {
+          return stream ? time_stream(ctx, samples, scales, nblocks, stream_iters, t)
+                        : time_sequence(ctx, samples, scales, nblocks, iters,
+                                        {LAM_K_CORRBEAM, LAM_K_CORR, LAM_K_BEAM2B, LAM_K_EIG, LAM_K_FEAT, LAM_K_MLP,
+                                         LAM_K_SUBARRAY},
+                                        t);
+        },
+        chosen);
   }
   // 2. plain eigen solve: workgroup width
   if (rc == LAM_OK && ctx->mode == LAM_MODE_ADAPTIVE) {
     const int method = ctx->eig_method;
     ctx->eig_method = LAM_EIG_PLAIN;
-    int best = ctx->eig_plain_threads;
-    double best_t = 1e30;
-    for (int th : {256, 512, 1024}) {  // 64 and 128 are slower at every size measured
-      ctx->eig_plain_threads = th;
-      double t = 0.0;
-      if ((rc = stream ? time_stream(ctx, samples, scales, nblocks, stream_iters, t)
-                       : time_sequence(ctx, samples, scales, nblocks, iters, {LAM_K_EIG}, t)))
-        break;
-      if (t < best_t) {
-        best_t = t;
-        best = th;
-      }
-    }
-    ctx->eig_plain_threads = best;
+    int chosen = ctx->eig_plain_threads;
+    rc = pick_by_time(
+        {256, 512, 1024}, ctx->eig_plain_threads, kRounds,  // 64 and 128 are slower at every size measured
+        [&](int v) { ctx->eig_plain_threads = v; },
+        [&](double &t) {
+          return stream ? time_stream(ctx, samples, scales, nblocks, stream_iters, t)
+                        : time_sequence(ctx, samples, scales, nblocks, iters, {LAM_K_EIG}, t);
+        },
+        chosen);
     ctx->eig_method = method;
     // 3. warm start: refinement steps (rejections fall back to the plain solver
     //    inside the warm launch and show up in its time).  Judged by the
@@ -1353,21 +1472,15 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
     //    count moves it by ~10 us, which the streaming time per block (bound
     //    by the voltage passes of neighbouring blocks) does not resolve.
     if (rc == LAM_OK && method != LAM_EIG_PLAIN && ctx->lr.ready) {
-      int bst = ctx->refine_sweeps;
-      double bt = 1e30;
-      for (int st : {1, 2, 3}) {
-        ctx->refine_sweeps = st;
-        double t = 0.0;
-        if ((rc = time_sequence(ctx, samples, scales, nblocks, iters, {LAM_K_EIG, LAM_K_FEAT, LAM_K_MLP}, t)))
-          break;
-        if (t < bt) {
-          bt = t;
-          bst = st;
-        }
-      }
-      ctx->refine_sweeps = bst;
+      rc = pick_by_time(
+          {1, 2, 3}, ctx->refine_sweeps, kRounds, [&](int v) { ctx->refine_sweeps = v; },
+          [&](double &t) {
+            return time_sequence(ctx, samples, scales, nblocks, iters, {LAM_K_EIG, LAM_K_FEAT, LAM_K_MLP}, t);
+          },
+          chosen);
     }
   }
+  L.cap_max = cap_max_was;
   ctx->profiling = prof_was;
   ctx->graphs_dirty = true;
   // the tuning blocks must not reach the visibility accumulator
@@ -1379,6 +1492,20 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
         hipStreamSynchronize(st) != hipSuccess)
       rc = fail(ctx, LAM_ERR_HIP, "autotune: clearing the accumulator failed");
     ctx->proj_runs = 0;
+  }
+  // range errors of the tuning blocks (run outside lam_execute, whose flag
+  // copies they do not have)
+  for (auto &b : ctx->bufs) {
+    int flag = 0;
+    if (hipMemcpy(&flag, b.d_err, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+      if (rc == LAM_OK) rc = fail(ctx, LAM_ERR_HIP, "autotune: reading the range flag failed");
+      continue;
+    }
+    if (flag) {
+      (void)hipMemset(b.d_err, 0, sizeof(int));
+      if (rc == LAM_OK)
+        rc = fail(ctx, LAM_ERR_RANGE, "autotune: a tuning block has samples outside the int8 range after scaling");
+    }
   }
   if (result) {
     result[0] = ctx->samples_per_wg;

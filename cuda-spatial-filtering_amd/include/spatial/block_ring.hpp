@@ -16,6 +16,10 @@
 // one buffer while the pipeline's H2D copy and kernels of the others run.
 // The blocks are pinned (lam_host_alloc), so the H2D copies are DMA at full
 // link speed and overlap the previous blocks' kernels on their own streams.
+// The ring relies on the pipeline releasing every block handed to
+// execute_pipeline exactly once, also when execute_pipeline throws (the
+// pipelines in spatial/pipeline.hpp do): a failed block comes back through
+// release_buffer, never twice.
 #pragma once
 #include <condition_variable>
 #include <cstring>
@@ -98,9 +102,10 @@ class BlockRing : public ProcessorStateBase {
   }
 
   // Execute every submitted block, join the feeder and wait until the
-  // pipeline has released every block it executed (a GPUPipeline releases
-  // each non-dummy block after its H2D copy), so that no release arrives
-  // after the ring is gone.  Rethrows an exception raised by execute_pipeline.
+  // pipeline has released every block handed to it (a GPUPipeline releases
+  // each non-dummy block after its H2D copy, or when execute_pipeline fails),
+  // so that no release arrives after the ring is gone.  Rethrows an exception
+  // raised by execute_pipeline.
   void stop() {
     {
       std::lock_guard<std::mutex> lock(mu_);
@@ -158,13 +163,15 @@ class BlockRing : public ProcessorStateBase {
       try {
         pipeline.execute_pipeline(blocks_[i].get());
       } catch (...) {
-        // the failed and the queued blocks go back to the producer, so that a
-        // producer waiting in acquire() is not stranded; stop() rethrows
+        // the failed block returns through release_buffer (the pipeline
+        // releases it once, as any handed block); the queued blocks go back to
+        // the producer, so that a producer waiting in acquire() is not
+        // stranded; stop() waits for the release and rethrows
         {
           std::lock_guard<std::mutex> lock(mu_);
           error_ = std::current_exception();
           failed_ = true;
-          free_.push_back(i);
+          ++executed_;
           for (size_t j : ready_) free_.push_back(j);
           ready_.clear();
         }

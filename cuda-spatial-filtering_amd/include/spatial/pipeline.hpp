@@ -21,12 +21,16 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <utility>
 #include <vector>
+
+#include <hip/hip_runtime_api.h>
 
 #include "lam.h"
 #include "spatial/output.hpp"
 #include "spatial/packet_formats.hpp"
 #include "spatial/pipeline_base.hpp"
+#include "spatial/pointing.hpp"
 
 // complex<half>[C][P][B][R] beam weights (pipeline/common.hpp:52-55).
 template <typename T>
@@ -34,13 +38,52 @@ struct BeamWeightsT {
   spatial::complex_half weights[T::NR_CHANNELS][T::NR_POLARIZATIONS][T::NR_BEAMS][T::NR_RECEIVERS];
 };
 
-// Beam steering as the pipelines see it when no targets are configured: an
-// inert maybe_refresh() (pipeline/common.hpp:262-305).  Target tracking
-// (pointing geometry) is not part of this path.
+// Beam steering with the reference's constructor (pipeline/common.hpp:220-248).
+// With no targets it is inert, as in the reference: maybe_refresh() returns
+// false and the pipeline keeps its static weights (every factory of the
+// reference's test harness and benchmark builds it so,
+// tests/support/pipeline_harness.hpp:73-96, 145-165; apps/bench_gpu.cu:122-125).
+// Target tracking needs the pointing geometry, which is not part of this
+// path: a non-empty target list throws std::runtime_error.
 template <typename T>
 struct BeamSteering {
+  using BeamWeights = BeamWeightsT<T>;
+
   BeamSteering() = default;
-  void maybe_refresh() {}
+  BeamSteering(std::vector<BeamTarget> targets, std::unordered_map<int, ENUPosition> antenna_positions,
+               std::unordered_map<int, int> antenna_mapping, FrequencyPlan frequency_plan, int min_freq_channel,
+               ArrayLocation array_location, double update_interval_seconds, int num_buffers,
+               const typename T::AntennaGains *calibration_gains = nullptr)
+      : targets_(std::move(targets)),
+        antenna_positions_(std::move(antenna_positions)),
+        antenna_mapping_(std::move(antenna_mapping)),
+        frequency_plan_(frequency_plan),
+        min_freq_channel_(min_freq_channel),
+        array_location_(array_location),
+        update_interval_seconds_(update_interval_seconds),
+        num_buffers_(num_buffers),
+        calibration_gains_(calibration_gains) {
+    if (!targets_.empty())
+      throw std::runtime_error(
+          "BeamSteering: beam target tracking (pointing geometry) is not available in this build; "
+          "construct it with no targets to keep the static beam weights");
+  }
+
+  // True once real targets have been supplied (never, here).
+  bool active() const { return !targets_.empty(); }
+  // Returns true if new weights were enqueued: with no targets, never.
+  bool maybe_refresh() { return false; }
+
+ private:
+  std::vector<BeamTarget> targets_;
+  std::unordered_map<int, ENUPosition> antenna_positions_;
+  std::unordered_map<int, int> antenna_mapping_;
+  FrequencyPlan frequency_plan_{};
+  int min_freq_channel_ = 0;
+  ArrayLocation array_location_{};
+  double update_interval_seconds_ = 0.0;
+  int num_buffers_ = 0;
+  const typename T::AntennaGains *calibration_gains_ = nullptr;
 };
 
 namespace spatial {
@@ -123,6 +166,27 @@ inline void eigen_done_host_func(void *data) {
   delete ctx;
 }
 
+// start_run / stop_run timing events of the benchmark ring
+// (lambda_corr_beam_only_gpu_pipeline.hpp:199-201, 224, 257): recorded on the
+// stream of the buffer that runs the block, around its whole enqueue.
+template <size_t N>
+struct RunEvents {
+  hipEvent_t start[N] = {}, stop[N] = {};
+  RunEvents() {
+    for (size_t i = 0; i < N; ++i)
+      if (hipEventCreate(&start[i]) != hipSuccess || hipEventCreate(&stop[i]) != hipSuccess)
+        throw std::runtime_error("hipEventCreate failed");
+  }
+  ~RunEvents() {
+    for (size_t i = 0; i < N; ++i) {
+      if (start[i]) (void)hipEventDestroy(start[i]);
+      if (stop[i]) (void)hipEventDestroy(stop[i]);
+    }
+  }
+  RunEvents(const RunEvents &) = delete;
+  RunEvents &operator=(const RunEvents &) = delete;
+};
+
 class Context {
  public:
   Context(const lam_shape &shape, int mode, int num_buffers) {
@@ -142,6 +206,21 @@ class Context {
   lam_ctx *ctx_ = nullptr;
 };
 
+// Calls lam_execute for one block whose io `fill` completes.  A block is
+// released exactly once whatever happens: lam_execute releases it itself, also
+// when it fails; if completing the io throws first (an Output callback, say),
+// the release happens here before the exception leaves.
+template <typename Fill>
+void execute_with_release(Context &ctx, lam_block_io &io, Fill &&fill) {
+  try {
+    fill();
+  } catch (...) {
+    if (io.release_fn) io.release_fn(io.release_user);
+    throw;
+  }
+  ctx(lam_execute(ctx.get(), &io));
+}
+
 }  // namespace detail
 }  // namespace spatial
 
@@ -150,6 +229,15 @@ class LambdaCorrBeamOnlyGPUPipeline : public GPUPipeline {
  public:
   static constexpr size_t NR_BENCHMARKING_RUNS = 100;
   using BeamWeights = BeamWeightsT<T>;
+
+ private:
+  spatial::detail::RunEvents<NR_BENCHMARKING_RUNS> run_events_;
+
+ public:
+  // the reference's benchmark ring (lambda_corr_beam_only_gpu_pipeline.hpp:199-201):
+  // hipEventElapsedTime(&ms, start_run[i], stop_run[i]) is the GPU time of run i
+  hipEvent_t *const start_run = run_events_.start;
+  hipEvent_t *const stop_run = run_events_.stop;
 
   LambdaCorrBeamOnlyGPUPipeline(const int num_buffers, BeamWeightsT<T> *h_weights,
                                 BeamSteering<T> beam_steering = {},
@@ -186,18 +274,22 @@ class LambdaCorrBeamOnlyGPUPipeline : public GPUPipeline {
     io.inputs_on_device = 0;
     io.release_fn = spatial::detail::release_buffer_host_func;
     io.release_user = new spatial::detail::ReleaseContext{state_, packet_data->buffer_index, dummy_run};
-    if (output_ != nullptr && !dummy_run) {
-      const size_t block_num = output_->register_beam_data_block(start_seq_num, end_seq_num);
-      if (block_num != std::numeric_limits<size_t>::max()) {
-        io.beams_dst = output_->get_beam_data_landing_pointer(block_num);
-        io.beams_done_fn = spatial::detail::beam_done_host_func;
-        io.beams_done_user = new spatial::detail::OutputDoneContext{output_, block_num};
-        std::memcpy(output_->get_arrivals_data_landing_pointer(block_num),
-                    packet_data->get_arrivals_ptr(), packet_data->get_arrivals_size());
-        output_->register_arrivals_transfer_complete(block_num);
+    auto stream = static_cast<hipStream_t>(lam_next_stream(ctx_.get()));
+    (void)hipEventRecord(start_run[benchmark_runs_done], stream);
+    spatial::detail::execute_with_release(ctx_, io, [&] {
+      if (output_ != nullptr && !dummy_run) {
+        const size_t block_num = output_->register_beam_data_block(start_seq_num, end_seq_num);
+        if (block_num != std::numeric_limits<size_t>::max()) {
+          io.beams_dst = output_->get_beam_data_landing_pointer(block_num);
+          io.beams_done_fn = spatial::detail::beam_done_host_func;
+          io.beams_done_user = new spatial::detail::OutputDoneContext{output_, block_num};
+          std::memcpy(output_->get_arrivals_data_landing_pointer(block_num),
+                      packet_data->get_arrivals_ptr(), packet_data->get_arrivals_size());
+          output_->register_arrivals_transfer_complete(block_num);
+        }
       }
-    }
-    ctx_(lam_execute(ctx_.get(), &io));
+    });
+    (void)hipEventRecord(stop_run[benchmark_runs_done], stream);
     if (output_ != nullptr && !dummy_run) {
       num_correlation_units_integrated_ += 1;
       if (num_correlation_units_integrated_ >= nr_blocks_to_accumulate_) dump_visibilities();
@@ -259,6 +351,14 @@ class LambdaGPUPipeline : public GPUPipeline {
   static constexpr size_t NR_BENCHMARKING_RUNS = 100;
   using BeamWeights = BeamWeightsT<T>;
 
+ private:
+  spatial::detail::RunEvents<NR_BENCHMARKING_RUNS> run_events_;
+
+ public:
+  // the reference's benchmark ring (lambda_gpu_pipeline.hpp:444, 466, 532)
+  hipEvent_t *const start_run = run_events_.start;
+  hipEvent_t *const stop_run = run_events_.stop;
+
   LambdaGPUPipeline(const int num_buffers, BeamWeightsT<T> *h_weights, BeamSteering<T> beam_steering = {})
       : num_buffers_(std::min(num_buffers, 4)),
         h_weights_(h_weights),
@@ -314,6 +414,9 @@ class LambdaGPUPipeline : public GPUPipeline {
     io.scales = packet_data->get_scales_ptr();
     io.release_fn = spatial::detail::release_buffer_host_func;
     io.release_user = new spatial::detail::ReleaseContext{state_, packet_data->buffer_index, dummy_run};
+    auto stream = static_cast<hipStream_t>(lam_next_stream(ctx_.get()));
+    (void)hipEventRecord(start_run[benchmark_runs_done], stream);
+    spatial::detail::execute_with_release(ctx_, io, [&] {
     if (output_ != nullptr && !dummy_run) {  // lambda_gpu_pipeline.hpp:534-580
       const size_t block_num = output_->register_beam_data_block(start_seq_num, end_seq_num);
       const size_t eig_block = output_->register_eigendecomposition_data_block(start_seq_num, end_seq_num);
@@ -338,7 +441,8 @@ class LambdaGPUPipeline : public GPUPipeline {
         io.eigen_done_user = new spatial::detail::OutputDoneContext{output_, eig_block};
       }
     }
-    ctx_(lam_execute(ctx_.get(), &io));
+    });
+    (void)hipEventRecord(stop_run[benchmark_runs_done], stream);
     if (output_ != nullptr && !dummy_run) {
       num_correlation_units_integrated_ += 1;
       if (num_correlation_units_integrated_ >= static_cast<int>(T::NR_CORRELATED_BLOCKS_TO_ACCUMULATE))
@@ -417,7 +521,7 @@ class LambdaProjectionPipeline : public GPUPipeline {
     io.scales = packet_data->get_scales_ptr();
     io.release_fn = spatial::detail::release_buffer_host_func;
     io.release_user = new spatial::detail::ReleaseContext{state_, packet_data->buffer_index, dummy_run};
-    ctx_(lam_execute(ctx_.get(), &io));
+    spatial::detail::execute_with_release(ctx_, io, [] {});
     if (!dummy_run && lam_projection_runs(ctx_.get()) >= NR_RUNS_TO_AVERAGE)
       dump_projection(packet_data->start_seq_id, packet_data->end_seq_id);
   }
@@ -529,6 +633,7 @@ class LambdaAdaptiveBeamformedSpectraPipeline : public GPUPipeline {
     io.scales = packet_data->get_scales_ptr();
     io.release_fn = spatial::detail::release_buffer_host_func;
     io.release_user = new spatial::detail::ReleaseContext{state_, packet_data->buffer_index, dummy_run};
+    spatial::detail::execute_with_release(ctx_, io, [&] {
     if (output_ != nullptr && !dummy_run) {
       auto *ectx = new spatial::detail::EigenDoneContext{};
       ectx->output = output_;
@@ -566,7 +671,7 @@ class LambdaAdaptiveBeamformedSpectraPipeline : public GPUPipeline {
       io.eigen_done_fn = spatial::detail::eigen_done_host_func;
       io.eigen_done_user = ectx;
     }
-    ctx_(lam_execute(ctx_.get(), &io));
+    });
     current_buffer_ = (current_buffer_ + 1) % num_buffers_;
   }
 

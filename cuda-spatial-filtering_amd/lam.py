@@ -134,6 +134,8 @@ def load_library(path: str = LIB_PATH):
     L.lam_learn_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, vp]
     L.lam_learn_params.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int]
     L.lam_learn_grad.argtypes = [vp, vp, vp, vp, ctypes.c_int, vp, vp]
+    L.lam_graph_replays.argtypes = [vp]
+    L.lam_graph_replays.restype = ctypes.c_longlong
     L.lam_correlate.argtypes = [ctypes.POINTER(LamShape), ctypes.c_int, vp, vp, vp, vp, vp]
     L.lam_correlate_noalloc.argtypes = [ctypes.POINTER(LamShape), ctypes.c_int, vp, vp, vp, vp, vp]
     L.lam_synth_block.argtypes = [ctypes.POINTER(LamShape), ctypes.c_uint64, i32, ctypes.c_float, i32,
@@ -402,6 +404,9 @@ class Pipeline:
 
     def set_eig_params(self, max_sweeps=12, tol=1e-6):
         self._chk(self.L.lam_set_eig_params(self.h, int(max_sweeps), float(tol)))
+
+    def graph_replays(self):
+        return int(self.L.lam_graph_replays(self.h))
 
     def set_graphs(self, enable=True):
         self._chk(self.L.lam_set_graphs(self.h, int(bool(enable))))

@@ -186,7 +186,14 @@ typedef struct lam_block_io {
 } lam_block_io;
 
 /* Enqueue one block on the next buffer's stream (execute_pipeline,
- * pipeline_base.hpp:28-29).  Returns after enqueueing. */
+ * pipeline_base.hpp:28-29).  Returns after enqueueing.  When io->release_fn
+ * is set it is called exactly once per call, also when the call fails: from
+ * the stream after the H2D copy (or after the block, for device inputs), or,
+ * if the call fails before that was enqueued, directly before returning.
+ * A block whose scaled samples left the int8 range (LAM_ERR_RANGE) is
+ * reported by the first lam_execute, lam_dump_* or lam_sync call after it
+ * has completed: that call fails with LAM_ERR_RANGE, names the block in
+ * lam_last_error, and runs no new block (lam_sync: waits first). */
 int lam_execute(lam_ctx *ctx, const lam_block_io *io);
 
 /* Enqueue a copy of the visibility accumulator into dst (float32 layout
@@ -216,17 +223,23 @@ int lam_projection_runs(const lam_ctx *ctx);
  * accumulated since the last dump. */
 int lam_dump_projection(lam_ctx *ctx, float *evals_dst, float *evecs_dst, lam_host_fn done_fn, void *user);
 
-/* Wait for all enqueued work; reports LAM_ERR_RANGE if any sample scaled
- * out of int8 range since the last call. */
+/* Wait for all enqueued work; reports LAM_ERR_RANGE if a block since the
+ * last report had samples scaled out of the int8 range. */
 int lam_sync(lam_ctx *ctx);
 
 /* Device stream of the buffer that the next lam_execute will use. */
 void *lam_next_stream(lam_ctx *ctx);
 
 /* hipGraph capture of the per-block kernel sequence: 1 = use graphs
- * (default; LAM_DISABLE_GRAPH=1 in the environment turns it off like
- * SPATIAL_DISABLE_CUDA_GRAPH, lambda_corr_beam_only_gpu_pipeline.hpp:561). */
+ * (default).  SPATIAL_DISABLE_CUDA_GRAPH in the environment (any value), as
+ * in the reference (lambda_corr_beam_only_gpu_pipeline.hpp:561), or
+ * LAM_DISABLE_GRAPH=1 turn it off at lam_create.  A failed capture or
+ * instantiation prints one warning to stderr and the context runs eagerly
+ * from then on (lambda_corr_beam_only_gpu_pipeline.hpp:566-577). */
 int lam_set_graphs(lam_ctx *ctx, int enable);
+/* Number of graph replays (hipGraphLaunch) so far; stays 0 when every block
+ * ran eagerly.  -1 for a NULL context. */
+long long lam_graph_replays(const lam_ctx *ctx);
 
 /* Per-kernel timing: when enabled, HIP events bracket every kernel of each
  * block on its stream; lam_kernel_times returns the accumulated milliseconds
@@ -270,7 +283,12 @@ int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads);
  *   - warm-start refinement steps 1..3 when LAM_EIG_LEARNED / PREVIOUS is set
  *     (rejected problems fall back to the plain solver inside the timing, so a
  *     too-short refinement loses on time).
- * The blocks do not reach the visibility accumulator.  result[3] (may be NULL)
+ * Each choice measures its candidates in three interleaved rounds (order
+ * rotated per round) after a warm-up of about 48 blocks, judges each by its
+ * median, and keeps the setting in force before the call unless the best
+ * median beats it by more than that setting's own spread over the rounds.
+ * The blocks do not reach the visibility accumulator, and training capture
+ * (lam_learn_capture) is suspended while tuning.  result[3] (may be NULL)
  * = {samples_per_wg, plain_threads, refine_steps}. */
 int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *scales, int nblocks,
                  int iters, int32_t *result);

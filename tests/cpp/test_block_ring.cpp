@@ -50,7 +50,16 @@ class FakePipeline : public GPUPipeline {
   void execute_pipeline(FinalPacketData *pd, const bool = false) override {
     auto *b = static_cast<FakeBlock *>(pd);
     const int n = calls_++;
-    if (n == fail_at_) throw std::runtime_error("injected pipeline failure");
+    if (n == fail_at_) {
+      // as lam_execute does when it fails after queuing the release: the
+      // block comes back once, later, from the callback thread
+      const size_t idx = pd->buffer_index;
+      releasers_.emplace_back([this, idx] {
+        std::this_thread::sleep_for(std::chrono::milliseconds(5));
+        state_->release_buffer(idx);
+      });
+      throw std::runtime_error("injected pipeline failure");
+    }
     const int tag = b->samples[0];
     {
       std::lock_guard<std::mutex> lock(mu_);
@@ -121,6 +130,20 @@ static void failure_does_not_strand_the_producer() {
   }
   CHECK(rethrown, "stop() rethrows the pipeline's exception");
   CHECK(refused > 0 && submitted + refused == 10, "submitted %d refused %d", submitted, refused);
+  // stop() returned after the failed block's release, and every buffer is
+  // free exactly once (a buffer pushed by both the feeder and the release
+  // would be handed out twice)
+  CHECK(ring.blocks_released() == ring.blocks_executed(), "released %zu of %zu", ring.blocks_released(),
+        ring.blocks_executed());
+  size_t handed = 0;
+  try {
+    for (int k = 0; k < 8; ++k) {
+      ring.acquire();
+      ++handed;
+    }
+  } catch (const std::runtime_error &) {
+  }
+  CHECK(handed == ring.size(), "free buffers after the failure: %zu of %zu", handed, ring.size());
 }
 
 int main() {

@@ -12,6 +12,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "spatial/block_ring.hpp"
@@ -486,6 +487,139 @@ TEST(BlockRingOverlappedIngest) {
   CHECK(output->beam_blocks_done == kBlocks, "beam callbacks %d", output->beam_blocks_done.load());
 }
 
+// The reference harness's factories, written exactly as
+// tests/support/pipeline_harness.hpp:73-96 and 145-165 write them: inert
+// BeamSteering from its full constructor, then the pipeline constructors.
+namespace pipeline_factories {
+template <typename Config>
+std::unique_ptr<LambdaGPUPipeline<Config>> make_gpu_pipeline(int num_buffers, BeamWeightsT<Config> *weights,
+                                                             int min_freq_channel = 0) {
+  BeamSteering<Config> beam_steering(/*targets=*/{}, /*antenna_positions=*/{},
+                                     /*antenna_mapping=*/{}, FrequencyPlan{}, min_freq_channel, ArrayLocation{},
+                                     /*update_interval_seconds=*/1.0, num_buffers);
+  return std::make_unique<LambdaGPUPipeline<Config>>(num_buffers, weights, std::move(beam_steering));
+}
+template <typename Config>
+std::unique_ptr<LambdaCorrBeamOnlyGPUPipeline<Config>> make_corr_beam_only_pipeline(int num_buffers,
+                                                                                     BeamWeightsT<Config> *weights,
+                                                                                     int min_freq_channel = 0) {
+  BeamSteering<Config> beam_steering(/*targets=*/{}, /*antenna_positions=*/{},
+                                     /*antenna_mapping=*/{}, FrequencyPlan{}, min_freq_channel, ArrayLocation{},
+                                     /*update_interval_seconds=*/1.0, num_buffers);
+  return std::make_unique<LambdaCorrBeamOnlyGPUPipeline<Config>>(num_buffers, weights, std::move(beam_steering));
+}
+template <typename Config>
+std::unique_ptr<LambdaAdaptiveBeamformedSpectraPipeline<Config>> make_adaptive_beamformed_spectra_pipeline(
+    int num_buffers, BeamWeightsT<Config> *weights, std::unordered_map<int, int> nr_signal_eigenvectors = {},
+    bool shrink_eigenvalues = false, int min_freq_channel = 0, bool detect_signal_eigenmodes = false,
+    float detection_threshold_delta = 3.0f, double eigenmode_stats_interval_seconds = 10.0) {
+  if (nr_signal_eigenvectors.empty()) nr_signal_eigenvectors[min_freq_channel] = 1;
+  BeamSteering<Config> beam_steering(/*targets=*/{}, /*antenna_positions=*/{},
+                                     /*antenna_mapping=*/{}, FrequencyPlan{}, min_freq_channel, ArrayLocation{},
+                                     /*update_interval_seconds=*/1.0, num_buffers);
+  return std::make_unique<LambdaAdaptiveBeamformedSpectraPipeline<Config>>(
+      num_buffers, weights, nr_signal_eigenvectors, min_freq_channel, std::move(beam_steering),
+      shrink_eigenvalues, detect_signal_eigenmodes, detection_threshold_delta, eigenmode_stats_interval_seconds);
+}
+}  // namespace pipeline_factories
+
+TEST(HarnessFactoriesAndRunEvents) {
+  FakeProcessorState state;
+  {  // corr-beam through the harness factory: beams (8, -8), vis 64 (test_corr_beam_pipeline.cu:107-140)
+    BeamWeightsT<Small> w = unity_weights<Small>();
+    auto p = pipeline_factories::make_corr_beam_only_pipeline<Small>(2, &w);
+    auto out = std::make_shared<SingleHostMemoryOutput<Small>>();
+    p->set_state(&state);
+    p->set_output(out);
+    for (int k = 0; k < 3; ++k) {
+      typename Small::PacketFinalDataType pd;
+      fill_block<Small>(pd, const_sample, unit_scale);
+      p->execute_pipeline(&pd);
+    }
+    p->dump_visibilities();
+    p->synchronize();
+    CHECK(spatial::half_to_float((*out->beam_data)[0][1][0][5][0]) == 8.0f, "factory corr-beam beam");
+    CHECK((*out->visibilities)[0][bl(2, 3)][0][1][0] == 3 * 64.0f, "factory corr-beam vis");
+    // the benchmark ring: hipEventElapsedTime(start_run[i], stop_run[i]) as apps/bench_gpu.cu:226-232 reads it
+    CHECK(p->benchmark_runs_done == 3, "runs %zu", p->benchmark_runs_done);
+    for (size_t i = 0; i < p->benchmark_runs_done; ++i) {
+      float ms = -1.0f;
+      const hipError_t e = hipEventElapsedTime(&ms, p->start_run[i], p->stop_run[i]);
+      CHECK(e == hipSuccess && ms >= 0.0f && ms < 1000.0f, "run %zu: %s %f ms", i, hipGetErrorString(e), ms);
+    }
+  }
+  {  // adaptive through the harness factory with the default K map {min_freq_channel: 1}
+    BeamWeightsT<Adaptive> w = unity_weights<Adaptive>();
+    auto p = pipeline_factories::make_adaptive_beamformed_spectra_pipeline<Adaptive>(2, &w);
+    auto out = std::make_shared<SingleHostMemoryOutput<Adaptive, 2 * Adaptive::NR_BEAMS>>();
+    p->set_state(&state);
+    p->set_output(out);
+    typename Adaptive::PacketFinalDataType pd;
+    fill_block<Adaptive>(pd, const_sample, unit_scale);
+    p->execute_pipeline(&pd);
+    p->synchronize();
+    CHECK(spatial::half_to_float((*out->beam_data)[0][0][0][7][0]) == 8.0f, "factory adaptive original beam");
+    double pw = 0.0;
+    for (size_t t = 0; t < 64; ++t) {
+      const float re = spatial::half_to_float((*out->beam_data)[0][0][1][t][0]);
+      const float im = spatial::half_to_float((*out->beam_data)[0][0][1][t][1]);
+      pw += re * re + im * im;
+    }
+    CHECK(pw < 1.0, "factory adaptive K=1 nulls the rank-one signal (%f)", pw);
+  }
+  {  // LambdaGPU through the harness factory: principal eigenvalue 256 (test_pipeline.cu:956-1052)
+    BeamWeightsT<SmallGPU> w = unity_weights<SmallGPU>();
+    auto p = pipeline_factories::make_gpu_pipeline<SmallGPU>(1, &w);
+    auto out = std::make_shared<SingleHostMemoryOutput<SmallGPU>>();
+    p->set_state(&state);
+    p->set_output(out);
+    typename SmallGPU::PacketFinalDataType pd;
+    fill_block<SmallGPU>(pd, const_sample, unit_scale);
+    p->execute_pipeline(&pd);
+    p->synchronize();
+    CHECK(std::fabs((*out->eigenvalues)[0][0][0][3] - 256.0f) <= 1e-4f, "factory LambdaGPU eigenvalue");
+    float ms = -1.0f;
+    CHECK(hipEventElapsedTime(&ms, p->start_run[0], p->stop_run[0]) == hipSuccess && ms >= 0.0f, "LambdaGPU run event");
+  }
+  {  // steering with targets needs the pointing geometry: refused loudly
+    bool threw = false;
+    try {
+      BeamSteering<Small> s({BeamTarget{"radec", 83.6, 22.0}}, {}, {}, FrequencyPlan{}, 0, ArrayLocation{}, 1.0, 1);
+    } catch (const std::runtime_error &) {
+      threw = true;
+    }
+    CHECK(threw, "BeamSteering with targets throws");
+    BeamSteering<Small> inert({}, {}, {}, FrequencyPlan{}, 0, ArrayLocation{}, 1.0, 1);
+    CHECK(!inert.active() && !inert.maybe_refresh(), "inert steering");
+  }
+}
+
+// A block whose scaled samples leave the exact int8 range (|x s| = 300) must
+// not corrupt a feeder loop silently: the next execute_pipeline after it
+// completes throws std::runtime_error, and every block is released once.
+TEST(RangeErrorSurfacesInFeederLoop) {
+  BeamWeightsT<Small> w = unity_weights<Small>();
+  auto p = pipeline_factories::make_corr_beam_only_pipeline<Small>(2, &w);
+  FakeProcessorState state;
+  p->set_state(&state);
+  typename Small::PacketFinalDataType bad, good;
+  fill_block<Small>(bad, [](size_t, size_t, int, int, int, int) { return std::complex<int8_t>{100, 0}; },
+                    [](size_t, size_t, int, int, int) { return (int16_t)3; });
+  fill_block<Small>(good, const_sample, unit_scale);
+  p->execute_pipeline(&bad);
+  std::this_thread::sleep_for(std::chrono::milliseconds(300));
+  bool threw = false;
+  try {
+    p->execute_pipeline(&good);
+  } catch (const std::runtime_error &e) {
+    threw = std::string(e.what()).find("int8 range") != std::string::npos;
+  }
+  CHECK(threw, "the block after a range error throws");
+  p->execute_pipeline(&good);  // reported once; the stream carries on
+  p->synchronize();
+  CHECK(state.released.size() == 3, "released %zu", state.released.size());
+}
+
 int main() {
   RUN(CorrBeamExactValues);
   RUN(CorrBeamZeroInputAndWeights);
@@ -499,6 +633,8 @@ int main() {
   RUN(LambdaGPUEigenvalueBasic);
   RUN(ProjectionAverageBasic);
   RUN(BlockRingOverlappedIngest);
+  RUN(HarnessFactoriesAndRunEvents);
+  RUN(RangeErrorSurfacesInFeederLoop);
   std::printf("%d checks, %d failures\n", g_checks, g_fail);
   return g_fail == 0 ? 0 : 1;
 }

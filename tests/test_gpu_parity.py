@@ -110,6 +110,56 @@ def test_legacy_tcc_golden_conjugation():
     assert tuple(vis[0, bl(5, 5), 0, 0]) == (190.0, 0.0)
 
 
+def _golden_weights(shape, w_br):
+    """FP16 weights [C][P][B][R][2] with beam rows w_br on receivers 0..1 of every (c, p)."""
+    halfs = np.zeros((shape.C, shape.P, shape.B, shape.R, 2), np.float16)
+    w = np.asarray(w_br)
+    halfs[..., :2, 0] = w.real
+    halfs[..., :2, 1] = w.imag
+    return halfs.view(np.uint16)
+
+
+@pytest.mark.parametrize("case", ["beamforming", "ccglib"])
+@pytest.mark.parametrize("mode", [lam.MODE_CORR_BEAM, lam.MODE_ADAPTIVE])
+def test_complex_weight_goldens(case, mode):
+    """Complex weights, no conjugation: tests/test_beamforming.cpp:55-80 (beams
+    0+6j, -1+5j / 0+12j, -2+10j; visibilities 10, 6-2j, 9) and the ccglib
+    a @ b.T golden of tests/test_spatial.cpp:221-283 (-1+13j, 4-3j / -3-1j, 7j),
+    through the corr-beam kernel and the adaptive 2B-beam kernel (K = 0, so the
+    mitigated beams equal the originals)."""
+    shape = syn.BlockShape(C=1, F=1, NT=8, R=4, P=2, RPP=4, NP=1, B=2, R_pad=32)
+    if case == "beamforming":
+        x = [[1 + 1j, 2 + 2j], [2 + 2j, 1j]]            # x[receiver][time]
+        w = [[1 + 1j, 1 + 1j], [2 + 2j, 2 + 2j]]        # w[beam][receiver]
+        want = np.array([[6j, -1 + 5j], [12j, -2 + 10j]])
+    else:
+        x = np.array([[1 + 1j, 2 + 4j], [-2 - 1j, 2]]).T
+        w = [[1 + 2j, 2 + 1j], [-1 - 2j, 1j]]
+        want = np.array([[-1 + 13j, 4 - 3j], [-3 - 1j, 7j]])
+    x = np.asarray(x)
+
+    def fn(c, f, k, t, r, p):
+        return (int(x[r, t].real), int(x[r, t].imag)) if (r < 2 and t < 2) else (0, 0)
+
+    _log(f"complex-weight golden {case} mode {mode}")
+    ring, scales = syn.ring_from_fn(shape, fn)
+    p = lam.Pipeline(shape, mode)
+    p.set_weights(_golden_weights(shape, w))
+    if mode == lam.MODE_ADAPTIVE:
+        p.set_fixed_k(np.zeros((1, 2), np.int32))
+    out = p.run_host(ring, scales)
+    vis = p.dump_visibilities()
+    p.close()
+    b = out["beams"]
+    for pol in range(2):
+        for rep in range(b.shape[2] // 2):
+            assert np.array_equal(b[0, pol, 2 * rep:2 * rep + 2, :2], want)
+            assert not b[0, pol, 2 * rep:2 * rep + 2, 2:].any()
+    if case == "beamforming":
+        for pq in [(0, 0), (0, 1), (1, 0), (1, 1)]:
+            assert [tuple(v) for v in vis[0, :3, pq[0], pq[1]]] == [(10, 0), (6, -2), (9, 0)]
+
+
 # ------------------------------------------------------- random-data parity
 @pytest.mark.parametrize("shape", [
     syn.BlockShape(C=4, F=1, NT=64, R=36, P=2, RPP=36, NP=4, B=4),
@@ -190,6 +240,65 @@ def test_range_error_is_reported():
         p.sync()
     assert ei.value.code == lam.LAM_ERR_RANGE
     p.close()
+
+
+def test_range_error_fails_the_next_call_and_releases_once():
+    """A block with |x s| > 127 fails the first call after it completes (no
+    silent corruption in a feeder loop that never calls lam_sync); every
+    host block handed to lam_execute is released exactly once, also when the
+    call fails."""
+    import time
+    shape = SMALL
+    bad_ring, bad_scales = syn.ring_from_fn(shape, lambda *a: (100, 0), lambda *a: 3)
+    ring, scales = syn.ring_from_fn(shape, lambda *a: (2, -2))
+    p = lam.Pipeline(shape, lam.MODE_CORR_BEAM)
+    p.set_weights(syn.unity_weights(shape)[0])
+    released = []
+    cb = lam.HOSTFN(lambda user: released.append(int(user or 0)))
+    p.execute(bad_ring, bad_scales, release_fn=cb, release_user=1)
+    time.sleep(0.5)
+    with pytest.raises(lam.LamError) as ei:
+        p.execute(ring, scales, release_fn=cb, release_user=2)
+    assert ei.value.code == lam.LAM_ERR_RANGE
+    assert "block 0" in str(ei.value)
+    p.dump_visibilities_i64()  # drops the corrupted block's visibilities
+    # the error was reported once: the next block runs
+    p.execute(ring, scales, release_fn=cb, release_user=3)
+    # a call that fails its own checks still releases its block
+    with pytest.raises(lam.LamError):
+        p.execute(ring, scales, spectra_out=np.zeros(4, np.float32), release_fn=cb, release_user=4)
+    p.sync()
+    assert sorted(released) == [1, 2, 3, 4]
+    # blocks 2 and 4 never reached the accumulator
+    vis = p.dump_visibilities_i64()
+    s = _o(shape)
+    assert np.array_equal(vis, orc.correlate(s, orc.ingest(s, ring, scales, orc.WINDOW_CORR_BEAM)))
+    p.close()
+
+
+def test_graphs_replay_and_disable_switch(monkeypatch):
+    """Blocks replay hipGraphs by default; SPATIAL_DISABLE_CUDA_GRAPH (any
+    value, lambda_corr_beam_only_gpu_pipeline.hpp:561) forces eager launches
+    with identical results."""
+    shape = syn.BlockShape(C=2, F=1, NT=64, R=36, P=2, RPP=36, NP=2, B=4)
+    rng = np.random.default_rng(3)
+    ring = rng.integers(-127, 128, shape.ring_shape(), dtype=np.int16).astype(np.int8)
+    scales = np.ones(shape.scale_shape(), np.int16)
+    w16 = syn.random_weights(shape, 1)[0]
+    outs = []
+    for disable in (False, True):
+        if disable:
+            monkeypatch.setenv("SPATIAL_DISABLE_CUDA_GRAPH", "")
+        p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+        p.set_weights(w16)
+        p.set_fixed_k(np.ones((shape.C, 2), np.int32))
+        for _ in range(3):
+            out = p.run_host(ring, scales)
+        replays = p.graph_replays()
+        p.close()
+        assert (replays == 0) if disable else (replays >= 3)
+        outs.append(out["beams"])
+    assert np.array_equal(outs[0], outs[1])
 
 
 def test_gains_and_fine_delays():
@@ -561,6 +670,28 @@ def test_autotune_adaptive_warm_start():
     p.close()
     s = _o(shape)
     assert np.array_equal(vis64, orc.correlate(s, orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)))
+
+
+@pytest.mark.parametrize("num_buffers", [1, 3])
+def test_autotune_does_not_capture_training_blocks(num_buffers):
+    """A pending lam_learn_capture records only the caller's blocks: the
+    tuning blocks (plain solver, many passes) are not training examples."""
+    shape = syn.BlockShape(C=4, F=1, NT=64, R=36, P=2, RPP=36, NP=8, B=4)
+    scene = syn.make_scene(shape, seed=31, n_tones=1, amp=30.0)
+    ring, scales = syn.make_block(shape, scene, seed=32)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE, num_buffers=num_buffers)
+    p.set_weights(syn.random_weights(shape, seed=5)[0])
+    p.set_fixed_k(np.ones((shape.C, 2), np.int32))
+    p.learn_init(4, 64, seed=3)
+    p.learn_capture(5)
+    p.run_host(ring, scales)
+    assert p.learn_captured() == 1
+    d_ring, d_scales = lam.DeviceArray.from_numpy(ring), lam.DeviceArray.from_numpy(scales)
+    p.autotune([d_ring], [d_scales], iters=1)
+    assert p.learn_captured() == 1
+    p.run_host(ring, scales)
+    assert p.learn_captured() == 2
+    p.close()
 
 
 def test_subarrays_uneven_sizes_persistent():

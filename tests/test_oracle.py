@@ -107,6 +107,50 @@ def test_two_packets():
     assert np.all(beams == 8 - 8j)
 
 
+def _two_receiver_case(x_rt, w_br):
+    """Oracle beams and visibilities of 2 receivers x 2 samples (pol 0)."""
+    s = orc.Shape(C=1, F=1, NT=2, R=2, P=2, RPP=2, NP=1, B=2)
+    y = np.zeros((1, 2, 2, 2, 2), np.int32)
+    x = np.asarray(x_rt)
+    y[0, :, :, 0, 0] = x.real.T
+    y[0, :, :, 0, 1] = x.imag.T
+    w = np.zeros((1, 2, 2, 2), complex)
+    w[0, 0] = w_br
+    return orc.beamform(s, y, w)[0, 0], orc.correlate(s, y)[0, :, 0, 0]
+
+
+def test_legacy_beamforming_golden_complex_weights():
+    """tests/test_beamforming.cpp:10-81 (legacy, not built): receiver series
+    x0 = (1+1j, 2+2j), x1 = (2+2j, 1j), beam weights (1+1j, 1+1j) and
+    (2+2j, 2+2j); `weights @ data` -> beams 0+6j, -1+5j / 0+12j, -2+10j with no
+    conjugation of the weights; visibilities 10, 6-2j, 9 (x_i conj(x_j))."""
+    beams, vis = _two_receiver_case([[1 + 1j, 2 + 2j], [2 + 2j, 1j]], [[1 + 1j, 1 + 1j], [2 + 2j, 2 + 2j]])
+    assert np.array_equal(beams, np.array([[6j, -1 + 5j], [12j, -2 + 10j]]))
+    assert [tuple(v) for v in vis] == [(10, 0), (6, -2), (9, 0)]
+
+
+def test_legacy_ccglib_golden_complex_gemm():
+    """tests/test_spatial.cpp:221-283 (legacy, commented out): ccglib's
+    row-major a times column-major b is a @ b.T; with a = beam weights
+    [[1+2j, 2+1j], [-1-2j, 1j]] and b[t][r] = samples [[1+1j, 2+4j], [-2-1j, 2]]
+    the beams are [[-1+13j, 4-3j], [-3-1j, 0+7j]]."""
+    b = np.array([[1 + 1j, 2 + 4j], [-2 - 1j, 2]])
+    beams, _ = _two_receiver_case(b.T, [[1 + 2j, 2 + 1j], [-1 - 2j, 1j]])
+    assert np.array_equal(beams, np.array([[-1 + 13j, 4 - 3j], [-3 - 1j, 7j]]))
+
+
+def test_beamform_brute_force_complex_weights():
+    """Random complex weights and int8 samples: oracle == einsum of w x (a
+    conjugated or transposed weight operand fails this)."""
+    rng = np.random.default_rng(17)
+    s = orc.Shape(C=3, F=1, NT=16, R=7, P=2, RPP=7, NP=2, B=5)
+    y = rng.integers(-128, 128, (3, 32, 7, 2, 2)).astype(np.int32)
+    w = rng.normal(size=(3, 2, 5, 7)) + 1j * rng.normal(size=(3, 2, 5, 7))
+    x = y[..., 0].astype(np.float64) + 1j * y[..., 1]
+    ref = np.einsum("cpbr,ctrp->cpbt", w, x)
+    assert np.allclose(orc.beamform(s, y, w), ref, rtol=1e-12, atol=1e-9)
+
+
 def test_correlator_brute_force_and_extremes():
     """Random int8 data including -128: oracle == numpy einsum of x x^H."""
     rng = np.random.default_rng(1)
