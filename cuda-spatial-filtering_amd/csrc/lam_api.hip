@@ -1152,7 +1152,7 @@ int lam_set_weight_precision(lam_ctx *ctx, int precision) {
 }
 
 int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads) {
-  auto ok = [](int t) { return t == 64 || t == 128 || t == 256 || t == 512 || t == 1024; };
+  auto ok = [](int t) { return t == 64 || t == 128 || t == 256; };
   if (!ctx || !ok(plain_threads)) return LAM_ERR_ARG;
   ctx->eig_plain_threads = plain_threads;
   ctx->graphs_dirty = true;
@@ -1458,7 +1458,7 @@ int lam_autotune(lam_ctx *ctx, const void *const *samples, const void *const *sc
     ctx->eig_method = LAM_EIG_PLAIN;
     int chosen = ctx->eig_plain_threads;
     rc = pick_by_time(
-        {256, 512, 1024}, ctx->eig_plain_threads, kRounds,  // 64 and 128 are slower at every size measured
+        {64, 128, 256}, ctx->eig_plain_threads, kRounds,
         [&](int v) { ctx->eig_plain_threads = v; },
         [&](double &t) {
           return stream ? time_stream(ctx, samples, scales, nblocks, stream_iters, t)

@@ -39,6 +39,7 @@ __device__ __forceinline__ int rr_pos(int j, int r, int ne) {
 
 __device__ __forceinline__ float wave_sum(float v);
 __device__ __forceinline__ float wave_max(float v);
+__device__ __forceinline__ void wave_sync();
 // Maximum over the workgroup of non-negative values.
 __device__ __forceinline__ float block_max(float v, float *red) {
   v = wave_max(v);
@@ -139,32 +140,51 @@ template <int NTHR> struct JacobiLists {
   static constexpr int kMaxBlk = (32 * 33 / 2 + NTHR - 1) / NTHR;  // blocks per thread at n = 64
 };
 
-// Entry (r, c) of a Hermitian matrix kept as its upper triangle (column-major,
-// A[c * ld + r] for r <= c); the lower triangle is never read.
-__device__ __forceinline__ float2 herm_get(const float2 *A, int ld, int r, int c) {
-  if (r <= c) return A[c * ld + r];
-  const float2 v = A[r * ld + c];
-  return make_float2(v.x, -v.y);
-}
-__device__ __forceinline__ void herm_set(float2 *A, int ld, int r, int c, float2 v) {
-  if (r <= c) A[c * ld + r] = v;
-  else A[r * ld + c] = make_float2(v.x, -v.y);
+// LDS scratch of the Jacobi solver (carved by carve()).
+struct JacobiLds {
+  Rot *rot;    // [2][32]: rotations of two consecutive rounds
+  int2 *rpq;   // [2][32]: their pairs (p < q)
+  float *red;  // block reductions
+};
+
+// Positions (p < q) of pair k in round r.
+__device__ __forceinline__ int2 rr_pair(int k, int r, int ne) {
+  const int a = rr_pos(k, r, ne), b = rr_pos(ne - 1 - k, r, ne);
+  return make_int2(min(a, b), max(a, b));
 }
 
-// Cyclic Jacobi on the Hermitian matrix A (column-major, leading dim n) in LDS;
-// V receives the eigenvectors (columns).  Each round rotates n/2 disjoint pairs
-// (round-robin order); the update visits only blocks kp <= kq and keeps only
-// the upper triangle current (the lower one goes stale during the sweeps).
-// Returns the number of sweeps.
+// Entry (r, c) of a Hermitian matrix kept as its upper triangle, without branches.
+__device__ __forceinline__ float2 herm_load(const float2 *A, int ld, int r, int c) {
+  const bool up = r <= c;
+  const float2 v = A[up ? c * ld + r : r * ld + c];
+  return make_float2(v.x, up ? v.y : -v.y);
+}
+__device__ __forceinline__ void herm_store(float2 *A, int ld, int r, int c, float2 v) {
+  const bool up = r <= c;
+  A[up ? c * ld + r : r * ld + c] = make_float2(v.x, up ? v.y : -v.y);
+}
+
+// Cyclic Jacobi on the Hermitian matrix A (column-major, leading dim n | 1,
+// upper triangle; the lower one is never read and goes stale) in LDS; V
+// receives the eigenvectors (columns).  Each round rotates n/2 disjoint pairs
+// (round-robin order) in two phases:
+//   Y: all threads update the 2x2 blocks kp <= kq, A <- J^H A J;
+//   X: wave 0 computes the next round's rotations (FP64 angles, lanes
+//      0..h-1) from the updated matrix while the other waves apply this
+//      round's rotations to the eigenvectors, V <- V J, so the angle chain's
+//      latency hides behind the eigenvector work instead of stalling it.
+// The rotation and pair tables are double-buffered by round.  Returns the
+// number of sweeps.
 template <int NTHR>
 __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol,
-                          int2 *rpq, Rot *rot, float *red, bool init_v = true) {
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int ne = n + (n & 1), h = ne / 2;
+                                          const JacobiLds &L, bool init_v = true) {
+  constexpr int NW = NTHR / 64;
+  const int tid = threadIdx.x, w = tid >> 6;
+  const int ne = n + (n & 1), h = ne / 2, nr = ne - 1;
   const int nblk = h * (h + 1) / 2;
   const float inv_n = 1.0f / (float)n;
   const int ld = n | 1;
-  for (int i = tid; i < n * n; i += nthr) {
+  for (int i = tid; i < n * n; i += NTHR) {
     const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
     if (init_v) V[c * ld + r] = make_float2(r == c ? 1.0f : 0.0f, 0.0f);
     if (r == c) A[c * ld + r].y = 0.0f;
@@ -186,14 +206,56 @@ __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_s
       nmine = j + 1;
     }
   }
-  // eigenvector update: threads grouped by pair, vrg consecutive threads per pair
-  const int vrg = nthr >= h ? nthr / h : 1;
-  const int vk = tid / vrg, vr0 = tid - vk * vrg;
+  // eigenvector update: the threads of waves 1.. (all of them with one wave),
+  // grouped by pair, vrg consecutive threads per pair, rows vr0, vr0 + vrg, ...
+  const int v0 = NW > 1 ? 64 : 0, nvt = NTHR - v0, vt = tid - v0;
+  const int vrg = nvt >= h ? nvt / h : 1;
+  const int vk = vt >= 0 ? vt / vrg : h, vr0 = vt >= 0 ? vt - vk * vrg : 0;
+  const bool vthread = vt >= 0 && vk < h;
+  // rotations of round r into table slot sl (lanes 0..h-1 of wave 0)
+  auto rotations = [&](int r, int sl) {
+    if (tid < h) {
+      const int2 P = rr_pair(tid, r, ne);
+      Rot R{1.0f, 0.0f, 1.0f, 0.0f};
+      if (P.y < n) R = make_rotation(A[P.y * ld + P.x], A[P.x * ld + P.x].x, A[P.y * ld + P.y].x);
+      L.rpq[sl * 32 + tid] = P;
+      L.rot[sl * 32 + tid] = R;
+    }
+  };
+  // V <- V J for this thread's items of the round in slot sl, 4 items per
+  // batch (all loads of a batch before its stores)
+  auto vupdate = [&](int sl) {
+    if (!vthread) return;
+    const int2 Q = L.rpq[sl * 32 + vk];
+    if (Q.y >= n) return;
+    const Rot RQ = L.rot[sl * 32 + vk];
+    const float2 eb = make_float2(RQ.er, -RQ.ei);
+    for (int row0 = vr0; row0 < n; row0 += 4 * vrg) {
+      float2 vp[4], vq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int row = min(row0 + u * vrg, n - 1);
+        vp[u] = V[Q.x * ld + row];
+        vq[u] = V[Q.y * ld + row];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int row = row0 + u * vrg;
+        const float2 y = cmul(eb, vq[u]);
+        if (row < n) {
+          V[Q.x * ld + row] = make_float2(RQ.c * vp[u].x - RQ.s * y.x, RQ.c * vp[u].y - RQ.s * y.y);
+          V[Q.y * ld + row] = make_float2(RQ.s * vp[u].x + RQ.c * y.x, RQ.s * vp[u].y + RQ.c * y.y);
+        }
+      }
+    }
+  };
   __syncthreads();
-  int sweep = 0;
+  rotations(0, 0);
+  __syncthreads();
+  int g = 0, sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
     float off = 0.0f, tot = 0.0f;
-    for (int i = tid; i < n * n; i += nthr) {  // upper triangle; off-diagonal entries count twice
+    for (int i = tid; i < n * n; i += NTHR) {  // upper triangle; off-diagonal entries count twice
       const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
       if (r > c) continue;
       const float2 a = A[c * ld + r];
@@ -201,36 +263,28 @@ __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_s
       tot += r == c ? m : 2.0f * m;
       if (r != c) off += 2.0f * m;
     }
-    off = block_sum(off, red);
-    tot = block_sum(tot, red);
+    off = block_sum(off, L.red);
+    tot = block_sum(tot, L.red);
     if (off == 0.0f || off <= tol * tol * tot) break;
-    for (int r = 0; r < ne - 1; ++r) {
-      if (tid < h) {
-        int p = rr_pos(tid, r, ne), q = rr_pos(ne - 1 - tid, r, ne);
-        if (p > q) {
-          const int t = p;
-          p = q;
-          q = t;
-        }
-        Rot R{1.0f, 0.0f, 1.0f, 0.0f};
-        if (q < n) R = make_rotation(A[q * ld + p], A[p * ld + p].x, A[q * ld + q].x);
-        rpq[tid] = make_int2(p, q);
-        rot[tid] = R;
-      }
-      __syncthreads();
-      // A <- J^H A J on blocks (kp <= kq), written to the upper triangle
+    for (int r = 0; r < nr; ++r, ++g) {
+      const int sl = g & 1;
+      // Y: A <- J^H A J on blocks (kp <= kq), written with their transposes
+      const int2 *pq = L.rpq + sl * 32;
+      const Rot *rt = L.rot + sl * 32;
 #pragma unroll
       for (int j = 0; j < kMaxBlk; ++j) {
         if (j >= nmine) continue;
         const int code = blk[j];
         const int kp = code & 0xff, kq = code >> 8;
-        const int2 P = rpq[kp], Q = rpq[kq];
-        const Rot RP = rot[kp], RQ = rot[kq];
+        const int2 P = pq[kp], Q = pq[kq];
+        const Rot RP = rt[kp], RQ = rt[kq];
         const bool pv = P.y < n, qv = Q.y < n;  // the second index of a pair may be the dummy
-        float2 x00 = herm_get(A, ld, P.x, Q.x);
-        float2 x01 = qv ? herm_get(A, ld, P.x, Q.y) : make_float2(0.f, 0.f);
-        float2 x10 = pv ? herm_get(A, ld, P.y, Q.x) : make_float2(0.f, 0.f);
-        float2 x11 = (pv && qv) ? herm_get(A, ld, P.y, Q.y) : make_float2(0.f, 0.f);
+        float2 x00 = herm_load(A, ld, P.x, Q.x);
+        float2 x01 = herm_load(A, ld, P.x, qv ? Q.y : Q.x);
+        float2 x10 = herm_load(A, ld, pv ? P.y : P.x, Q.x);
+        float2 x11 = herm_load(A, ld, pv ? P.y : P.x, qv ? Q.y : Q.x);
+        if (!qv) x01 = x11 = make_float2(0.f, 0.f);
+        if (!pv) x10 = x11 = make_float2(0.f, 0.f);
         rotate_block(RP, RQ, x00, x01, x10, x11);
         if (kp == kq) {  // the annihilated pair: exact zeros, real diagonal (P.x < P.y)
           x00.y = 0.f;
@@ -241,25 +295,22 @@ __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_s
             A[P.y * ld + P.y] = x11;
           }
         } else {
-          herm_set(A, ld, P.x, Q.x, x00);
-          if (qv) herm_set(A, ld, P.x, Q.y, x01);
-          if (pv) herm_set(A, ld, P.y, Q.x, x10);
-          if (pv && qv) herm_set(A, ld, P.y, Q.y, x11);
+          herm_store(A, ld, P.x, Q.x, x00);
+          if (qv) herm_store(A, ld, P.x, Q.y, x01);
+          if (pv) herm_store(A, ld, P.y, Q.x, x10);
+          if (pv && qv) herm_store(A, ld, P.y, Q.y, x11);
         }
       }
-      // V <- V J: this thread's pair k (rotation read once), rows r0, r0 + vrg, ...
-      if (vk < h) {
-        const int2 Q = rpq[vk];
-        if (Q.y < n) {
-          const Rot RQ = rot[vk];
-          const float2 eb = make_float2(RQ.er, -RQ.ei);
-          for (int row = vr0; row < n; row += vrg) {
-            const float2 vp = V[Q.x * ld + row], vq = V[Q.y * ld + row];
-            const float2 y = cmul(eb, vq);
-            V[Q.x * ld + row] = make_float2(RQ.c * vp.x - RQ.s * y.x, RQ.c * vp.y - RQ.s * y.y);
-            V[Q.y * ld + row] = make_float2(RQ.s * vp.x + RQ.c * y.x, RQ.s * vp.y + RQ.c * y.y);
-          }
-        }
+      __syncthreads();
+      // X: next round's rotations (wave 0) beside this round's V <- V J
+      const int r1 = r + 1 < nr ? r + 1 : 0;
+      if (NW == 1) {
+        rotations(r1, sl ^ 1);
+        vupdate(sl);
+      } else if (w == 0) {
+        rotations(r1, sl ^ 1);
+      } else {
+        vupdate(sl);
       }
       __syncthreads();
     }
@@ -298,9 +349,12 @@ struct EigSmem {
   float2 *A0, *Y, *hv, *t1, *t2;  // warm start: copy of A, proposal, reflectors, scratch
   float *ev, *d, *f, *red, *htau, *res;
   int *perm;
-  int2 *rpq;
-  Rot *rot;
+  int2 *rpq;                      // small (warm-start) Jacobi: pairs (shares pivo)
+  Rot *rot;                       // small (warm-start) Jacobi: rotations (shares rotw)
+  Rot *rot2;                      // Jacobi solver: rotations [2][32]
+  int2 *rpq2;                     // Jacobi solver: pairs [2][32]
   int *kdet, *mark;
+  __device__ JacobiLds jl() const { return JacobiLds{rot2, rpq2, red}; }
 };
 
 // LDS layout.  The warm-start buffers exist only in the warm kernel; its copy
@@ -309,7 +363,7 @@ struct EigSmem {
 // been copied out of V) reuses V: 32.7 KB instead of 37.3 KB at n = 36,
 // B = 16, so three eigen workgroups leave room on a CU for a voltage
 // workgroup of the neighbouring blocks (55 KB) instead of blocking it.
-__device__ __forceinline__ EigSmem carve(char *base, int n, int B, bool warm) {
+__device__ __forceinline__ EigSmem carve(char *base, int n, int B, bool warm, int nw) {
   EigSmem s;
   const int ld = n | 1;  // odd column stride of A: conflict-free strided LDS access
   s.A = reinterpret_cast<float2 *>(base);
@@ -341,19 +395,23 @@ __device__ __forceinline__ EigSmem carve(char *base, int n, int B, bool warm) {
   s.res = fp + 216;  // kMaxWarm * 64 (warm only)
   s.perm = reinterpret_cast<int *>(s.res + (warm ? kMaxWarm * 64 : 0));
   s.mark = s.perm + 64;
-  s.rpq = reinterpret_cast<int2 *>(s.mark + 64);
-  s.rot = reinterpret_cast<Rot *>(s.rpq + 32);
-  s.kdet = reinterpret_cast<int *>(s.rot + 32);
+  s.rot2 = reinterpret_cast<Rot *>((reinterpret_cast<uintptr_t>(s.mark + 64) + 15) & ~uintptr_t(15));
+  s.rpq2 = reinterpret_cast<int2 *>(s.rot2 + 64);
+  s.kdet = reinterpret_cast<int *>(s.rpq2 + 64);
+  // the warm start's small Jacobi (wave 0, never beside the solver) reuses them
+  s.rot = s.rot2;
+  s.rpq = s.rpq2;
+  (void)nw;
   return s;
 }
 
-size_t eig_smem_bytes(int n, int B, bool warm) {
+size_t eig_smem_bytes(int n, int B, bool warm, int nw) {
   const int ld = n | 1;
   const bool g_in_v = !warm && B <= ld;  // as in carve()
   size_t f2 = 2 * (size_t)ld * n + (g_in_v ? 2 : 3) * (size_t)B * n;
   if (warm) f2 += (3 * B * n >= ld * n ? 0 : (size_t)ld * n) + 2 * kMaxWarm * n + 128;
   return sizeof(float2) * f2 + sizeof(float) * (216 + (warm ? kMaxWarm * 64 : 0)) + sizeof(int) * 128 +
-         sizeof(int2) * 32 + sizeof(Rot) * 32 + 16;
+         15 + sizeof(Rot) * 64 + sizeof(int2) * 64 + sizeof(int) * 4 + 0 * nw;
 }
 
 // Sum over the 64 lanes of a wavefront with DPP lane moves (row_shr 1/2/4/8
@@ -764,7 +822,7 @@ template <int NTHR, bool WARM>
 __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   extern __shared__ __align__(16) char smem_raw[];
   const int n = P.R, B = P.B;
-  EigSmem S = carve(smem_raw, n, B, WARM);
+  EigSmem S = carve(smem_raw, n, B, WARM, NTHR / 64);
   const int c = blockIdx.x >> 1, p = blockIdx.x & 1;
   const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1;
   const long nbl = (long)n * (n + 1) / 2;
@@ -818,13 +876,13 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
     } else {
       // rejected: the plain solver on the same matrix (the warm solve leaves A
       // untouched), in this workgroup, so no second launch is needed
-      sweeps += jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
+      sweeps += jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.jl());
       sort_eigen(S.A, n, S.perm, S.ev);
       K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
       fell = 1;
     }
   } else {
-    sweeps = jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
+    sweeps = jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.jl());
     sort_eigen(S.A, n, S.perm, S.ev);
     K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
   }
@@ -973,7 +1031,7 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
 
 template <int NTHR, bool WARM>
 hipError_t launch_eig_t(const EigParams &p, hipStream_t s) {
-  const size_t sm = eig_smem_bytes(p.R, p.B, WARM);
+  const size_t sm = eig_smem_bytes(p.R, p.B, WARM, NTHR / 64);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eig_null_kernel<NTHR, WARM>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -986,10 +1044,8 @@ hipError_t launch_eig_t(const EigParams &p, hipStream_t s) {
 hipError_t launch_eig_plain(const EigParams &p, int nthr, hipStream_t s) {
   switch (nthr) {
     case 64: return launch_eig_t<64, false>(p, s);
-    case 128: return launch_eig_t<128, false>(p, s);
-    case 512: return launch_eig_t<512, false>(p, s);
-    case 1024: return launch_eig_t<1024, false>(p, s);
-    default: return launch_eig_t<256, false>(p, s);
+    case 256: return launch_eig_t<256, false>(p, s);
+    default: return launch_eig_t<128, false>(p, s);
   }
 }
 
@@ -1043,7 +1099,7 @@ __global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P)
       __syncthreads();
       continue;
     }
-    EigSmem S = carve(smem_raw, n, 0, false);
+    EigSmem S = carve(smem_raw, n, 0, false, kEigThreads / 64);
     const int ld = n | 1;
     const int nb = n * (n + 1) / 2;
     for (int idx = tid; idx < nb; idx += nthr) {  // packed (i <= j) of the subarray
@@ -1073,7 +1129,7 @@ __global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P)
       S.A[col * ld + row].y /= den;
     }
     __syncthreads();
-    const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, P.max_sweeps, P.tol, S.rpq, S.rot, S.red);
+    const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, P.max_sweeps, P.tol, S.jl());
     sort_eigen(S.A, n, S.perm, S.ev);
     const int K = max(0, min(n, P.k[q]));
     // unwhitened, normalised eigenvectors (ascending) into A (stride n)
@@ -1120,7 +1176,7 @@ __global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P)
 
 hipError_t launch_subarray_null(const SubParams &p, int num_cus, hipStream_t s) {
   // LDS for the largest group; as many workgroups as fit on the machine at once
-  const size_t sm = eig_smem_bytes(p.max_n, 0, false);
+  const size_t sm = eig_smem_bytes(p.max_n, 0, false, kEigThreads / 64);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(subarray_null_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -1138,7 +1194,7 @@ __global__ void __launch_bounds__(kEigThreads)
 eigh_batched_kernel(const float2 *Ain, int n, int whiten, int max_sweeps, float tol,
                     float *evals, float2 *evecs, int *sweeps_out) {
   extern __shared__ __align__(16) char smem_raw[];
-  EigSmem S = carve(smem_raw, n, 0, false);
+  EigSmem S = carve(smem_raw, n, 0, false, kEigThreads / 64);
   const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
   for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * ld + i % n] = Ain[(long)m * n * n + i];
   __syncthreads();
@@ -1153,7 +1209,7 @@ eigh_batched_kernel(const float2 *Ain, int n, int whiten, int max_sweeps, float 
     }
     __syncthreads();
   }
-  const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.rpq, S.rot, S.red);
+  const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.jl());
   sort_eigen(S.A, n, S.perm, S.ev);
   if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
   for (int i = tid; i < n * n; i += blockDim.x) {
@@ -1178,7 +1234,7 @@ __global__ void __launch_bounds__(kEigThreads)
 pol_eig_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *gains, const float2 *phase,
                int max_sweeps, float tol, float *evals, float2 *evecs, int *sweeps_out) {
   extern __shared__ __align__(16) char smem_raw[];
-  EigSmem S = carve(smem_raw, n, 1, false);
+  EigSmem S = carve(smem_raw, n, 1, false, kEigThreads / 64);
   const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
   const int c = m >> 2, pq = m & 3, p = pq >> 1, q = pq & 1;
   const int nbl = n * (n + 1) / 2;
@@ -1200,7 +1256,7 @@ pol_eig_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *gain
     S.A[j * ld + i] = cmul(f, make_float2((float)re, (float)im));
   }
   __syncthreads();
-  const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.rpq, S.rot, S.red);
+  const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.jl());
   sort_eigen(S.A, n, S.perm, S.ev);
   float2 *ph = S.W;  // per output column: conj(v_max) / |v_max|
   if (tid < n) {
@@ -1237,7 +1293,7 @@ __global__ void __launch_bounds__(kEigThreads)
 proj_block_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *gains, const float2 *phase,
                   int K, int max_sweeps, float tol, float2 *acc) {
   extern __shared__ __align__(16) char smem_raw[];
-  EigSmem S = carve(smem_raw, n, 1, false);
+  EigSmem S = carve(smem_raw, n, 1, false, kEigThreads / 64);
   const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
   const int c = m >> 2, pq = m & 3, p = pq >> 1, q = pq & 1;
   const int nbl = n * (n + 1) / 2;
@@ -1259,7 +1315,7 @@ proj_block_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *g
     S.A[j * ld + i] = cmul(f, make_float2((float)re, (float)im));
   }
   __syncthreads();
-  jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.rpq, S.rot, S.red);
+  jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.jl());
   sort_eigen(S.A, n, S.perm, S.ev);
   float2 *out = acc + (long)m * n * n;
   for (int e = tid; e < n * n; e += blockDim.x) {
@@ -1284,7 +1340,7 @@ proj_block_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *g
 __global__ void __launch_bounds__(kEigThreads)
 proj_dump_kernel(float2 *acc, int n, float inv_runs, int max_sweeps, float tol, float *evals, float2 *evecs) {
   extern __shared__ __align__(16) char smem_raw[];
-  EigSmem S = carve(smem_raw, n, 1, false);
+  EigSmem S = carve(smem_raw, n, 1, false, kEigThreads / 64);
   const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
   float2 *a = acc + (long)m * n * n;
   for (int e = tid; e < n * n; e += blockDim.x) {
@@ -1294,7 +1350,7 @@ proj_dump_kernel(float2 *acc, int n, float inv_runs, int max_sweeps, float tol, 
     if (i <= j) S.A[j * ld + i] = make_float2(v.x * inv_runs, v.y * inv_runs);
   }
   __syncthreads();
-  jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.rpq, S.rot, S.red);
+  jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.jl());
   sort_eigen(S.A, n, S.perm, S.ev);
   float2 *ph = S.W;
   if (tid < n) {
@@ -1322,7 +1378,7 @@ proj_dump_kernel(float2 *acc, int n, float inv_runs, int max_sweeps, float tol, 
 
 hipError_t launch_proj_block(const int *vis_part, int nsplit, int C, int n, const float2 *gains,
                              const float2 *phase, int K, int max_sweeps, float tol, float *acc, hipStream_t s) {
-  const size_t sm = eig_smem_bytes(n, 1, false);
+  const size_t sm = eig_smem_bytes(n, 1, false, kEigThreads / 64);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(proj_block_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -1335,7 +1391,7 @@ hipError_t launch_proj_block(const int *vis_part, int nsplit, int C, int n, cons
 
 hipError_t launch_proj_dump(float *acc, int C, int n, int runs, int max_sweeps, float tol, float *evals,
                             float *evecs, hipStream_t s) {
-  const size_t sm = eig_smem_bytes(n, 1, false);
+  const size_t sm = eig_smem_bytes(n, 1, false, kEigThreads / 64);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(proj_dump_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -1349,7 +1405,7 @@ hipError_t launch_proj_dump(float *acc, int C, int n, int runs, int max_sweeps, 
 hipError_t launch_pol_eig(const int *vis_part, int nsplit, int C, int n, const float2 *gains,
                           const float2 *phase, int max_sweeps, float tol, float *evals, float *evecs,
                           int *sweeps, hipStream_t s) {
-  const size_t sm = eig_smem_bytes(n, 1, false);
+  const size_t sm = eig_smem_bytes(n, 1, false, kEigThreads / 64);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(pol_eig_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -1367,7 +1423,7 @@ __global__ void __launch_bounds__(kEigThreads)
 eigh_warm_kernel(const float2 *Ain, const float2 *prop, int n, int KW, int K, int steps, float tol,
                  float *evals, float2 *evecs, float *ratio) {
   extern __shared__ __align__(16) char smem_raw[];
-  EigSmem S = carve(smem_raw, n, 0, true);
+  EigSmem S = carve(smem_raw, n, 0, true, kEigThreads / 64);
   const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
   for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * ld + i % n] = Ain[(long)m * n * n + i];
   for (int e = tid; e < KW * n; e += blockDim.x) S.Y[e] = prop[(long)m * KW * n + e];
@@ -1384,7 +1440,7 @@ eigh_warm_kernel(const float2 *Ain, const float2 *prop, int n, int KW, int K, in
 hipError_t launch_eigh_warm(const float *A, const float *prop, int n, int nmat, int KW, int K,
                             int steps, float tol, float *evals, float *evecs, float *ratio,
                             hipStream_t s) {
-  const size_t sm = eig_smem_bytes(n, 0, true);
+  const size_t sm = eig_smem_bytes(n, 0, true, kEigThreads / 64);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eigh_warm_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -1399,7 +1455,7 @@ hipError_t launch_eigh_warm(const float *A, const float *prop, int n, int nmat, 
 hipError_t launch_eigh_batched(const float *A, int n, int nmat, int whiten, int max_sweeps,
                                float tol, float *evals, float *evecs, int *sweeps,
                                hipStream_t s) {
-  const size_t sm = eig_smem_bytes(n, 0, false);
+  const size_t sm = eig_smem_bytes(n, 0, false, kEigThreads / 64);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eigh_batched_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);

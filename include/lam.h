@@ -267,8 +267,9 @@ int lam_set_tuning(lam_ctx *ctx, int samples_per_wg);
 enum { LAM_WPREC_F16_HILO = 0, LAM_WPREC_F16 = 1, LAM_WPREC_BF16 = 2 };
 int lam_set_weight_precision(lam_ctx *ctx, int precision);
 
-/* Workgroup width (64, 128, 256, 512 or 1024 threads) of the plain Jacobi
- * launch. */
+/* Workgroup width (64, 128 or 256 threads: one to four wavefronts per
+ * matrix) of the plain Jacobi launch; 256 by default.  Results are
+ * identical bit for bit across launches of the same width. */
 int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads);
 
 /* Autotuner (no counterpart in the reference): times this context's kernels
@@ -277,7 +278,7 @@ int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads);
  *   - time samples per voltage workgroup: automatic, 1024, 2048, 3072, 4096,
  *     8192, judged on the whole block sequence (more splits speed the voltage
  *     kernels but multiply the partial visibilities the eigen stage reads);
- *   - plain Jacobi workgroup width: 256, 512, 1024 (adaptive mode);
+ *   - plain Jacobi workgroup width: 64, 128, 256 (adaptive mode);
  *   with several buffers both are judged on streaming time per block (blocks
  *   overlapping, at least 16 timed passes), with one buffer on kernel time;
  *   - warm-start refinement steps 1..3 when LAM_EIG_LEARNED / PREVIOUS is set
