@@ -816,6 +816,94 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
   return S.kdet[1] != 0;
 }
 
+// ---- compensated FP32 ("double-float") arithmetic: value = hi + lo ----
+struct DF {
+  float hi, lo;
+};
+__device__ __forceinline__ DF df_two_sum(float a, float b) {
+  const float s = a + b, bb = s - a;
+  return DF{s, (a - (s - bb)) + (b - bb)};
+}
+__device__ __forceinline__ DF df_add(DF a, DF b) {
+  const DF s = df_two_sum(a.hi, b.hi);
+  const float e = s.lo + (a.lo + b.lo);
+  const float hi = s.hi + e;
+  return DF{hi, e - (hi - s.hi)};
+}
+// a * x for a double-float a and a float x: the hi product exact through FMA
+__device__ __forceinline__ DF df_mul(DF a, float x) {
+  const float p = a.hi * x;
+  return DF{p, fmaf(a.hi, x, -p) + a.lo * x};
+}
+__device__ __forceinline__ DF df_neg(DF a) { return DF{-a.hi, -a.lo}; }
+// an integer below 2^48 in magnitude, exactly
+__device__ __forceinline__ DF df_from_i64(long long v) {
+  const float hi = (float)v;
+  return DF{hi, (float)(v - (long long)hi)};
+}
+
+// Eigenvalues of the K removed modes as Rayleigh quotients of their computed
+// (whitened) eigenvectors u: lambda = u^H S A S u / u^H u with A the block's
+// exact integer covariance and S = diag(1 / d) the whitening, in compensated
+// FP32.  The FP32 Jacobi (or Rayleigh-Ritz) eigenvalue of a weak mode carries
+// the absolute error eps |A_w| of the FP32 whitened matrix, large relative to
+// the mode itself when a strong interferer sets |A_w|; the quotient's error is
+// second order in the eigenvector error (|r|^2 / gap) and the products of the
+// matrix-vector product are exact, so its sums keep the small mode.  The
+// diagonal scaling in FP32 perturbs the quotient only relatively (O(eps)).
+// Runs between the eigen solve and the outputs (S.A, whose diagonal the sort
+// has already copied, holds the per-row partial quotients); overwrites S.ev
+// of the top K.
+__device__ __forceinline__ void refine_removed_eigenvalues(const EigParams &P, EigSmem &S, int c, int p, int n,
+                                                           int K) {
+  const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1;
+  const long nbl = (long)n * (n + 1) / 2;
+  float2 *zb = S.A;  // [kk][i]: Re(conj(x_i) (A x)_i) of mode kk as (hi, lo); K * n <= ld * n
+  // per (mode, row): x = S u, (A x)_i from the exact covariance (read again
+  // from the partials), and the row's term of the quotient
+  for (int e = tid; e < K * n; e += nthr) {
+    const int kk = e / n, i = e - kk * n;
+    const float2 *u = S.V + S.perm[n - 1 - kk] * ld;
+    DF yr{0.f, 0.f}, yi{0.f, 0.f};
+    for (int j = 0; j < n; ++j) {
+      const int lo = min(i, j), hi = max(i, j);
+      const long bl = (long)hi * (hi + 1) / 2 + lo;
+      long long re = 0, im = 0;
+      for (int k = 0; k < P.nsplit; ++k) {
+        const int2 v = reinterpret_cast<const int2 *>(P.vis_part)[(((long)k * P.C + c) * 4 + 3 * p) * nbl + bl];
+        re += v.x;
+        im += v.y;
+      }
+      if (i > j) im = -im;  // A(i, j) = conj(A(j, i)) below the diagonal
+      const float sj = P.whiten ? 1.0f / S.d[j] : 1.0f;
+      const float xr = u[j].x * sj, xi = u[j].y * sj;
+      const DF ar = df_from_i64(re), ai = df_from_i64(im);
+      // (ar + i ai)(xr + i xi)
+      yr = df_add(yr, df_mul(ar, xr));
+      yr = df_add(yr, df_neg(df_mul(ai, xi)));
+      yi = df_add(yi, df_mul(ar, xi));
+      yi = df_add(yi, df_mul(ai, xr));
+    }
+    const float si = P.whiten ? 1.0f / S.d[i] : 1.0f;
+    const DF z = df_add(df_mul(yr, u[i].x * si), df_mul(yi, u[i].y * si));  // Re(conj(x_i) y_i)
+    zb[e] = make_float2(z.hi, z.lo);
+  }
+  __syncthreads();
+  // lambda = sum_i Re(conj(x_i) y_i) / |u|^2
+  for (int kk = tid; kk < K; kk += nthr) {
+    const float2 *u = S.V + S.perm[n - 1 - kk] * ld;
+    DF acc{0.f, 0.f};
+    float nrm = 0.0f;
+    for (int i = 0; i < n; ++i) {
+      const float2 z = zb[kk * n + i];
+      acc = df_add(acc, DF{z.x, z.y});
+      nrm += u[i].x * u[i].x + u[i].y * u[i].y;
+    }
+    S.ev[n - 1 - kk] = (acc.hi + acc.lo) / nrm;
+  }
+  __syncthreads();
+}
+
 // WARM = false: plain Jacobi (and the second launch that re-solves deferred
 // problems); WARM = true: warm-start solve, rejected problems are deferred.
 template <int NTHR, bool WARM>
@@ -886,6 +974,8 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
     sort_eigen(S.A, n, S.perm, S.ev);
     K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
   }
+  // 4. eigenvalues of the removed modes to the precision of the exact covariance
+  if (K > 0) refine_removed_eigenvalues(P, S, c, p, n, K);
   // modes clearly above the noise floor (> 4x the mean of the remaining eigenvalues):
   // the label count of the warm-start training set
   if (tid == 0 && P.nsig) {

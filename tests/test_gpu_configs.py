@@ -1,0 +1,154 @@
+"""GPU tests at the benchmark configurations' own sizes (384 channels x 2 pols
+x 36 antennas in the 64-receiver layout, 8192 samples per block), against
+the CPU oracle and LAPACK.
+
+Bars: visibilities bit-exact (integers); beams within 1e-3 normwise relative
+error of the FP64 oracle (FP16 hi+lo weights); eigenvalues of the nulled
+subspace within 1e-5 relative of LAPACK (numpy eigvalsh) on the FP64
+whitened covariance of the block's exact visibilities, on every (channel,
+pol); the other eigenvalues within 1e-4 of each matrix's largest."""
+import sys
+
+import numpy as np
+import pytest
+
+import lam
+import lam_synth as syn
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+C, R, NP, B = 384, 36, 128, 16
+SHAPE = syn.BlockShape(C=C, F=1, NT=64, R=R, P=2, RPP=R, NP=NP, B=B)
+OSHAPE = orc.Shape(C=C, F=1, NT=64, R=R, P=2, RPP=R, NP=NP, B=B)
+EIG_TOL = 1e-5
+BEAM_TOL = 1e-3
+
+
+def _log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _synth(n_src, inr_db, seed, block=0, drift=0.0):
+    sc = syn.DriftScenario(C, R, seed=seed, n_blocks=block + 1, n_src=(n_src, n_src), inr_db=inr_db,
+                           drift=drift, p_switch=0.0, max_sources=n_src)
+    tones, st, gain, _, _ = sc.block(block)
+    dr = lam.DeviceArray(SHAPE.ring_shape(), np.int8)
+    ds = lam.DeviceArray(SHAPE.scale_shape(), np.int16)
+    dt, dst, dg = (lam.DeviceArray.from_numpy(a) for a in (tones, st, gain))
+    lam.synth_block(SHAPE, seed, block, 16.0, dt, dst, dg, n_src, dr, ds)
+    return dr, ds, tones
+
+
+def _whitened(vis):
+    """FP64 whitened XX / YY covariances [C][2][R][R] of exact visibilities."""
+    Vc = vis[..., 0].astype(np.float64) + 1j * vis[..., 1].astype(np.float64)
+    ii, jj = np.triu_indices(R)
+    bl = jj * (jj + 1) // 2 + ii
+    A = np.zeros((C, 2, R, R), np.complex128)
+    for p in range(2):
+        A[:, p, ii, jj] = Vc[:, bl, p, p]
+        A[:, p, jj, ii] = np.conj(Vc[:, bl, p, p])
+    d = np.sqrt(np.maximum(np.einsum("cpii->cpi", A).real, 1e-10))
+    return A / (d[..., :, None] * d[..., None, :])
+
+
+def _run(dr, ds, w16, K, wprec=lam.WPREC_F16_HILO):
+    p = lam.Pipeline(SHAPE, lam.MODE_ADAPTIVE)
+    p.set_weights(w16)
+    p.set_fixed_k(np.full((C, 2), K, np.int32))
+    p.set_eig_params(12, 1e-6)
+    p.set_weight_precision(wprec)
+    beams = p.beams_array()
+    evals = np.zeros((C, 2, R), np.float32)
+    evecs = np.zeros((C, 2, R, R, 2), np.float32)
+    p.execute(dr, ds, beams_out=beams, evals_out=evals, evecs_out=evecs, on_device=True)
+    p.sync()
+    vis = p.dump_visibilities_i64()
+    orth = p.eig_stats(with_orthogonality=True)[2]
+    p.close()
+    return beams, evals, evecs, vis, orth
+
+
+def _complex(beams):
+    return beams[..., 0].astype(np.float32) + 1j * beams[..., 1].astype(np.float32)
+
+
+def test_cfg3_nulled_eigenvalues_every_channel():
+    """3 interferers per channel at INR 0-30 dB, rank-3 null, each weight
+    precision: the 3 removed eigenvalues of all 768 covariances within 1e-5
+    of LAPACK and of the oracle's FP64 Jacobi; null depth from the beams."""
+    dr, ds, tones = _synth(3, (0.0, 30.0), seed=20261203 + 3)
+    w16, _ = syn.random_weights(SHAPE, seed=20261203 + 4)
+    T = SHAPE.T
+    ph = np.exp(-2j * np.pi * tones[..., 1].astype(np.float64)[..., None] * np.arange(T))  # [C][S][T]
+    ref = None
+    for name, mode in (("f16_hilo", lam.WPREC_F16_HILO), ("f16", lam.WPREC_F16), ("bf16", lam.WPREC_BF16)):
+        _log(f"cfg3 {name}")
+        beams, evals, _, vis, orth = _run(dr, ds, w16, 3, mode)
+        if ref is None:
+            Aw = _whitened(vis)
+            ref = np.linalg.eigvalsh(Aw)
+            # the oracle's FP64 Jacobi on a sample of the matrices agrees with LAPACK
+            for c in range(0, C, 48):
+                ev_o, _, _ = orc.jacobi_eigh(Aw[c, 1])
+                assert np.max(np.abs(ev_o - ref[c, 1])) <= 1e-12 * ref[c, 1].max()
+        top = np.abs(evals[..., -3:] - ref[..., -3:]) / np.abs(ref[..., -3:])
+        rest = np.abs(evals[..., :-3] - ref[..., :-3]) / ref[..., -1:]
+        depth = np.zeros((C, 2))
+        for c0 in range(0, C, 32):  # known tones' amplitudes in the original and the nulled beams
+            amp = np.einsum("cpbt,cst->cpbs", _complex(beams[c0:c0 + 32]), ph[c0:c0 + 32].astype(np.complex64)) / T
+            orig = np.sum(np.abs(amp[:, :, :B]) ** 2, axis=(2, 3))
+            mitig = np.sum(np.abs(amp[:, :, B:]) ** 2, axis=(2, 3))
+            depth[c0:c0 + 32] = 10 * np.log10(np.maximum(mitig, 1e-30) / orig)
+        _log(f"  top-3 eigenvalue rel err max {top.max():.2e}, others {rest.max():.2e}, "
+             f"null depth median {np.median(depth):.1f} dB, orthogonality {orth.max():.1e}")
+        assert top.max() <= EIG_TOL, (name, top.max(), np.unravel_index(np.argmax(top), top.shape))
+        assert rest.max() <= 1e-4
+        assert orth.max() < 1e-5
+        assert np.median(depth) < -40.0
+
+
+def test_cfg2_full_band_exact_and_sampled_beams():
+    """cfg2: one CW tone per channel, 16 -> 32 beams, rank-1 null, visibilities
+    accumulated over 4 blocks: visibilities bit-exact for every channel (one
+    block) and for 16 sampled channels (4 blocks), beams and eigenvalues
+    against the oracle on the sampled channels."""
+    w16, w = syn.random_weights(SHAPE, seed=20261203 + 2)
+    blocks = [_synth(1, (10.0, 30.0), seed=20261203 + 2, block=b, drift=0.01) for b in range(4)]
+    p = lam.Pipeline(SHAPE, lam.MODE_ADAPTIVE, num_buffers=3)
+    p.set_weights(w16)
+    p.set_fixed_k(np.ones((C, 2), np.int32))
+    p.set_eig_params(12, 1e-6)
+    beams = p.beams_array()
+    evals = np.zeros((C, 2, R), np.float32)
+    p.execute(blocks[0][0], blocks[0][1], beams_out=beams, evals_out=evals, on_device=True)
+    p.sync()
+    vis1 = p.dump_visibilities_i64()
+    for dr, ds, _ in blocks:
+        p.execute(dr, ds, on_device=True)
+    p.sync()
+    vis4 = p.dump_visibilities_i64()
+    p.close()
+    rings = [(dr.numpy(), ds.numpy()) for dr, ds, _ in blocks]
+    # every channel, block 0, in chunks of 48 channels
+    for c0 in range(0, C, 48):
+        s = orc.Shape(C=48, F=1, NT=64, R=R, P=2, RPP=R, NP=NP, B=B)
+        yc = orc.ingest(s, rings[0][0][c0:c0 + 48], rings[0][1][c0:c0 + 48], orc.WINDOW_ADAPTIVE)
+        assert np.array_equal(vis1[c0:c0 + 48], orc.correlate(s, yc)), c0
+    _log("cfg2 visibilities of all 384 channels bit-exact")
+    chans = np.arange(0, C, C // 16)
+    s = orc.Shape(C=len(chans), F=1, NT=64, R=R, P=2, RPP=R, NP=NP, B=B)
+    acc = 0
+    for ring, scales in rings:
+        acc = acc + orc.correlate(s, orc.ingest(s, ring[chans], scales[chans], orc.WINDOW_ADAPTIVE))
+    assert np.array_equal(vis4[chans], acc)
+    yb = _complex(beams[chans]).astype(np.complex128)
+    y0 = orc.ingest(s, rings[0][0][chans], rings[0][1][chans], orc.WINDOW_ADAPTIVE)
+    refb = orc.adaptive_block(s, y0, w[chans], fixed_k=np.ones((len(chans), 2)))
+    eb = np.max(np.abs(yb - refb["beams2b"])) / np.max(np.abs(refb["beams2b"]))
+    top = np.abs(evals[chans, :, -1] - refb["evals"][..., -1]) / np.abs(refb["evals"][..., -1])
+    _log(f"cfg2 sampled: beam rel err {eb:.2e}, top eigenvalue rel err {top.max():.2e}")
+    assert eb < BEAM_TOL
+    assert top.max() <= EIG_TOL
