@@ -462,11 +462,13 @@ def run_precision(args):
 
 def run_subarrays(args):
     """Uneven batching: 384 ch x 2 pol x 2 subarray groups of 12, 24 or 36
-    antennas (1536 eigenproblems per block), 1-3 CW interferers per channel at
-    INR 10-30 dB, 8192 samples.  The per-block sequence (correlator, eigen /
-    null stage, persistent subarray launch, 2B beams) runs from its hipGraph;
-    the autotuner picks the launch shapes first.  Reports the graph-replayed
-    block time and the persistent launch's time and problem rate."""
+    antennas (1536 subarray eigenproblems per block) beside the 768 full-array
+    problems, 1-3 CW interferers per channel at INR 10-30 dB, 8192 samples.
+    The per-block sequence (correlator, one persistent launch for all 2304
+    eigen / null problems, (2 + 2) x 16 beams incl. the groups' mitigated
+    beams) runs from its hipGraphs; the autotuner picks the launch shapes
+    first.  Reports the graph-replayed block time and the persistent launch's
+    time and problem rate."""
     C, G = args.channels, 2
     shape = syn.BlockShape(C=C, F=1, NT=64, R=36, P=2, RPP=36, NP=128, B=16, blocks_to_accumulate=4)
     seed = 20261203 + 4 + 100
@@ -510,9 +512,11 @@ def run_subarrays(args):
     kt = p.kernel_times()
     res = p.subarray_results()
     p.close()
-    sub_ms = kt["subarray_eig"][0] / kt["subarray_eig"][1]
-    nprob = C * 2 * G
-    return dict(workload=f"{C} ch x 2 pol x {G} groups of 12/24/36 antennas ({nprob} eigenproblems), "
+    key = "subarray_eig" if "subarray_eig" in kt else "eig_null"  # one launch: the queue of all problems
+    sub_ms = kt[key][0] / kt[key][1]
+    nprob = C * 2 * G + C * 2
+    return dict(workload=f"{C} ch x 2 pol x {G} groups of 12/24/36 antennas + the full array ({nprob} "
+                         f"eigenproblems in one persistent launch), {(2 + G) * 16} output beams, "
                          f"1-3 CW interferers at INR 10-30 dB, {shape.T} samples/block",
                 graph_ms_per_block=round(graph_ms, 4),
                 real_time_factor=round((shape.T / CHANNEL_RATE_HZ) / (graph_ms * 1e-3), 3),

@@ -125,6 +125,9 @@ struct lam_ctx {
   int num_cus = 256;
   // subarrays of uneven size (persistent eigen launch)
   int sub_G = 0, sub_max_n = 0;
+  int *d_queue_order = nullptr, *d_queue_counter = nullptr;  // full-array + subarray problems, largest first
+  int queue_n = 0;
+  int persistent_wgs = 0;  // cap on the persistent launches' grid (0: as many as fit)
   uint8_t *d_sub_mask = nullptr;
   int *d_sub_k = nullptr, *d_sub_order = nullptr, *d_sub_counter = nullptr;
   float *d_sub_evals = nullptr, *d_sub_evecs = nullptr, *d_sub_proj = nullptr;
@@ -236,6 +239,39 @@ bool eig_chained(const lam_ctx *ctx) {
   return ctx->mode == LAM_MODE_ADAPTIVE && (ctx->lr.ready || ctx->sub_G > 0);
 }
 
+// Parameters of the subarray problems of one block on buffer b.
+SubParams sub_params(lam_ctx *ctx, Buf &b, int nsplit) {
+  SubParams q{};
+  q.C = ctx->C;
+  q.R = ctx->R;
+  q.G = ctx->sub_G;
+  q.vis_part = b.d_vis_part;
+  q.nsplit = nsplit;
+  q.mask = ctx->d_sub_mask;
+  q.k = ctx->d_sub_k;
+  q.order = ctx->d_sub_order;
+  q.nprob = ctx->C * 2 * ctx->sub_G;
+  q.max_n = ctx->sub_max_n;
+  q.max_sweeps = ctx->max_sweeps;
+  q.tol = ctx->tol;
+  q.counter = ctx->d_sub_counter;
+  q.evals = ctx->d_sub_evals;
+  q.evecs = ctx->d_sub_evecs;
+  q.proj = ctx->d_sub_proj;
+  q.counts = ctx->d_sub_counts;
+  q.sweeps = ctx->d_sub_sweeps;
+  q.sizes = ctx->d_sub_sizes;
+  // the groups' mitigated beams: rows 2B + g B + b of the 2B-beam operands
+  q.weights = ctx->d_weights;
+  q.B = ctx->B;
+  q.NMT = ctx->NMT2;
+  q.NK = ctx->NK;
+  q.row0 = 2 * ctx->B;
+  q.wprec = ctx->wprec;
+  q.wop = b.d_wop2;
+  return q;
+}
+
 // One phase of the per-block kernel sequence of one buffer (captured into a
 // graph per phase when enabled).  The correlator-beamformer mode runs in phase 0.
 int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales, int phase) {
@@ -306,7 +342,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
   if (phase == 2) {
     VoltParams pb = p;
     pb.wop = b.d_wop2;
-    pb.nbeams_out = 2 * ctx->B;
+    pb.nbeams_out = ctx->Bout;  // 2B, plus B per subarray group
     pb.NMT = ctx->NMT2;
     pb.beams = b.d_beams;
     {
@@ -315,7 +351,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
     }
     if (ctx->spectra) {  // adaptive 808-848: FFT per packet, edge bins dropped, |X| summed
       Prof pr(ctx, s, LAM_K_SPECTRA);
-      HIPC(ctx, launch_spectra(b.d_beams, ctx->d_spec_table, ctx->C, 2 * ctx->B, ctx->T, ctx->NT, b.d_spectra, s));
+      HIPC(ctx, launch_spectra(b.d_beams, ctx->d_spec_table, ctx->C, ctx->Bout, ctx->T, ctx->NT, b.d_spectra, s));
     }
     HIPC(ctx, launch_accumulate_vis(b.d_vis_part, p.nsplit, vis_count, (int)ctx->NBL, ctx->d_vis_acc, atomic_acc, s));
     return LAM_OK;
@@ -384,34 +420,26 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
         e.prop_stride = L.KW * ctx->R;
       }
     }
-    {
+    if (ctx->sub_G > 0) {
+      // subarrays: every (channel, pol, group) problem in one launch; with
+      // the plain solver the full-array problems join the same queue
+      SubParams q = sub_params(ctx, b, p.nsplit);
+      if (!eig_is_warm(e)) {
+        Prof pr(ctx, s, LAM_K_EIG);
+        HIPC(ctx, launch_eig_queue(e, q, ctx->d_queue_order, ctx->queue_n, ctx->d_queue_counter, ctx->num_cus,
+                                   ctx->persistent_wgs, s));
+      } else {
+        {
+          Prof pr(ctx, s, LAM_K_EIG);
+          HIPC(ctx, launch_eig(e, s));
+        }
+        Prof pr(ctx, s, LAM_K_SUBARRAY);
+        HIPC(ctx, launch_subarray_null(q, ctx->num_cus, ctx->persistent_wgs, s));
+      }
+    } else {
       Prof pr(ctx, s, LAM_K_EIG);
       HIPC(ctx, launch_eig(e, s));
     }
-  }
-  if (ctx->sub_G > 0) {  // subarrays: every (channel, pol, group) problem in one launch
-    SubParams q{};
-    q.C = ctx->C;
-    q.R = ctx->R;
-    q.G = ctx->sub_G;
-    q.vis_part = b.d_vis_part;
-    q.nsplit = p.nsplit;
-    q.mask = ctx->d_sub_mask;
-    q.k = ctx->d_sub_k;
-    q.order = ctx->d_sub_order;
-    q.nprob = ctx->C * 2 * ctx->sub_G;
-    q.max_n = ctx->sub_max_n;
-    q.max_sweeps = ctx->max_sweeps;
-    q.tol = ctx->tol;
-    q.counter = ctx->d_sub_counter;
-    q.evals = ctx->d_sub_evals;
-    q.evecs = ctx->d_sub_evecs;
-    q.proj = ctx->d_sub_proj;
-    q.counts = ctx->d_sub_counts;
-    q.sweeps = ctx->d_sub_sweeps;
-    q.sizes = ctx->d_sub_sizes;
-    Prof pr(ctx, s, LAM_K_SUBARRAY);
-    HIPC(ctx, launch_subarray_null(q, ctx->num_cus, s));
   }
   if (capture) {  // labels: this block's plain-Jacobi vectors and mode counts
     const size_t nu = (size_t)L.KW * ctx->R;
@@ -532,13 +560,38 @@ void free_subarrays(lam_ctx *ctx) {
   for (void *q : {(void *)ctx->d_sub_mask, (void *)ctx->d_sub_k, (void *)ctx->d_sub_order,
                   (void *)ctx->d_sub_counter, (void *)ctx->d_sub_evals, (void *)ctx->d_sub_evecs,
                   (void *)ctx->d_sub_proj, (void *)ctx->d_sub_counts, (void *)ctx->d_sub_sweeps,
-                  (void *)ctx->d_sub_sizes})
+                  (void *)ctx->d_sub_sizes, (void *)ctx->d_queue_order, (void *)ctx->d_queue_counter})
     if (q) (void)hipFree(q);
+  ctx->d_queue_order = ctx->d_queue_counter = nullptr;
+  ctx->queue_n = 0;
   ctx->d_sub_mask = nullptr;
   ctx->d_sub_k = ctx->d_sub_order = ctx->d_sub_counter = nullptr;
   ctx->d_sub_evals = ctx->d_sub_evecs = ctx->d_sub_proj = nullptr;
   ctx->d_sub_counts = ctx->d_sub_sweeps = ctx->d_sub_sizes = nullptr;
   ctx->sub_G = 0;
+}
+
+// (Re)allocate every buffer's output beams, 2B-beam operands and spectra for
+// ctx->Bout output beams per (channel, pol); the operands start at zero
+// (rows no kernel writes stay zero).
+int size_beam_outputs(lam_ctx *ctx, int Bout) {
+  ctx->Bout = Bout;
+  ctx->NMT2 = (2 * Bout + 31) / 32;
+  if (ctx->NMT2 == 3) ctx->NMT2 = 4;
+  ctx->beams_bytes = (size_t)ctx->C * 2 * ctx->Bout * ctx->T * 4;
+  for (auto &b : ctx->bufs) {
+    for (void *q : {(void *)b.d_beams, (void *)b.d_wop2, (void *)b.d_spectra})
+      if (q) (void)hipFree(q);
+    b.d_beams = nullptr;
+    b.d_wop2 = nullptr;
+    b.d_spectra = nullptr;
+    if (int r = dalloc(ctx, &b.d_beams, ctx->beams_bytes)) return r;
+    if (int r = dalloc(ctx, &b.d_wop2, (size_t)ctx->C * 2 * ctx->NMT2 * ctx->NK * 2 * 64 * sizeof(v8h))) return r;
+    if (ctx->NT > 10)
+      if (int r = dalloc(ctx, &b.d_spectra, (size_t)ctx->C * 2 * ctx->Bout * (ctx->NT - 10) * sizeof(float)))
+        return r;
+  }
+  return LAM_OK;
 }
 
 void free_learn(Learn &L) {
@@ -1520,10 +1573,14 @@ int lam_set_subarrays(lam_ctx *ctx, int ngroups, const uint8_t *mask, const int3
   if (ctx->mode != LAM_MODE_ADAPTIVE) return fail(ctx, LAM_ERR_STATE, "subarrays need the adaptive mode");
   if (ngroups > 0 && !ctx->whiten)
     return fail(ctx, LAM_ERR_STATE, "the subarray eigen stage whitens: turn whitening back on first");
+  if (ngroups > 0 && (2 + ngroups) * ctx->B > 64)
+    return fail(ctx, LAM_ERR_UNSUPPORTED, "(2 + groups) x beams = %d output beams per (channel, pol); at most 64",
+                (2 + ngroups) * ctx->B);
   if (int r = sync_all(ctx)) return r;
   free_subarrays(ctx);
   ctx->graphs_dirty = true;
-  if (ngroups == 0) return LAM_OK;
+  for (auto &b : ctx->bufs) destroy_graphs(b);
+  if (ngroups == 0) return size_beam_outputs(ctx, 2 * ctx->B);
   const int C = ctx->C, R = ctx->R, G = ngroups, np = C * 2 * G;
   // problem order: largest subarray first (the persistent workgroups take them in turn)
   std::vector<int> size(np), order(np);
@@ -1536,6 +1593,13 @@ int lam_set_subarrays(lam_ctx *ctx, int ngroups, const uint8_t *mask, const int3
       }
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return size[a] > size[b]; });
+  // one queue with the 2C full-array problems (items 0..2C-1, size R) and the
+  // subarray problems (items 2C + q), largest first
+  std::vector<int> qsize(2 * C + np), qorder(2 * C + np);
+  for (int i = 0; i < 2 * C; ++i) qsize[i] = R;
+  for (int q = 0; q < np; ++q) qsize[2 * C + q] = size[q];
+  std::iota(qorder.begin(), qorder.end(), 0);
+  std::stable_sort(qorder.begin(), qorder.end(), [&](int a, int b) { return qsize[a] > qsize[b]; });
   int rc = LAM_OK;
   auto chk = [&](int r) {
     if (r != LAM_OK && rc == LAM_OK) rc = r;
@@ -1550,15 +1614,27 @@ int lam_set_subarrays(lam_ctx *ctx, int ngroups, const uint8_t *mask, const int3
   chk(dalloc(ctx, &ctx->d_sub_counts, (size_t)np * sizeof(int)));
   chk(dalloc(ctx, &ctx->d_sub_sweeps, (size_t)np * sizeof(int)));
   chk(dalloc(ctx, &ctx->d_sub_sizes, (size_t)np * sizeof(int)));
+  chk(dalloc(ctx, &ctx->d_queue_order, qorder.size() * sizeof(int)));
+  chk(dalloc(ctx, &ctx->d_queue_counter, sizeof(int)));
+  if (rc == LAM_OK) rc = size_beam_outputs(ctx, (2 + G) * ctx->B);
   if (rc != LAM_OK) {
     free_subarrays(ctx);
     return rc;
   }
+  HIPC(ctx, hipMemcpy(ctx->d_queue_order, qorder.data(), qorder.size() * sizeof(int), hipMemcpyHostToDevice));
+  ctx->queue_n = (int)qorder.size();
   HIPC(ctx, hipMemcpy(ctx->d_sub_mask, mask, (size_t)C * G * R, hipMemcpyHostToDevice));
   HIPC(ctx, hipMemcpy(ctx->d_sub_k, k, (size_t)np * sizeof(int), hipMemcpyHostToDevice));
   HIPC(ctx, hipMemcpy(ctx->d_sub_order, order.data(), (size_t)np * sizeof(int), hipMemcpyHostToDevice));
   ctx->sub_G = G;
   ctx->sub_max_n = std::max(1, *std::max_element(size.begin(), size.end()));
+  return LAM_OK;
+}
+
+int lam_set_persistent_workgroups(lam_ctx *ctx, int max_workgroups) {
+  if (!ctx || max_workgroups < 0) return LAM_ERR_ARG;
+  ctx->persistent_wgs = max_workgroups;
+  ctx->graphs_dirty = true;
   return LAM_OK;
 }
 

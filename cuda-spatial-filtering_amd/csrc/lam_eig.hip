@@ -853,11 +853,10 @@ __device__ __forceinline__ DF df_from_i64(long long v) {
 // diagonal scaling in FP32 perturbs the quotient only relatively (O(eps)).
 // Runs between the eigen solve and the outputs (S.A, whose diagonal the sort
 // has already copied, holds the per-row partial quotients); overwrites S.ev
-// of the top K.
-__device__ __forceinline__ void refine_removed_eigenvalues(const EigParams &P, EigSmem &S, int c, int p, int n,
-                                                           int K) {
+// of the top K.  cov(i, j) (i <= j) gives the exact A(i, j) as a longlong2.
+template <typename Cov>
+__device__ __forceinline__ void refine_removed_eigenvalues(EigSmem &S, int n, int K, int whiten, Cov cov) {
   const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1;
-  const long nbl = (long)n * (n + 1) / 2;
   float2 *zb = S.A;  // [kk][i]: Re(conj(x_i) (A x)_i) of mode kk as (hi, lo); K * n <= ld * n
   // per (mode, row): x = S u, (A x)_i from the exact covariance (read again
   // from the partials), and the row's term of the quotient
@@ -866,16 +865,9 @@ __device__ __forceinline__ void refine_removed_eigenvalues(const EigParams &P, E
     const float2 *u = S.V + S.perm[n - 1 - kk] * ld;
     DF yr{0.f, 0.f}, yi{0.f, 0.f};
     for (int j = 0; j < n; ++j) {
-      const int lo = min(i, j), hi = max(i, j);
-      const long bl = (long)hi * (hi + 1) / 2 + lo;
-      long long re = 0, im = 0;
-      for (int k = 0; k < P.nsplit; ++k) {
-        const int2 v = reinterpret_cast<const int2 *>(P.vis_part)[(((long)k * P.C + c) * 4 + 3 * p) * nbl + bl];
-        re += v.x;
-        im += v.y;
-      }
-      if (i > j) im = -im;  // A(i, j) = conj(A(j, i)) below the diagonal
-      const float sj = P.whiten ? 1.0f / S.d[j] : 1.0f;
+      const longlong2 a = cov(min(i, j), max(i, j));
+      const long long re = a.x, im = i > j ? -a.y : a.y;  // A(i, j) = conj(A(j, i)) below the diagonal
+      const float sj = whiten ? 1.0f / S.d[j] : 1.0f;
       const float xr = u[j].x * sj, xi = u[j].y * sj;
       const DF ar = df_from_i64(re), ai = df_from_i64(im);
       // (ar + i ai)(xr + i xi)
@@ -884,7 +876,7 @@ __device__ __forceinline__ void refine_removed_eigenvalues(const EigParams &P, E
       yi = df_add(yi, df_mul(ar, xi));
       yi = df_add(yi, df_mul(ai, xr));
     }
-    const float si = P.whiten ? 1.0f / S.d[i] : 1.0f;
+    const float si = whiten ? 1.0f / S.d[i] : 1.0f;
     const DF z = df_add(df_mul(yr, u[i].x * si), df_mul(yi, u[i].y * si));  // Re(conj(x_i) y_i)
     zb[e] = make_float2(z.hi, z.lo);
   }
@@ -904,14 +896,47 @@ __device__ __forceinline__ void refine_removed_eigenvalues(const EigParams &P, E
   __syncthreads();
 }
 
-// WARM = false: plain Jacobi (and the second launch that re-solves deferred
-// problems); WARM = true: warm-start solve, rejected problems are deferred.
+// MFMA A-operand fragments of beam rows [row0, row0 + nrows) of one (channel,
+// pol) in the layout [NMT][NK][hi, lo][64 lanes] x 8 halves that the beam
+// kernel reads (rows of [Re; Im] pairs, K = (antenna, re/im)); weight(e, a)
+// gives the complex weight of beam e (0-based within the rows) on antenna a.
+// Each lane of a fragment holds one row, so writers of different rows never
+// share a fragment lane; rows no writer owns stay as allocated (zero).
+template <typename Weight>
+__device__ __forceinline__ void write_beam_operands(v8h *wop_cp, int NMT, int NK, int wprec, int row0, int nrows,
+                                                    Weight weight) {
+  for (int idx = threadIdx.x; idx < NMT * NK * 64; idx += blockDim.x) {
+    const int lane = idx & 63, j = (idx >> 6) % NK, mt = (idx >> 6) / NK;
+    const int m = 32 * mt + (lane & 31), kg = lane >> 5;
+    const int e = (m >> 1) - row0, u = m & 1;
+    if (e < 0 || e >= nrows) continue;
+    v8h hi, lo;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float2 w = weight(e, 8 * j + 4 * kg + i);
+      const float v0 = u == 0 ? w.x : w.y;
+      const float v1 = u == 0 ? -w.y : w.x;
+      _Float16 h0, l0, h1, l1;
+      weight_operands(v0, wprec, h0, l0);
+      weight_operands(v1, wprec, h1, l1);
+      hi[2 * i] = h0;
+      lo[2 * i] = l0;
+      hi[2 * i + 1] = h1;
+      lo[2 * i + 1] = l1;
+    }
+    v8h *dst = wop_cp + ((long)mt * NK + j) * 2 * 64;
+    dst[lane] = hi;
+    dst[64 + lane] = lo;
+  }
+}
+
+// One (channel, pol) eigen / RFI-null problem in this workgroup.
+// WARM = false: plain Jacobi; WARM = true: warm-start solve, a rejected
+// problem re-solved with the plain solver in the same workgroup.
 template <int NTHR, bool WARM>
-__global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
-  extern __shared__ __align__(16) char smem_raw[];
+__device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_raw, int c, int p) {
   const int n = P.R, B = P.B;
   EigSmem S = carve(smem_raw, n, B, WARM, NTHR / 64);
-  const int c = blockIdx.x >> 1, p = blockIdx.x & 1;
   const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1;
   const long nbl = (long)n * (n + 1) / 2;
 
@@ -975,7 +1000,17 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
     K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
   }
   // 4. eigenvalues of the removed modes to the precision of the exact covariance
-  if (K > 0) refine_removed_eigenvalues(P, S, c, p, n, K);
+  if (K > 0)
+    refine_removed_eigenvalues(S, n, K, P.whiten, [&](int i, int j) {
+      const long bl = (long)j * (j + 1) / 2 + i;
+      long long re = 0, im = 0;
+      for (int k = 0; k < P.nsplit; ++k) {
+        const int2 v = reinterpret_cast<const int2 *>(P.vis_part)[(((long)k * P.C + c) * 4 + 3 * p) * nbl + bl];
+        re += v.x;
+        im += v.y;
+      }
+      return make_longlong2(re, im);
+    });
   // modes clearly above the noise floor (> 4x the mean of the remaining eigenvalues):
   // the label count of the warm-start training set
   if (tid == 0 && P.nsig) {
@@ -1092,31 +1127,14 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   }
   __syncthreads();
   // 8. MFMA operands of the 2B beam rows [W; W'] (split hi + lo FP16, FP16 or BF16)
-  const int nrow = 2 * B;
-  for (int idx = tid; idx < P.NMT * P.NK * 64; idx += nthr) {
-    const int lane = idx & 63, j = (idx >> 6) % P.NK, mt = (idx >> 6) / P.NK;
-    const int m = 32 * mt + (lane & 31), kg = lane >> 5;
-    const int e = m >> 1, u = m & 1;
-    v8h hi, lo;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int a = 8 * j + 4 * kg + i;
-      float2 w = make_float2(0.f, 0.f);
-      if (e < nrow && a < n) w = S.W[e * n + a];
-      const float v0 = u == 0 ? w.x : w.y;
-      const float v1 = u == 0 ? -w.y : w.x;
-      _Float16 h0, l0, h1, l1;
-      weight_operands(v0, P.wprec, h0, l0);
-      weight_operands(v1, P.wprec, h1, l1);
-      hi[2 * i] = h0;
-      lo[2 * i] = l0;
-      hi[2 * i + 1] = h1;
-      lo[2 * i + 1] = l1;
-    }
-    v8h *dst = P.wop + ((cp * P.NMT + mt) * P.NK + j) * 2 * 64;
-    dst[lane] = hi;
-    dst[64 + lane] = lo;
-  }
+  write_beam_operands(P.wop + cp * P.NMT * P.NK * 2 * 64, P.NMT, P.NK, P.wprec, 0, 2 * B,
+                      [&](int e, int a) { return a < n ? S.W[e * n + a] : make_float2(0.f, 0.f); });
+}
+
+template <int NTHR, bool WARM>
+__global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
+  extern __shared__ __align__(16) char smem_raw[];
+  eig_null_problem<NTHR, WARM>(P, smem_raw, blockIdx.x >> 1, blockIdx.x & 1);
 }
 
 template <int NTHR, bool WARM>
@@ -1162,111 +1180,210 @@ hipError_t launch_eig(const EigParams &p, hipStream_t s) {
 // normalised eigenvectors and P = I - U_K U_K^H, all in the group's n_g x n_g
 // antenna space.  Workgroups take problems from an atomic counter until the
 // list is exhausted, so sizes 12 .. 64 share the machine without tail waste.
-__global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P) {
-  extern __shared__ __align__(16) char smem_raw[];
-  __shared__ int s_next, s_n, s_ant[64];
+// One subarray problem q in this workgroup; s_n, s_ant, s_pos: shared
+// scratch of the caller (group size, member antennas, antenna -> member).
+// With beam operands requested (P.wop), the group's mitigated weights
+// W_g' = W_g P_g (W_g: the beam weights on the member antennas) become beam
+// rows row0 + g B + b of the (channel, pol)'s operand tiles, zero on the
+// other antennas, so the beam kernel forms the subarray beams beside the
+// full-array ones.
+__device__ __forceinline__ void subarray_problem(const SubParams &P, char *smem_raw, int q, int *s_n, int *s_ant,
+                                                 int *s_pos) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   const long nbl_full = (long)P.R * (P.R + 1) / 2;
-  for (;;) {
-    if (tid == 0) s_next = atomicAdd(P.counter, 1);
-    __syncthreads();
-    const int qi = s_next;
-    if (qi >= P.nprob) break;  // every wave of the workgroup leaves together
-    const int q = P.order[qi];
-    const int g = q % P.G, cp = q / P.G, c = cp >> 1, pol = cp & 1;
-    // antenna list of the group: ballot prefix over the mask
-    if (tid < 64) {
-      const bool on = tid < P.R && P.mask[((long)c * P.G + g) * P.R + tid] != 0;
-      const unsigned long long bits = __ballot(on);
-      const int pos = __popcll(bits & ((1ull << tid) - 1ull));
-      if (on) s_ant[pos] = tid;
-      if (tid == 0) s_n = __popcll(bits);
+  const int g = q % P.G, cp = q / P.G, c = cp >> 1, pol = cp & 1;
+  // antenna list of the group: ballot prefix over the mask
+  if (tid < 64) {
+    const bool on = tid < P.R && P.mask[((long)c * P.G + g) * P.R + tid] != 0;
+    const unsigned long long bits = __ballot(on);
+    const int pos = __popcll(bits & ((1ull << tid) - 1ull));
+    s_pos[tid] = on ? pos : -1;
+    if (on) s_ant[pos] = tid;
+    if (tid == 0) s_n[0] = __popcll(bits);
+  }
+  __syncthreads();
+  const int n = s_n[0];
+  const long obase = ((long)q) * P.R * P.R;  // output slot of this problem
+  if (n == 0) return;
+  const int B = P.wop ? P.B : 0;
+  EigSmem S = carve(smem_raw, n, B, false, kEigThreads / 64);
+  const int ld = n | 1;
+  const int nb = n * (n + 1) / 2;
+  auto cov = [&](int i, int j) {  // exact A(i, j), i <= j, of the group's antennas
+    const int ai = s_ant[i], aj = s_ant[j];
+    const long bl = (long)aj * (aj + 1) / 2 + ai;
+    long long re = 0, im = 0;
+    for (int k = 0; k < P.nsplit; ++k) {
+      const int *v = P.vis_part + ((((long)k * P.C + c) * 4 + 3 * pol) * nbl_full + bl) * 2;
+      re += v[0];
+      im += v[1];
+    }
+    return make_longlong2(re, im);
+  };
+  for (int idx = tid; idx < nb; idx += nthr) {  // packed (i <= j) of the subarray
+    int j = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
+    while (j * (j + 1) / 2 > idx) --j;
+    while ((j + 1) * (j + 2) / 2 <= idx) ++j;
+    const int i = idx - j * (j + 1) / 2;
+    const longlong2 v = cov(i, j);
+    const float2 a = make_float2((float)v.x, (float)v.y);
+    S.A[j * ld + i] = a;
+    S.A[i * ld + j] = make_float2(a.x, -a.y);
+  }
+  __syncthreads();
+  if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f));
+  __syncthreads();
+  for (int i = tid; i < n * n; i += nthr) {
+    const int col = i / n, row = i - col * n;
+    const float den = S.d[row] * S.d[col];
+    S.A[col * ld + row].x /= den;
+    S.A[col * ld + row].y /= den;
+  }
+  __syncthreads();
+  const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, P.max_sweeps, P.tol, S.jl());
+  sort_eigen(S.A, n, S.perm, S.ev);
+  const int K = max(0, min(n, P.k[q]));
+  if (K > 0) refine_removed_eigenvalues(S, n, K, 1, cov);
+  // unwhitened, normalised eigenvectors (ascending) into A (stride n)
+  if (tid < n) {
+    const int src = S.perm[tid];
+    float nrm = 0.0f;
+    for (int i = 0; i < n; ++i) {
+      float2 v = S.V[src * ld + i];
+      v.x *= S.d[i];
+      v.y *= S.d[i];
+      S.A[tid * n + i] = v;
+      nrm += v.x * v.x + v.y * v.y;
+    }
+    const float inv = nrm > 0.0f ? rsqrtf(nrm) : 0.0f;
+    for (int i = 0; i < n; ++i) {
+      S.A[tid * n + i].x *= inv;
+      S.A[tid * n + i].y *= inv;
+    }
+    P.evals[(long)q * P.R + tid] = S.ev[tid];
+  }
+  __syncthreads();
+  float2 *ev_out = reinterpret_cast<float2 *>(P.evecs) + obase;
+  float2 *pr_out = reinterpret_cast<float2 *>(P.proj) + obase;
+  for (int e = tid; e < n * n; e += nthr) {
+    const int col = e / n, row = e - col * n;
+    ev_out[e] = S.A[e];
+    // P(row, col) = delta - sum_{k >= n-K} u_k(row) conj(u_k(col))
+    float2 acc = make_float2(row == col ? 1.0f : 0.0f, 0.0f);
+    for (int k = n - K; k < n; ++k) {
+      const float2 t = cmulc(S.A[k * n + row], S.A[k * n + col]);
+      acc.x -= t.x;
+      acc.y -= t.y;
+    }
+    pr_out[e] = acc;
+  }
+  if (tid == 0) {
+    P.counts[q] = K;
+    if (P.sweeps) P.sweeps[q] = sw;
+    P.sizes[q] = n;
+  }
+  if (P.wop) {
+    // W_g (rows 0..B-1 of S.W) and G = W_g U_K; W_g' = W_g - G U_K^H (rows B..2B-1)
+    for (int idx = tid; idx < B * n; idx += nthr) {
+      const int bb = idx / n, i = idx - bb * n;
+      const __half2 w = reinterpret_cast<const __half2 *>(P.weights)[((long)cp * B + bb) * P.R + s_ant[i]];
+      S.W[idx] = __half22float2(w);
     }
     __syncthreads();
-    const int n = s_n;
-    const long obase = ((long)q) * P.R * P.R;  // output slot of this problem
-    if (n == 0) {
-      __syncthreads();
-      continue;
-    }
-    EigSmem S = carve(smem_raw, n, 0, false, kEigThreads / 64);
-    const int ld = n | 1;
-    const int nb = n * (n + 1) / 2;
-    for (int idx = tid; idx < nb; idx += nthr) {  // packed (i <= j) of the subarray
-      int j = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
-      while (j * (j + 1) / 2 > idx) --j;
-      while ((j + 1) * (j + 2) / 2 <= idx) ++j;
-      const int i = idx - j * (j + 1) / 2;
-      const int ai = s_ant[i], aj = s_ant[j];
-      const long bl = (long)aj * (aj + 1) / 2 + ai;
-      long re = 0, im = 0;
-      for (int k = 0; k < P.nsplit; ++k) {
-        const int *v = P.vis_part + ((((long)k * P.C + c) * 4 + 3 * pol) * nbl_full + bl) * 2;
-        re += v[0];
-        im += v[1];
-      }
-      const float2 a = make_float2((float)re, (float)im);
-      S.A[j * ld + i] = a;
-      S.A[i * ld + j] = make_float2(a.x, -a.y);
-    }
-    __syncthreads();
-    if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f));
-    __syncthreads();
-    for (int i = tid; i < n * n; i += nthr) {
-      const int col = i / n, row = i - col * n;
-      const float den = S.d[row] * S.d[col];
-      S.A[col * ld + row].x /= den;
-      S.A[col * ld + row].y /= den;
-    }
-    __syncthreads();
-    const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, P.max_sweeps, P.tol, S.jl());
-    sort_eigen(S.A, n, S.perm, S.ev);
-    const int K = max(0, min(n, P.k[q]));
-    // unwhitened, normalised eigenvectors (ascending) into A (stride n)
-    if (tid < n) {
-      const int src = S.perm[tid];
-      float nrm = 0.0f;
+    for (int idx = tid; idx < B * K; idx += nthr) {
+      const int bb = idx / K, k = n - K + idx % K;
+      float2 acc = make_float2(0.f, 0.f);
       for (int i = 0; i < n; ++i) {
-        float2 v = S.V[src * ld + i];
-        v.x *= S.d[i];
-        v.y *= S.d[i];
-        S.A[tid * n + i] = v;
-        nrm += v.x * v.x + v.y * v.y;
+        const float2 t = cmul(S.W[bb * n + i], S.A[k * n + i]);
+        acc.x += t.x;
+        acc.y += t.y;
       }
-      const float inv = nrm > 0.0f ? rsqrtf(nrm) : 0.0f;
-      for (int i = 0; i < n; ++i) {
-        S.A[tid * n + i].x *= inv;
-        S.A[tid * n + i].y *= inv;
-      }
-      P.evals[(long)q * P.R + tid] = S.ev[tid];
+      S.G[bb * n + k] = acc;
     }
     __syncthreads();
-    float2 *ev_out = reinterpret_cast<float2 *>(P.evecs) + obase;
-    float2 *pr_out = reinterpret_cast<float2 *>(P.proj) + obase;
-    for (int e = tid; e < n * n; e += nthr) {
-      const int col = e / n, row = e - col * n;
-      ev_out[e] = S.A[e];
-      // P(row, col) = delta - sum_{k >= n-K} u_k(row) conj(u_k(col))
-      float2 acc = make_float2(row == col ? 1.0f : 0.0f, 0.0f);
+    for (int idx = tid; idx < B * n; idx += nthr) {
+      const int bb = idx / n, i = idx - bb * n;
+      float2 acc = S.W[bb * n + i];
       for (int k = n - K; k < n; ++k) {
-        const float2 t = cmulc(S.A[k * n + row], S.A[k * n + col]);
+        const float2 t = cmulc(S.G[bb * n + k], S.A[k * n + i]);
         acc.x -= t.x;
         acc.y -= t.y;
       }
-      pr_out[e] = acc;
+      S.W[(B + bb) * n + i] = acc;
     }
-    if (tid == 0) {
-      P.counts[q] = K;
-      if (P.sweeps) P.sweeps[q] = sw;
-      P.sizes[q] = n;
-    }
-    __syncthreads();  // LDS is reused by the next problem
+    __syncthreads();
+    write_beam_operands(P.wop + (long)cp * P.NMT * P.NK * 2 * 64, P.NMT, P.NK, P.wprec, P.row0 + g * B, B,
+                        [&](int e, int a) {
+                          const int m = a < P.R ? s_pos[a] : -1;
+                          return m >= 0 ? S.W[(B + e) * n + m] : make_float2(0.f, 0.f);
+                        });
   }
 }
 
-hipError_t launch_subarray_null(const SubParams &p, int num_cus, hipStream_t s) {
+// ---- subarrays of uneven size: one persistent launch ----
+// Problem q = order[q'] (largest first) is (channel c, pol p, group g): the
+// principal submatrix of the (c, p) covariance on the group's antennas (mask
+// bits, ascending antenna order), whitened, solved by cyclic Jacobi, K_g
+// largest modes removed: outputs eigenvalues (ascending; the removed ones as
+// compensated Rayleigh quotients), unwhitened normalised eigenvectors and
+// P = I - U_K U_K^H, all in the group's n_g x n_g antenna space.  Workgroups
+// take problems from an atomic counter until the list is exhausted, so sizes
+// 12 .. 64 share the machine without tail waste.
+__global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P) {
+  extern __shared__ __align__(16) char smem_raw[];
+  __shared__ int s_next, s_n, s_ant[64], s_pos[64];
+  for (;;) {
+    if (threadIdx.x == 0) s_next = atomicAdd(P.counter, 1);
+    __syncthreads();
+    const int qi = s_next;
+    if (qi >= P.nprob) break;  // every wave of the workgroup leaves together
+    subarray_problem(P, smem_raw, P.order[qi], &s_n, s_ant, s_pos);
+    __syncthreads();  // LDS and the counter slot are reused by the next problem
+  }
+}
+
+// Every eigenproblem of a block in one persistent launch: the 2C full-array
+// (channel, pol) problems (items 0 .. 2C-1, the plain eigen / null stage with
+// its [W; W'] operands) and the subarray problems (items 2C + q), taken by
+// the workgroups from one largest-first queue.
+__global__ void __launch_bounds__(kEigThreads) eig_queue_kernel(EigParams E, SubParams Q, const int *order,
+                                                                int ntotal, int *counter) {
+  extern __shared__ __align__(16) char smem_raw[];
+  __shared__ int s_next, s_n, s_ant[64], s_pos[64];
+  for (;;) {
+    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1);
+    __syncthreads();
+    const int qi = s_next;
+    if (qi >= ntotal) break;
+    const int item = order[qi];
+    if (item < 2 * E.C)
+      eig_null_problem<kEigThreads, false>(E, smem_raw, item >> 1, item & 1);
+    else
+      subarray_problem(Q, smem_raw, item - 2 * E.C, &s_n, s_ant, s_pos);
+    __syncthreads();
+  }
+}
+
+hipError_t launch_eig_queue(const EigParams &e, const SubParams &q, const int *order, int ntotal, int *counter,
+                            int num_cus, int max_wgs, hipStream_t s) {
+  const size_t sm = std::max(eig_smem_bytes(e.R, e.B, false, kEigThreads / 64),
+                             eig_smem_bytes(q.max_n, q.wop ? q.B : 0, false, kEigThreads / 64));
+  if (sm > 65536) {
+    const hipError_t er = hipFuncSetAttribute(reinterpret_cast<const void *>(eig_queue_kernel),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    if (er != hipSuccess) return er;
+  }
+  if (hipError_t er = launch_zero(counter, sizeof(int), s)) return er;
+  const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / (sm + 1024))));
+  int grid = std::max(1, std::min(ntotal, num_cus * per_cu));
+  if (max_wgs > 0) grid = std::min(grid, max_wgs);
+  hipLaunchKernelGGL(eig_queue_kernel, dim3(grid), dim3(kEigThreads), sm, s, e, q, order, ntotal, counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_subarray_null(const SubParams &p, int num_cus, int max_wgs, hipStream_t s) {
   // LDS for the largest group; as many workgroups as fit on the machine at once
-  const size_t sm = eig_smem_bytes(p.max_n, 0, false, kEigThreads / 64);
+  const size_t sm = eig_smem_bytes(p.max_n, p.wop ? p.B : 0, false, kEigThreads / 64);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(subarray_null_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -1274,7 +1391,8 @@ hipError_t launch_subarray_null(const SubParams &p, int num_cus, hipStream_t s) 
   }
   if (hipError_t e = launch_zero(p.counter, sizeof(int), s)) return e;
   const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / (sm + 1024))));
-  const int grid = std::max(1, std::min(p.nprob, num_cus * per_cu));
+  int grid = std::max(1, std::min(p.nprob, num_cus * per_cu));
+  if (max_wgs > 0) grid = std::min(grid, max_wgs);
   hipLaunchKernelGGL(subarray_null_kernel, dim3(grid), dim3(kEigThreads), sm, s, p);
   return hipGetLastError();
 }

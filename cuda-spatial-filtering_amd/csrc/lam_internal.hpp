@@ -120,7 +120,7 @@ struct EigParams {
   int *fallback;           // out: [C*2] 1 where the plain solver ran after the check failed
   int *nsig;               // out: [C*2] modes above 4x the noise floor among the K removed, or nullptr
   float *orth;             // out: [C*2] FP64 check max |U_K^H U_K - I| of the removed (whitened) basis
-  int plain_threads;       // workgroup width of the plain solver (256, 512, 1024)
+  int plain_threads;       // workgroup width of the plain solver (64, 128, 256)
   int wprec;               // precision of the [W; W'] beam operands (kWprec*)
   int whiten;              // 1: whiten the covariance first (adaptive); 0: raw (pulsar-fold RFI branch)
 };
@@ -142,6 +142,11 @@ struct SubParams {
   float *evecs;            // out: [nprob][R*R][2] (n_g x n_g, columns)
   float *proj;             // out: [nprob][R*R][2] P = I - U_K U_K^H (n_g x n_g)
   int *counts, *sweeps, *sizes;  // out: [nprob]
+  // subarray beams (wop == nullptr: none): W_g' = W_g P_g as beam rows
+  // row0 + g * B + b of the (channel, pol) operand tiles [NMT][NK][2][64]
+  const __half *weights;   // [C][2][B][R][2]
+  int B, NMT, NK, row0, wprec;
+  v8h *wop;
 };
 
 // Synthetic voltage generator (benchmark / test input).
@@ -219,7 +224,13 @@ hipError_t launch_gather(const int *idx, int nb, const uint16_t *DX, int Dp, con
 hipError_t launch_synth(const GenParams &g, hipStream_t s);
 hipError_t launch_volt(const VoltParams &p, bool do_corr, bool do_beam, hipStream_t s);
 hipError_t launch_eig(const EigParams &p, hipStream_t s);
-hipError_t launch_subarray_null(const SubParams &p, int max_wgs, hipStream_t s);
+// num_cus: compute units to fill; max_wgs > 0 caps the persistent grid.
+hipError_t launch_subarray_null(const SubParams &p, int num_cus, int max_wgs, hipStream_t s);
+// Full-array and subarray problems of a block from one largest-first queue
+// order[ntotal] (item < 2C: (channel, pol) = (item / 2, item % 2); else
+// subarray problem item - 2C).
+hipError_t launch_eig_queue(const EigParams &e, const SubParams &q, const int *order, int ntotal, int *counter,
+                            int num_cus, int max_wgs, hipStream_t s);
 bool eig_is_warm(const EigParams &p);
 hipError_t launch_prep_weights(const __half *w, const float2 *gains, const float2 *phase,
                                int C, int B, int R, int NMT, int NK, int wprec, v8h *wop,

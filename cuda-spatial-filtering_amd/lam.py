@@ -121,6 +121,7 @@ def load_library(path: str = LIB_PATH):
     L.lam_set_eig_tuning.argtypes = [vp, ctypes.c_int]
     L.lam_set_weight_precision.argtypes = [vp, ctypes.c_int]
     L.lam_set_subarrays.argtypes = [vp, ctypes.c_int, vp, vp]
+    L.lam_set_persistent_workgroups.argtypes = [vp, ctypes.c_int]
     L.lam_subarray_results.argtypes = [vp, vp, vp, vp, vp, vp, vp]
     L.lam_autotune.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp]
     L.lam_eigh_warm.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -323,7 +324,9 @@ class Pipeline:
 
     @property
     def n_out_beams(self):
-        return self.shape.B * (2 if self.mode == MODE_ADAPTIVE else 1)
+        if self.mode != MODE_ADAPTIVE:
+            return self.shape.B
+        return self.shape.B * (2 + getattr(self, "_sub_G", 0))
 
     # -- setters (GPUPipeline virtuals)
     def set_weights(self, halfs_u16):
@@ -488,6 +491,13 @@ class Pipeline:
         kk = np.ascontiguousarray(k, np.int32)
         self._chk(self.L.lam_set_subarrays(self.h, m.shape[1], m.ctypes.data, kk.ctypes.data))
         self._sub_G = m.shape[1]
+
+    def clear_subarrays(self):
+        self._chk(self.L.lam_set_subarrays(self.h, 0, None, None))
+        self._sub_G = 0
+
+    def set_persistent_workgroups(self, max_workgroups=0):
+        self._chk(self.L.lam_set_persistent_workgroups(self.h, int(max_workgroups)))
 
     def subarray_results(self):
         s = self.shape

@@ -18,7 +18,9 @@
  *   gains        float32  [C][P][R][2]                  packet_formats.hpp:384-387
  *   beams        float16  [C][P][Bout][NP*NT][2]        packet_formats.hpp:346-348
  *                (Bout = B in corr-beam mode, 2B in adaptive mode: beams
- *                 0..B-1 original, B..2B-1 RFI-mitigated, adaptive 773-806)
+ *                 0..B-1 original, B..2B-1 RFI-mitigated, adaptive 773-806;
+ *                 with G subarray groups (lam_set_subarrays) (2 + G)B: beams
+ *                 2B + gB .. 2B + gB + B-1 mitigated on group g's antennas)
  *   visibilities float32  [C][R(R+1)/2][P][P][2]        packet_formats.hpp:354-356
  *                V[bl(i,j)][p][q] = sum_t x_ip conj(x_jq), bl(i,j) = j(j+1)/2 + i, i <= j
  *   eigenvalues  float32  [C][P][R]     ascending       adaptive 41-45, 914-950
@@ -343,12 +345,23 @@ int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback, float *ortho
 
 /* Subarrays (no counterpart in the reference): ngroups antenna groups per
  * channel, mask[C][ngroups][R] (non-zero = member), k[C][2][ngroups] modes to
- * remove.  Every block then also solves, in one persistent launch after the
- * eigen stage, all C x 2 x ngroups eigenproblems of uneven size (the principal
- * submatrices of each pol's covariance on the group's antennas, whitened,
- * plain Jacobi, largest first across the machine) and forms each group's null
- * projector P = I - U_K U_K^H in its own antenna space.  ngroups = 0 clears. */
+ * remove.  Every block then also solves all C x 2 x ngroups eigenproblems of
+ * uneven size (the principal submatrices of each pol's covariance on the
+ * group's antennas, whitened, plain Jacobi, the removed eigenvalues refined
+ * as compensated Rayleigh quotients), forms each group's null projector
+ * P_g = I - U_K U_K^H in its own antenna space and its mitigated beams with
+ * the weights W_g P_g (W_g: the beam weights on the group's antennas) as
+ * output beams 2B + gB .. (see the beams layout; (2 + ngroups) x B <= 64).
+ * With the plain solver the full-array problems and the subarray problems of
+ * a block share one persistent launch that takes them largest first from one
+ * queue; with a warm start the subarray problems run in a second persistent
+ * launch.  ngroups = 0 clears (and restores 2B output beams). */
 int lam_set_subarrays(lam_ctx *ctx, int ngroups, const uint8_t *mask, const int32_t *k);
+/* Upper bound on the grid of the persistent eigen launches (0: as many
+ * workgroups as fit on the device at once, the default).  No counterpart in
+ * the reference; a tuning and test knob (a small grid makes every workgroup
+ * take many problems from the queue). */
+int lam_set_persistent_workgroups(lam_ctx *ctx, int max_workgroups);
 /* Results of the last block, problem q = (c * 2 + pol) * ngroups + g, synchronous
  * host copies (any pointer may be NULL): evals [q][R] (first n_q valid,
  * ascending), evecs and proj [q][R*R][2] (n_q x n_q column-major at the start of

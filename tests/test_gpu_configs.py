@@ -152,3 +152,89 @@ def test_cfg2_full_band_exact_and_sampled_beams():
     _log(f"cfg2 sampled: beam rel err {eb:.2e}, top eigenvalue rel err {top.max():.2e}")
     assert eb < BEAM_TOL
     assert top.max() <= EIG_TOL
+
+
+def test_cfg4_one_queue_uneven_subarrays():
+    """cfg4: 384 ch x 2 pol x 2 subarray groupings of 12/24/36 antennas (1536
+    subarray problems) plus the 768 full-array problems of the block in one
+    persistent launch, 1-3 tones at INR 10-30 dB: eigenvalues and projectors
+    of a sample of problems against LAPACK, the groups' beams against the
+    oracle on sampled channels, and the same results bit for bit from a
+    7-workgroup grid (every workgroup takes ~330 problems from the queue)."""
+    G = 2
+    seed = 20261203 + 4
+    sc = syn.DriftScenario(C, R, seed=seed, n_blocks=1, n_src=(1, 3), inr_db=(10.0, 30.0), drift=0.0,
+                           p_switch=0.0, max_sources=3)
+    tones, st, gain, _, act = sc.block(0)
+    dr = lam.DeviceArray(SHAPE.ring_shape(), np.int8)
+    ds = lam.DeviceArray(SHAPE.scale_shape(), np.int16)
+    dt, dst, dg = (lam.DeviceArray.from_numpy(a) for a in (tones, st, gain))
+    lam.synth_block(SHAPE, seed, 0, 16.0, dt, dst, dg, 3, dr, ds)
+    rng = np.random.default_rng(seed)
+    mask = np.zeros((C, G, R), np.uint8)
+    sizes = rng.choice([12, 24, 36], (C, G))
+    for c in range(C):
+        for g in range(G):
+            mask[c, g, rng.choice(R, sizes[c, g], replace=False)] = 1
+    nk = act.sum(1).astype(np.int32)
+    k = np.repeat(nk[:, None, None], 2, axis=1).repeat(G, axis=2)
+    w16, w = syn.random_weights(SHAPE, seed=seed + 1)
+    results = []
+    for wgs in (0, 7):
+        _log(f"cfg4 persistent grid {wgs or 'auto'}")
+        p = lam.Pipeline(SHAPE, lam.MODE_ADAPTIVE)
+        p.set_weights(w16)
+        p.set_fixed_k(np.repeat(nk[:, None], 2, axis=1))
+        p.set_subarrays(mask, k)
+        p.set_persistent_workgroups(wgs)
+        beams = p.beams_array()
+        evals = np.zeros((C, 2, R), np.float32)
+        p.execute(dr, ds, beams_out=beams, evals_out=evals, on_device=True)
+        p.sync()
+        res = p.subarray_results()
+        vis = p.dump_visibilities_i64()
+        p.close()
+        results.append((beams, evals, res))
+    (b0, e0, r0), (b1, e1, r1) = results
+    assert np.array_equal(b0, b1) and np.array_equal(e0, e1)
+    for key in ("evals", "evecs", "proj", "counts", "sizes", "sweeps"):
+        assert np.array_equal(r0[key], r1[key]), key
+    assert sorted(np.unique(r0["sizes"])) == [12, 24, 36]
+    # LAPACK on a sample of the subarray problems (principal submatrices, whitened)
+    Vc = vis[..., 0].astype(np.float64) + 1j * vis[..., 1].astype(np.float64)
+    ii, jj = np.triu_indices(R)
+    blv = jj * (jj + 1) // 2 + ii
+    for q in range(0, C * 2 * G, 24):
+        g, cp = q % G, q // G
+        c, pol = cp // 2, cp % 2
+        A = np.zeros((R, R), np.complex128)
+        A[ii, jj] = Vc[c, blv, pol, pol]
+        A[jj, ii] = np.conj(Vc[c, blv, pol, pol])
+        ant = np.flatnonzero(mask[c, g])
+        n = len(ant)
+        sub = A[np.ix_(ant, ant)]
+        dd = np.sqrt(np.maximum(sub.diagonal().real, 1e-10))
+        lam_ref, V = np.linalg.eigh(sub / np.outer(dd, dd))
+        Kq = k[c, pol, g]
+        ev = r0["evals"][q, :n]
+        assert np.max(np.abs(ev - lam_ref)) <= 1e-4 * lam_ref.max(), q
+        if Kq:
+            assert np.max(np.abs(ev[-Kq:] - lam_ref[-Kq:]) / np.abs(lam_ref[-Kq:])) <= EIG_TOL, q
+        U = dd[:, None] * V[:, n - Kq:]
+        U /= np.linalg.norm(U, axis=0)
+        Pref = np.eye(n) - U @ U.conj().T
+        Pg = r0["proj"][q, :n * n].reshape(n, n).T
+        assert np.max(np.abs(Pg - Pref)) < 1e-3, q
+    # the groups' beams on sampled channels against the oracle
+    from test_gpu_parity import subarray_beams_oracle
+    chans = np.arange(0, C, 48)
+    s = orc.Shape(C=len(chans), F=1, NT=64, R=R, P=2, RPP=R, NP=NP, B=B)
+    y = orc.ingest(s, dr.numpy()[chans], ds.numpy()[chans], orc.WINDOW_ADAPTIVE)
+    for ci, c in enumerate(chans):
+        for pol in range(2):
+            for g in range(G):
+                ref = subarray_beams_oracle(SHAPE, vis[c], y[ci], w[c, pol], mask[c, g], pol, k[c, pol, g])
+                got = _complex(b0[c, pol, (2 + g) * B:(3 + g) * B]).astype(np.complex128)
+                err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+                assert err < BEAM_TOL, (c, pol, g, err)
+    _log("cfg4: 2304 problems in one launch; grid-independent; sample vs LAPACK and the oracle")

@@ -694,6 +694,20 @@ def test_autotune_does_not_capture_training_blocks(num_buffers):
     p.close()
 
 
+def subarray_beams_oracle(shape, vis_c, y_c, w_cp, member, pol, K):
+    """Beams of one subarray group through the oracle's RFI-null chain
+    (rfi_null_cp on the group's packed-triangular visibilities, then W_g' x_g)."""
+    ant = np.flatnonzero(member)
+    n = len(ant)
+    sub = np.zeros((n * (n + 1) // 2, 2, 2, 2), np.int64)
+    for j in range(n):
+        for i in range(j + 1):
+            sub[j * (j + 1) // 2 + i] = vis_c[bl(ant[i], ant[j])]
+    W2, _, _, _ = orc.rfi_null_cp(n, 2, sub, pol, w_cp[:, ant], fixed_k=K)
+    x = y_c[:, ant, pol, 0].astype(np.float64) + 1j * y_c[:, ant, pol, 1]  # [T][n]
+    return W2[shape.B:] @ x.T
+
+
 def test_subarrays_uneven_sizes_persistent():
     """Subarray groups of 12/24/36 antennas: every (channel, pol, group)
     eigenproblem in one persistent launch, against LAPACK on the oracle's
@@ -709,15 +723,28 @@ def test_subarrays_uneven_sizes_persistent():
             n = rng.choice([12, 24, 36])
             mask[c, g, rng.choice(shape.R, n, replace=False)] = 1
     k = rng.integers(1, 4, (shape.C, 2, G)).astype(np.int32)
+    w16, w = syn.random_weights(shape, seed=54)
     p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
-    p.set_weights(syn.random_weights(shape, seed=54)[0])
+    p.set_weights(w16)
     p.set_fixed_k(np.ones((shape.C, 2), np.int32))
     p.set_subarrays(mask, k)
-    p.run_host(ring, scales)
+    out = p.run_host(ring, scales)
     res = p.subarray_results()
     p.close()
     s = _o(shape)
-    vis = orc.correlate(s, orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE))
+    y = orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)
+    vis = orc.correlate(s, y)
+    # the groups' mitigated beams (output beams 2B + gB + b) against the oracle's
+    # RFI-null chain on each group's own visibilities
+    assert out["beams"].shape[2] == (2 + G) * shape.B
+    ref2b = orc.adaptive_block(s, y, w, fixed_k=np.ones((shape.C, 2)))["beams2b"]
+    assert rel_err(out["beams"][:, :, :2 * shape.B], ref2b) < BEAM_TOL
+    for c in range(shape.C):
+        for pol in range(2):
+            for g in range(G):
+                ref = subarray_beams_oracle(shape, vis[c], y[c], w[c, pol], mask[c, g], pol, k[c, pol, g])
+                got = out["beams"][c, pol, (2 + g) * shape.B:(3 + g) * shape.B]
+                assert rel_err(got, ref) < BEAM_TOL, (c, pol, g)
     for c in range(shape.C):
         for pol in range(2):
             A = orc.covariance(shape.R, 2, vis[c], pol)
