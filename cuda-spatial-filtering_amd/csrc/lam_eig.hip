@@ -851,49 +851,82 @@ __device__ __forceinline__ DF df_from_i64(long long v) {
 // second order in the eigenvector error (|r|^2 / gap) and the products of the
 // matrix-vector product are exact, so its sums keep the small mode.  The
 // diagonal scaling in FP32 perturbs the quotient only relatively (O(eps)).
+// Work item (row i, chunk of kJc columns): the chunk's exact entries are
+// loaded once for all modes and its terms Re(conj(x_i) A_ij x_j) (x = S u)
+// summed per mode; a block reduction in compensated arithmetic finishes.
 // Runs between the eigen solve and the outputs (S.A, whose diagonal the sort
-// has already copied, holds the per-row partial quotients); overwrites S.ev
-// of the top K.  cov(i, j) (i <= j) gives the exact A(i, j) as a longlong2.
+// has already copied, is the reduction scratch); overwrites S.ev of the top
+// K, up to kRefineModes modes per pass.  cov(i, j) (i <= j) gives the exact
+// A(i, j) as a longlong2.
+constexpr int kRefineModes = 4;
 template <typename Cov>
 __device__ __forceinline__ void refine_removed_eigenvalues(EigSmem &S, int n, int K, int whiten, Cov cov) {
-  const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1;
-  float2 *zb = S.A;  // [kk][i]: Re(conj(x_i) (A x)_i) of mode kk as (hi, lo); K * n <= ld * n
-  // per (mode, row): x = S u, (A x)_i from the exact covariance (read again
-  // from the partials), and the row's term of the quotient
-  for (int e = tid; e < K * n; e += nthr) {
-    const int kk = e / n, i = e - kk * n;
-    const float2 *u = S.V + S.perm[n - 1 - kk] * ld;
-    DF yr{0.f, 0.f}, yi{0.f, 0.f};
-    for (int j = 0; j < n; ++j) {
-      const longlong2 a = cov(min(i, j), max(i, j));
-      const long long re = a.x, im = i > j ? -a.y : a.y;  // A(i, j) = conj(A(j, i)) below the diagonal
-      const float sj = whiten ? 1.0f / S.d[j] : 1.0f;
-      const float xr = u[j].x * sj, xi = u[j].y * sj;
-      const DF ar = df_from_i64(re), ai = df_from_i64(im);
-      // (ar + i ai)(xr + i xi)
-      yr = df_add(yr, df_mul(ar, xr));
-      yr = df_add(yr, df_neg(df_mul(ai, xi)));
-      yi = df_add(yi, df_mul(ar, xi));
-      yi = df_add(yi, df_mul(ai, xr));
+  constexpr int kJc = 6;
+  const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1, lane = tid & 63, w = tid >> 6, nw = nthr >> 6;
+  const int nch = (n + kJc - 1) / kJc;
+  float2 *red = S.A;  // [kRefineModes][waves] (hi, lo)
+  for (int m0 = 0; m0 < K; m0 += kRefineModes) {
+    const int km = min(kRefineModes, K - m0);
+    DF acc[kRefineModes];
+#pragma unroll
+    for (int m = 0; m < kRefineModes; ++m) acc[m] = DF{0.f, 0.f};
+    for (int item = tid; item < n * nch; item += nthr) {
+      const int i = item / nch, j0 = (item - i * nch) * kJc;
+      DF ar[kJc], ai[kJc];
+#pragma unroll
+      for (int jj = 0; jj < kJc; ++jj) {
+        const int j = min(j0 + jj, n - 1);
+        const longlong2 a = cov(min(i, j), max(i, j));
+        ar[jj] = df_from_i64(a.x);
+        ai[jj] = df_from_i64(i > j ? -a.y : a.y);  // A(i, j) = conj(A(j, i)) below the diagonal
+      }
+      const float si = whiten ? 1.0f / S.d[i] : 1.0f;
+#pragma unroll
+      for (int m = 0; m < kRefineModes; ++m) {
+        if (m >= km) continue;
+        const float2 *u = S.V + S.perm[n - 1 - (m0 + m)] * ld;
+        DF yr{0.f, 0.f}, yi{0.f, 0.f};
+#pragma unroll
+        for (int jj = 0; jj < kJc; ++jj) {
+          const int j = j0 + jj;
+          if (j >= n) continue;
+          const float sj = whiten ? 1.0f / S.d[j] : 1.0f;
+          const float xr = u[j].x * sj, xi = u[j].y * sj;
+          // (ar + i ai)(xr + i xi)
+          yr = df_add(yr, df_mul(ar[jj], xr));
+          yr = df_add(yr, df_neg(df_mul(ai[jj], xi)));
+          yi = df_add(yi, df_mul(ar[jj], xi));
+          yi = df_add(yi, df_mul(ai[jj], xr));
+        }
+        // Re(conj(x_i) y_i)
+        acc[m] = df_add(acc[m], df_add(df_mul(yr, u[i].x * si), df_mul(yi, u[i].y * si)));
+      }
     }
-    const float si = whiten ? 1.0f / S.d[i] : 1.0f;
-    const DF z = df_add(df_mul(yr, u[i].x * si), df_mul(yi, u[i].y * si));  // Re(conj(x_i) y_i)
-    zb[e] = make_float2(z.hi, z.lo);
-  }
-  __syncthreads();
-  // lambda = sum_i Re(conj(x_i) y_i) / |u|^2
-  for (int kk = tid; kk < K; kk += nthr) {
-    const float2 *u = S.V + S.perm[n - 1 - kk] * ld;
-    DF acc{0.f, 0.f};
-    float nrm = 0.0f;
-    for (int i = 0; i < n; ++i) {
-      const float2 z = zb[kk * n + i];
-      acc = df_add(acc, DF{z.x, z.y});
-      nrm += u[i].x * u[i].x + u[i].y * u[i].y;
+    // block reduction per mode: across the wave by shuffles, then across waves
+#pragma unroll
+    for (int m = 0; m < kRefineModes; ++m) {
+      DF v = acc[m];
+      for (int off = 32; off > 0; off >>= 1)
+        v = df_add(v, DF{__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)});
+      acc[m] = v;
     }
-    S.ev[n - 1 - kk] = (acc.hi + acc.lo) / nrm;
+    __syncthreads();  // the previous users of S.A are done
+    if (lane == 0) {
+#pragma unroll
+      for (int m = 0; m < kRefineModes; ++m)
+        if (m < km) red[m * nw + w] = make_float2(acc[m].hi, acc[m].lo);
+    }
+    __syncthreads();
+    if (tid < km) {
+      DF t{0.f, 0.f};
+      for (int ww = 0; ww < nw; ++ww) t = df_add(t, DF{red[tid * nw + ww].x, red[tid * nw + ww].y});
+      const float2 *u = S.V + S.perm[n - 1 - (m0 + tid)] * ld;
+      float nrm = 0.0f;
+      for (int i = 0; i < n; ++i) nrm += u[i].x * u[i].x + u[i].y * u[i].y;
+      S.ev[n - 1 - (m0 + tid)] = (t.hi + t.lo) / nrm;
+    }
+    __syncthreads();
   }
-  __syncthreads();
 }
 
 // MFMA A-operand fragments of beam rows [row0, row0 + nrows) of one (channel,
@@ -1202,7 +1235,7 @@ __device__ __forceinline__ void subarray_problem(const SubParams &P, char *smem_
     if (tid == 0) s_n[0] = __popcll(bits);
   }
   __syncthreads();
-  const int n = s_n[0];
+  const int n = __builtin_amdgcn_readfirstlane(s_n[0]);
   const long obase = ((long)q) * P.R * P.R;  // output slot of this problem
   if (n == 0) return;
   const int B = P.wop ? P.B : 0;
@@ -1335,7 +1368,7 @@ __global__ void __launch_bounds__(kEigThreads) subarray_null_kernel(SubParams P)
   for (;;) {
     if (threadIdx.x == 0) s_next = atomicAdd(P.counter, 1);
     __syncthreads();
-    const int qi = s_next;
+    const int qi = __builtin_amdgcn_readfirstlane(s_next);  // uniform: scalar registers and branches
     if (qi >= P.nprob) break;  // every wave of the workgroup leaves together
     subarray_problem(P, smem_raw, P.order[qi], &s_n, s_ant, s_pos);
     __syncthreads();  // LDS and the counter slot are reused by the next problem
@@ -1353,7 +1386,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_queue_kernel(EigParams E, Sub
   for (;;) {
     if (threadIdx.x == 0) s_next = atomicAdd(counter, 1);
     __syncthreads();
-    const int qi = s_next;
+    const int qi = __builtin_amdgcn_readfirstlane(s_next);  // uniform: scalar registers and branches
     if (qi >= ntotal) break;
     const int item = order[qi];
     if (item < 2 * E.C)
@@ -1362,6 +1395,21 @@ __global__ void __launch_bounds__(kEigThreads) eig_queue_kernel(EigParams E, Sub
       subarray_problem(Q, smem_raw, item - 2 * E.C, &s_n, s_ant, s_pos);
     __syncthreads();
   }
+}
+
+// The same list with one workgroup per item (item = order[blockIdx.x]): the
+// dispatcher starts workgroups in index order as slots free up, so the list
+// is still taken largest first, and without the persistent loop the kernel
+// needs far fewer registers (values hoisted out of a loop around both item
+// bodies stay live across it), which raises its occupancy.
+__global__ void __launch_bounds__(kEigThreads) eig_list_kernel(EigParams E, SubParams Q, const int *order) {
+  extern __shared__ __align__(16) char smem_raw[];
+  __shared__ int s_n, s_ant[64], s_pos[64];
+  const int item = order[blockIdx.x];
+  if (item < 2 * E.C)
+    eig_null_problem<kEigThreads, false>(E, smem_raw, item >> 1, item & 1);
+  else
+    subarray_problem(Q, smem_raw, item - 2 * E.C, &s_n, s_ant, s_pos);
 }
 
 hipError_t launch_eig_queue(const EigParams &e, const SubParams &q, const int *order, int ntotal, int *counter,
@@ -1373,10 +1421,19 @@ hipError_t launch_eig_queue(const EigParams &e, const SubParams &q, const int *o
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (er != hipSuccess) return er;
   }
+  if (max_wgs <= 0) {  // one workgroup per item
+    if (sm > 65536) {
+      const hipError_t er = hipFuncSetAttribute(reinterpret_cast<const void *>(eig_list_kernel),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+      if (er != hipSuccess) return er;
+    }
+    if (ntotal <= 0) return hipSuccess;
+    hipLaunchKernelGGL(eig_list_kernel, dim3(ntotal), dim3(kEigThreads), sm, s, e, q, order);
+    return hipGetLastError();
+  }
   if (hipError_t er = launch_zero(counter, sizeof(int), s)) return er;
   const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / (sm + 1024))));
-  int grid = std::max(1, std::min(ntotal, num_cus * per_cu));
-  if (max_wgs > 0) grid = std::min(grid, max_wgs);
+  const int grid = std::max(1, std::min({ntotal, num_cus * per_cu, max_wgs}));
   hipLaunchKernelGGL(eig_queue_kernel, dim3(grid), dim3(kEigThreads), sm, s, e, q, order, ntotal, counter);
   return hipGetLastError();
 }

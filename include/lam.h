@@ -353,14 +353,19 @@ int lam_eig_stats(lam_ctx *ctx, int32_t *sweeps, int32_t *fallback, float *ortho
  * the weights W_g P_g (W_g: the beam weights on the group's antennas) as
  * output beams 2B + gB .. (see the beams layout; (2 + ngroups) x B <= 64).
  * With the plain solver the full-array problems and the subarray problems of
- * a block share one persistent launch that takes them largest first from one
- * queue; with a warm start the subarray problems run in a second persistent
- * launch.  ngroups = 0 clears (and restores 2B output beams). */
+ * a block share one launch that takes them largest first from one list
+ * (one workgroup per problem, or a persistent grid under
+ * lam_set_persistent_workgroups); with a warm start the subarray problems run
+ * in a second, persistent launch.  ngroups = 0 clears (and restores 2B output
+ * beams). */
 int lam_set_subarrays(lam_ctx *ctx, int ngroups, const uint8_t *mask, const int32_t *k);
-/* Upper bound on the grid of the persistent eigen launches (0: as many
- * workgroups as fit on the device at once, the default).  No counterpart in
- * the reference; a tuning and test knob (a small grid makes every workgroup
- * take many problems from the queue). */
+/* Grid of the subarray eigen launches.  0 (the default): the combined
+ * full-array + subarray list runs one workgroup per problem, and the
+ * warm-start subarray launch is persistent with as many workgroups as fit on
+ * the device at once.  > 0: both run as persistent workgroups, at most this
+ * many, taking problems from an atomic queue.  Results are bit-identical
+ * either way.  No counterpart in the reference; a tuning and test knob (a
+ * small grid makes every workgroup take many problems from the queue). */
 int lam_set_persistent_workgroups(lam_ctx *ctx, int max_workgroups);
 /* Results of the last block, problem q = (c * 2 + pol) * ngroups + g, synchronous
  * host copies (any pointer may be NULL): evals [q][R] (first n_q valid,

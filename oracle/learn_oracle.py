@@ -62,9 +62,10 @@ def features(A_white, basis, d):
             x[2 * e + 1] = A_white[i, j].imag
             e += 1
     off = n * (n - 1)
-    b = np.sqrt(n) * np.asarray(basis).reshape(KW * n)
-    x[off:off + 2 * KW * n:2] = b.real
-    x[off + 1:off + 2 * KW * n:2] = b.imag
+    # componentwise (a complex product would turn a -0 imaginary part into +0)
+    b = np.asarray(basis).reshape(KW * n)
+    x[off:off + 2 * KW * n:2] = np.sqrt(n) * b.real
+    x[off + 1:off + 2 * KW * n:2] = np.sqrt(n) * b.imag
     return bf16_bits(x)
 
 

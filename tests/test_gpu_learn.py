@@ -103,7 +103,10 @@ def _max_feature_step(shp, block, basis_labels, x, d):
             A[ii[up], jj[up]] = vis[c, bl, pol, pol, 0] + 1j * vis[c, bl, pol, pol, 1]
             dd = np.sqrt(np.maximum(A.diagonal().real, 1e-10))
             Aw = A / np.outer(dd, dd)
-            basis = basis_labels[c * 2 + pol, ..., 0] + 1j * basis_labels[c * 2 + pol, ..., 1]
+            # real and imaginary parts set directly: 1j * x would turn a -0 into +0
+            basis = np.empty(basis_labels.shape[1:-1], np.complex128)
+            basis.real = basis_labels[c * 2 + pol, ..., 0]
+            basis.imag = basis_labels[c * 2 + pol, ..., 1]
             ref = lo.features(Aw, basis, d).astype(np.int32)
             got = x[c * 2 + pol].astype(np.int32)
             worst = max(worst, int(np.max(np.abs(got - ref))))

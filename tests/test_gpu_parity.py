@@ -336,8 +336,32 @@ def test_gains_and_fine_delays():
 
 
 # ---------------------------------------------------------- eigensolver
-@pytest.mark.parametrize("n", [2, 4, 5, 12, 24, 36, 64])
+def _special_hermitian(n, rng):
+    """Matrices that stress an eigensolver: degenerate spectra (the
+    all-ones rank-1 matrix: eigenvalue n and n - 1 zeros, the reference's
+    EigenvalueBasic shape), a diagonal one (already reduced), a complex
+    tridiagonal one, the zero matrix, and a strong rank-2 interferer over noise
+    whitened like a covariance."""
+    out = [np.ones((n, n), complex), np.diag(rng.standard_normal(n)).astype(complex), np.zeros((n, n), complex)]
+    T = np.diag(rng.standard_normal(n)).astype(complex)
+    if n > 1:
+        off = rng.standard_normal(n - 1) + 1j * rng.standard_normal(n - 1)
+        T += np.diag(off, -1) + np.diag(off.conj(), 1)
+    out.append(T)
+    X = rng.standard_normal((n, 4 * n)) + 1j * rng.standard_normal((n, 4 * n))
+    a = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    X += 30 * a @ (rng.standard_normal((2, 4 * n)) + 1j * rng.standard_normal((2, 4 * n)))
+    C = X @ X.conj().T
+    d = np.sqrt(np.diag(C).real)
+    out.append(C / np.outer(d, d))
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 12, 24, 36, 64])
 def test_eigh_batched_vs_oracle(n):
+    """The batched Jacobi against the FP64 oracle Jacobi: eigenvalues,
+    residuals and orthogonality, on random covariances and the special
+    matrices above."""
     _log(f"eigh n={n}")
     rng = np.random.default_rng(n)
     nmat = 7
@@ -345,6 +369,8 @@ def test_eigh_batched_vs_oracle(n):
     for _ in range(nmat):
         X = rng.standard_normal((n, 2 * n)) + 1j * rng.standard_normal((n, 2 * n))
         As.append(X @ X.conj().T / (2 * n))
+    As += _special_hermitian(n, rng)
+    nmat = len(As)
     As = np.array(As)
     buf = np.stack([As.transpose(0, 2, 1).real, As.transpose(0, 2, 1).imag], -1).astype(np.float32)
     dA = lam.DeviceArray.from_numpy(buf)
@@ -357,12 +383,12 @@ def test_eigh_batched_vs_oracle(n):
     V = V[..., 0] + 1j * V[..., 1]  # [m][k][i]
     for m in range(nmat):
         ref_ev, ref_V, _ = orc.jacobi_eigh(As[m])
-        scale = np.max(np.abs(ref_ev))
-        assert np.max(np.abs(ev[m] - ref_ev)) / scale < 2e-5
+        scale = max(np.max(np.abs(ref_ev)), 1e-30)
+        assert np.max(np.abs(ev[m] - ref_ev)) / scale < 2e-5, m
         U = V[m].T  # columns
-        resid = np.linalg.norm(As[m] @ U - U * ev[m][None, :]) / np.linalg.norm(As[m])
-        assert resid < 1e-4
-        assert np.allclose(U.conj().T @ U, np.eye(n), atol=1e-4)
+        resid = np.linalg.norm(As[m] @ U - U * ev[m][None, :]) / max(np.linalg.norm(As[m]), 1e-30)
+        assert resid < 1e-4, m
+        assert np.allclose(U.conj().T @ U, np.eye(n), atol=1e-4), m
 
 
 # ---------------------------------------------------------- adaptive path
@@ -663,7 +689,7 @@ def test_autotune_adaptive_warm_start():
     dev = [(lam.DeviceArray.from_numpy(r), lam.DeviceArray.from_numpy(s)) for r, s in blocks]
     res = p.autotune([d[0] for d in dev], [d[1] for d in dev], iters=1)
     _log(f"autotune {res}")
-    assert res["plain_threads"] in (256, 512, 1024) and res["refine_steps"] in (1, 2, 3)
+    assert res["plain_threads"] in (64, 128, 256) and res["refine_steps"] in (1, 2, 3)
     ring, scales = blocks[0]
     p.run_host(ring, scales)
     vis64 = p.dump_visibilities_i64()
