@@ -13,6 +13,7 @@
 // cyclic Jacobi method: each round applies n/2 disjoint complex rotations,
 // rotation angles are computed in FP64 from the FP32 matrix.
 #include <algorithm>
+#include <type_traits>
 
 #include "lam_internal.hpp"
 
@@ -134,198 +135,456 @@ __device__ __forceinline__ Rot make_rotation(float2 b, float app, float aqq) {
   return R;
 }
 
-// Per-thread block list of one Jacobi round for NTHR threads and n <= 64:
-// upper-triangle 2x2 blocks (at most 32 * 33 / 2).
-template <int NTHR> struct JacobiLists {
-  static constexpr int kMaxBlk = (32 * 33 / 2 + NTHR - 1) / NTHR;  // blocks per thread at n = 64
-};
+// Sizes of the Jacobi solver's LDS arrays.  jac_ne(n) = n + (n & 1): a zero
+// dummy row and column pad odd n.  The input matrix is column-major with
+// leading dimension jac_ne(n) (the solver's matrix buffers hold jac_ne(n)^2
+// entries).  The eigenvectors V are column-major with leading dimension
+// jac_ldv(n, nthr): its eigenvector threads give consecutive lanes of a pair
+// consecutive rows and consecutive pairs consecutive columns, so with
+// ldv = tpp (mod 32) (tpp: threads per pair) a wave's accesses tile the 32
+// bank pairs.
+__host__ __device__ constexpr int jac_ne(int n) { return n + (n & 1); }
+__host__ __device__ inline void jac_vplan(int n, int nthr, int &vm, int &tpp) {
+  const int h = jac_ne(n) >> 1, vcap = nthr > 64 ? nthr - 64 : nthr;
+  vm = (h * n + vcap - 1) / vcap;
+  while (h * ((n + vm - 1) / vm) > vcap) ++vm;
+  tpp = (n + vm - 1) / vm;
+}
+__host__ __device__ inline int jac_ldv(int n, int nthr) {
+  int vm, tpp;
+  jac_vplan(n, nthr, vm, tpp);
+  return n + (((tpp - n) % 32) + 32) % 32;
+}
 
 // LDS scratch of the Jacobi solver (carved by carve()).
 struct JacobiLds {
-  Rot *rot;    // [2][32]: rotations of two consecutive rounds
-  int2 *rpq;   // [2][32]: their pairs (p < q)
-  float *red;  // block reductions
+  float2 *A1;  // second matrix buffer, jac_ne(n)^2
+  Rot *rot;    // rotations [2][32], by round parity
+  float *dn;   // rotated diagonals [2][64], by round parity
+  float *red;  // per-wave partial sums
 };
 
-// Positions (p < q) of pair k in round r.
-__device__ __forceinline__ int2 rr_pair(int k, int r, int ne) {
-  const int a = rr_pos(k, r, ne), b = rr_pos(ne - 1 - k, r, ne);
-  return make_int2(min(a, b), max(a, b));
+// 1/sqrt(x) in FP64 for x in [1e-36, 1e36]: the FP32 hardware estimate and one
+// FP64 Newton step (about 46 correct bits), a shorter dependency than the FP64
+// estimate.
+__device__ __forceinline__ double rsq64f(double x) {
+  const double y = (double)__builtin_amdgcn_rsqf((float)x);
+  return y * (1.5 - 0.5 * (x * y) * y);
 }
 
-// Entry (r, c) of a Hermitian matrix kept as its upper triangle, without branches.
-__device__ __forceinline__ float2 herm_load(const float2 *A, int ld, int r, int c) {
-  const bool up = r <= c;
-  const float2 v = A[up ? c * ld + r : r * ld + c];
-  return make_float2(v.x, up ? v.y : -v.y);
-}
-__device__ __forceinline__ void herm_store(float2 *A, int ld, int r, int c, float2 v) {
-  const bool up = r <= c;
-  A[up ? c * ld + r : r * ld + c] = make_float2(v.x, up ? v.y : -v.y);
+// Rotation that annihilates a_pq, as make_rotation (angles in FP64), in a form
+// with a short dependent chain: with d = a_qq - a_pp, r = sqrt(d^2 + 4|b|^2),
+// w = r + |d|, z = 1/sqrt(2 r w):
+//   t = sgn(d) 2|b| / w  (= sgn(tau) / (|tau| + sqrt(1 + tau^2))),
+//   c = 1/sqrt(1 + t^2) = w z,  s = t c = sgn(d) 2|b| z,
+// and tb = t |b| = sgn(d) 2|b|^2 / w = sgn(d) 4 |b|^2 r z^2: the rotation leaves
+// the diagonal a_pp - tb, a_qq + tb (the rotated diagonal in closed form).
+// |a_pq|^2 below 1e-24 (a_pp^2 + a_qq^2) or 1e-36 is left in place (identity):
+// such a rotation changes no FP32 entry.  Entries are assumed below 1e18.
+__device__ __forceinline__ Rot make_rotation_t(float2 b, float app, float aqq, double &tb) {
+  const double br = b.x, bi = b.y, d = (double)aqq - (double)app;
+  const double b2 = br * br + bi * bi;
+  const bool rot = b2 > 1e-36 && b2 > 1e-24 * ((double)app * app + (double)aqq * aqq);
+  const double q = rot ? d * d + 4.0 * b2 : 1.0;
+  const double yb = rsq64f(rot ? b2 : 1.0), bm = b2 * yb;
+  const double r = q * rsq64f(q), w = r + fabs(d);
+  const double z = rsq64f(2.0 * r * w);
+  const double sg = d >= 0.0 ? 1.0 : -1.0;
+  Rot R;
+  R.c = rot ? (float)(w * z) : 1.0f;
+  R.s = rot ? (float)(sg * 2.0 * bm * z) : 0.0f;
+  R.er = rot ? (float)(br * yb) : 1.0f;
+  R.ei = rot ? (float)(bi * yb) : 0.0f;
+  tb = rot ? sg * 4.0 * b2 * r * z * z : 0.0;
+  return R;
 }
 
-// Cyclic Jacobi on the Hermitian matrix A (column-major, leading dim n | 1,
-// upper triangle; the lower one is never read and goes stale) in LDS; V
-// receives the eigenvectors (columns).  Each round rotates n/2 disjoint pairs
-// (round-robin order) in two phases:
-//   Y: all threads update the 2x2 blocks kp <= kq, A <- J^H A J;
-//   X: wave 0 computes the next round's rotations (FP64 angles, lanes
-//      0..h-1) from the updated matrix while the other waves apply this
-//      round's rotations to the eigenvectors, V <- V J, so the angle chain's
-//      latency hides behind the eigenvector work instead of stalling it.
-// The rotation and pair tables are double-buffered by round.  Returns the
-// number of sweeps.
+// Round-robin pairing in slot coordinates: pair k is slots (k, ne-1-k) in
+// every round; between rounds the contents of slots 1..ne-1 move down one
+// slot (slot 1 to slot ne-1), slot 0 stays.  After ne-1 rounds (a sweep) the
+// slots are back in matrix order.
+__device__ __forceinline__ int jac_sig(int s, int ne) { return s == 0 ? 0 : (s == 1 ? ne - 1 : s - 1); }
+__device__ __forceinline__ int jac_sig_inv(int t, int ne) { return t == 0 ? 0 : (t == ne - 1 ? 1 : t + 1); }
+__device__ __forceinline__ int jac_pair_of(int s, int ne) { return min(s, ne - 1 - s); }
+// Storage of the off-diagonal entry (row slot a, column slot b): in the
+// orientation with the row in the lower pair (within one pair: the lower
+// slot; conj when stored transposed), as entry e = 2 [row is the pair's high
+// slot] + [column is the high slot] of the 2x2 block (pair of row, pair of
+// column), in plane e: offset e h^2 + kp h + kq.  A round's thread reads its
+// block's entries from four planes and lanes of a wave hold neighbouring
+// blocks, so the accesses of a wave are mostly consecutive words.
+__device__ __forceinline__ int jac_slot_off(int a, int b, int ne, bool &conj) {
+  const int h = ne >> 1;
+  int pa = jac_pair_of(a, ne), pb = jac_pair_of(b, ne);
+  const bool keep = pa < pb || (pa == pb && a < b);
+  conj = !keep;
+  if (!keep) {
+    const int t = a; a = b; b = t;
+    const int u = pa; pa = pb; pb = u;
+  }
+  const int e = 2 * (a >= h) + (b >= h);
+  return (e * h + pa) * h + pb;
+}
+// Off-diagonal block (pair kp < pair kq) at list position pos of h pairs: the
+// blocks holding the next round's pivots first (position k' holds the pivot
+// of pair k' of the next round: (0,1), (k'-1,k'+1), (h-2,h-1)), then the
+// others row by row.
+__device__ __forceinline__ int2 jac_block(int pos, int h) {
+  if (h == 2) return make_int2(0, 1);
+  if (pos < h) return pos == 0 ? make_int2(0, 1) : (pos == h - 1 ? make_int2(h - 2, h - 1) : make_int2(pos - 1, pos + 1));
+  int j = pos - h;
+  for (int kp = 0; kp < h - 2; ++kp) {
+    const int cnt = (h - 1 - kp) - (kp <= h - 3) - (kp == 0);
+    if (j < cnt) return make_int2(kp, kp == 0 ? 3 + j : (j == 0 ? kp + 1 : kp + 2 + j));
+    j -= cnt;
+  }
+  return make_int2(0, 1);
+}
+
+// Parallel cyclic Jacobi on the Hermitian n x n matrix A (column-major,
+// leading dimension jac_ne(n), upper triangle read; n <= 64) in LDS; V
+// (leading dimension jac_ldv(n, NTHR)) receives the eigenvectors (columns) and
+// the diagonal of A the eigenvalues (unsorted); the rest of A is overwritten.
+// Returns the number of sweeps.
+//
+// Each round rotates the ne/2 disjoint pairs (round-robin order) with one
+// workgroup barrier.  The matrix lives in slot coordinates, double-buffered:
+// a round reads its entries from one buffer and writes the rotated ones to
+// the other at the slots they occupy in the next round, so every thread's
+// addresses are fixed for the whole solve.  Only the off-diagonal 2x2 blocks
+// of pair pairs are stored (one orientation); the annihilated pivots are
+// exact zeros and the diagonal is carried in closed form (a_pp - t|a_pq|,
+// a_qq + t|a_pq|) in a small table.  The thread whose block holds a pivot of
+// the next round computes that rotation (angles in FP64) right after its
+// update, so rotations never wait for a separate phase; threads of the last
+// waves apply the round's rotations to the eigenvectors, V <- V J.
 template <int NTHR>
 __device__ __forceinline__ int jacobi_lds(float2 *A, float2 *V, int n, int max_sweeps, float tol,
-                                          const JacobiLds &L, bool init_v = true) {
+                                          const JacobiLds &L) {
   constexpr int NW = NTHR / 64;
-  const int tid = threadIdx.x, w = tid >> 6;
-  const int ne = n + (n & 1), h = ne / 2, nr = ne - 1;
-  const int nblk = h * (h + 1) / 2;
-  const float inv_n = 1.0f / (float)n;
-  const int ld = n | 1;
-  for (int i = tid; i < n * n; i += NTHR) {
-    const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
-    if (init_v) V[c * ld + r] = make_float2(r == c ? 1.0f : 0.0f, 0.0f);
-    if (r == c) A[c * ld + r].y = 0.0f;
-  }
-  // this thread's blocks (kp <= kq) of the upper triangle, decoded once
-  constexpr int kMaxBlk = JacobiLists<NTHR>::kMaxBlk;
-  int blk[kMaxBlk];
-  int nmine = 0;
-#pragma unroll
-  for (int j = 0; j < kMaxBlk; ++j) {
-    const int idx = tid + j * NTHR;
-    blk[j] = 0;
-    if (idx < nblk) {
-      int kp = (int)((2.0f * h + 1.0f - sqrtf((2.0f * h + 1.0f) * (2.0f * h + 1.0f) - 8.0f * idx)) * 0.5f);
-      while (kp > 0 && kp * (2 * h - kp + 1) / 2 > idx) --kp;
-      while ((kp + 1) * (2 * h - kp) / 2 <= idx) ++kp;
-      const int kq = kp + idx - kp * (2 * h - kp + 1) / 2;
-      blk[j] = kp | (kq << 8);
-      nmine = j + 1;
-    }
-  }
-  // eigenvector update: the threads of waves 1.. (all of them with one wave),
-  // grouped by pair, vrg consecutive threads per pair, rows vr0, vr0 + vrg, ...
-  const int v0 = NW > 1 ? 64 : 0, nvt = NTHR - v0, vt = tid - v0;
-  const int vrg = nvt >= h ? nvt / h : 1;
-  const int vk = vt >= 0 ? vt / vrg : h, vr0 = vt >= 0 ? vt - vk * vrg : 0;
-  const bool vthread = vt >= 0 && vk < h;
-  // rotations of round r into table slot sl (lanes 0..h-1 of wave 0)
-  auto rotations = [&](int r, int sl) {
-    if (tid < h) {
-      const int2 P = rr_pair(tid, r, ne);
-      Rot R{1.0f, 0.0f, 1.0f, 0.0f};
-      if (P.y < n) R = make_rotation(A[P.y * ld + P.x], A[P.x * ld + P.x].x, A[P.y * ld + P.y].x);
-      L.rpq[sl * 32 + tid] = P;
-      L.rot[sl * 32 + tid] = R;
-    }
+  constexpr int MAXB = (32 * 31 / 2 + NTHR - 1) / NTHR;  // off-diagonal blocks per thread at n = 64
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int ne = jac_ne(n), h = ne >> 1, nr = ne - 1, lda = ne, ldv = jac_ldv(n, NTHR);
+  const int nb = h * (h - 1) / 2;
+  // the buffers of the rounds: parity 0 reads A1, parity 1 reads A
+  float2 *const B0 = L.A1, *const B1 = A;
+  // input entry (r, c) from the upper triangle, zero on the dummy
+  auto ain = [&](int r, int c) {
+    if (r >= n || c >= n) return make_float2(0.f, 0.f);
+    if (r == c) return make_float2(A[c * lda + c].x, 0.f);
+    return r < c ? A[c * lda + r] : make_float2(A[r * lda + c].x, -A[r * lda + c].y);
   };
-  // V <- V J for this thread's items of the round in slot sl, 4 items per
-  // batch (all loads of a batch before its stores)
-  auto vupdate = [&](int sl) {
-    if (!vthread) return;
-    const int2 Q = L.rpq[sl * 32 + vk];
-    if (Q.y >= n) return;
-    const Rot RQ = L.rot[sl * 32 + vk];
-    const float2 eb = make_float2(RQ.er, -RQ.ei);
-    for (int row0 = vr0; row0 < n; row0 += 4 * vrg) {
-      float2 vp[4], vq[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int row = min(row0 + u * vrg, n - 1);
-        vp[u] = V[Q.x * ld + row];
-        vq[u] = V[Q.y * ld + row];
+  // set-up: the off-diagonal entries into the planes of buffer 0, V = I, and
+  // the convergence measure of the input: off = sum of |a_ij|^2 over i != j,
+  // tot = off + sum a_ii^2
+  {
+    float off = 0.f, dg = 0.f;
+    const float inv_ne = 1.0f / (float)ne;
+    for (int i = tid; i < ne * ne; i += NTHR) {
+      const int c = (int)(((float)i + 0.5f) * inv_ne), r = i - c * ne;
+      const float2 a = ain(r, c);
+      if (r < c) {
+        bool cj;
+        const int o = jac_slot_off(r, c, ne, cj);
+        B0[o] = make_float2(a.x, cj ? -a.y : a.y);
       }
+      if (r == c) dg += a.x * a.x;
+      else off += a.x * a.x + a.y * a.y;
+      if (r < n && c < n) V[c * ldv + r] = make_float2(r == c ? 1.0f : 0.0f, 0.0f);
+    }
+    off = wave_sum(off);
+    dg = wave_sum(dg);
+    if (lane == 0) {
+      L.red[2 * w] = off;
+      L.red[2 * w + 1] = dg;
+    }
+    __syncthreads();
+    off = 0.f;
+    dg = 0.f;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int row = row0 + u * vrg;
-        const float2 y = cmul(eb, vq[u]);
-        if (row < n) {
-          V[Q.x * ld + row] = make_float2(RQ.c * vp[u].x - RQ.s * y.x, RQ.c * vp[u].y - RQ.s * y.y);
-          V[Q.y * ld + row] = make_float2(RQ.s * vp[u].x + RQ.c * y.x, RQ.s * vp[u].y + RQ.c * y.y);
-        }
+    for (int i = 0; i < NW; ++i) {
+      off += L.red[2 * i];
+      dg += L.red[2 * i + 1];
+    }
+    if (max_sweeps <= 0 || off == 0.0f || off <= tol * tol * (off + dg)) {
+      __syncthreads();
+      return 0;
+    }
+  }
+  // rotations of round 0 (table 0)
+  if (tid < h) {
+    const int k = tid, kb = ne - 1 - k;
+    double tbd;
+    const float app = ain(k, k).x, aqq = ain(kb, kb).x;
+    L.rot[k] = make_rotation_t(ain(k, kb), app, aqq, tbd);
+    L.dn[k] = (float)((double)app - tbd);
+    L.dn[kb] = (float)((double)aqq + tbd);
+  }
+  // this thread's blocks (slot j: list position tid + j NTHR): entry e = 2 i + j
+  // is (row slot i of pair kp, column slot j of pair kq); read offset, write
+  // offset in the other buffer, conj mask.  Threads without a block read
+  // entry 0 and do not store.
+  int rd[MAXB][4], wr[MAXB][4], bkp[MAXB], bkq[MAXB];
+  unsigned cjm[MAXB];
+  bool bval[MAXB];
+#pragma unroll
+  for (int j = 0; j < MAXB; ++j) {
+    const int pos = tid + j * NTHR;
+    bval[j] = pos < nb;
+    bkp[j] = bkq[j] = 0;
+    cjm[j] = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) rd[j][e] = wr[j][e] = 0;
+    if (bval[j]) {
+      const int2 B = jac_block(pos, h);
+      bkp[j] = B.x;
+      bkq[j] = B.y;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int a = (e >> 1) ? ne - 1 - B.x : B.x, b = (e & 1) ? ne - 1 - B.y : B.y;
+        bool cj;
+        rd[j][e] = (e * h + B.x) * h + B.y;
+        wr[j][e] = jac_slot_off(jac_sig(a, ne), jac_sig(b, ne), ne, cj);
+        cjm[j] |= cj ? (0x80000000u >> e) : 0u;
       }
     }
-  };
-  __syncthreads();
-  rotations(0, 0);
-  __syncthreads();
-  int g = 0, sweep = 0;
-  for (; sweep < max_sweeps; ++sweep) {
-    float off = 0.0f, tot = 0.0f;
-    for (int i = tid; i < n * n; i += NTHR) {  // upper triangle; off-diagonal entries count twice
-      const int c = (int)(((float)i + 0.5f) * inv_n), r = i - c * n;
-      if (r > c) continue;
-      const float2 a = A[c * ld + r];
-      const float m = a.x * a.x + a.y * a.y;
-      tot += r == c ? m : 2.0f * m;
-      if (r != c) off += 2.0f * m;
-    }
-    off = block_sum(off, L.red);
-    tot = block_sum(tot, L.red);
-    if (off == 0.0f || off <= tol * tol * tot) break;
-    for (int r = 0; r < nr; ++r, ++g) {
-      const int sl = g & 1;
-      // Y: A <- J^H A J on blocks (kp <= kq), written with their transposes
-      const int2 *pq = L.rpq + sl * 32;
-      const Rot *rt = L.rot + sl * 32;
+  }
+  // pivot tasks (wave 0): the next round's rotation of pair tk from entry te
+  // of block 0 (te < 0: the pivot is the zero just annihilated, n <= 2), its
+  // diagonal from slots ta, tq of this round's table.  One task per thread
+  // k < h; with h = 2 thread 0 has both.
+  int tk[2] = {0, 0}, te[2] = {-1, -1}, ta[2] = {0, 0}, tq[2] = {0, 0};
+  bool tval[2] = {false, false};
+  {
+    const int first = h == 2 ? 0 : tid, count = h == 2 ? (tid == 0 ? 2 : 0) : (tid < h ? 1 : 0);
 #pragma unroll
-      for (int j = 0; j < kMaxBlk; ++j) {
-        if (j >= nmine) continue;
-        const int code = blk[j];
-        const int kp = code & 0xff, kq = code >> 8;
-        const int2 P = pq[kp], Q = pq[kq];
-        const Rot RP = rt[kp], RQ = rt[kq];
-        const bool pv = P.y < n, qv = Q.y < n;  // the second index of a pair may be the dummy
-        float2 x00 = herm_load(A, ld, P.x, Q.x);
-        float2 x01 = herm_load(A, ld, P.x, qv ? Q.y : Q.x);
-        float2 x10 = herm_load(A, ld, pv ? P.y : P.x, Q.x);
-        float2 x11 = herm_load(A, ld, pv ? P.y : P.x, qv ? Q.y : Q.x);
-        if (!qv) x01 = x11 = make_float2(0.f, 0.f);
-        if (!pv) x10 = x11 = make_float2(0.f, 0.f);
-        rotate_block(RP, RQ, x00, x01, x10, x11);
-        if (kp == kq) {  // the annihilated pair: exact zeros, real diagonal (P.x < P.y)
-          x00.y = 0.f;
-          x11.y = 0.f;
-          A[P.x * ld + P.x] = x00;
-          if (pv) {
-            A[P.y * ld + P.x] = make_float2(0.f, 0.f);
-            A[P.y * ld + P.y] = x11;
+    for (int t = 0; t < 2; ++t) {
+      if (t < count) {
+        const int kn = first + t, a0 = jac_sig_inv(kn, ne), b0 = jac_sig_inv(ne - 1 - kn, ne);
+        int e = -1;
+        if (h > 1) {
+          for (int ee = 0; ee < 4; ++ee) {
+            const int a = (ee >> 1) ? ne - 1 - bkp[0] : bkp[0], b = (ee & 1) ? ne - 1 - bkq[0] : bkq[0];
+            if ((a == a0 && b == b0) || (a == b0 && b == a0)) e = ee;
           }
-        } else {
-          herm_store(A, ld, P.x, Q.x, x00);
-          if (qv) herm_store(A, ld, P.x, Q.y, x01);
-          if (pv) herm_store(A, ld, P.y, Q.x, x10);
-          if (pv && qv) herm_store(A, ld, P.y, Q.y, x11);
         }
+        tk[t] = kn;
+        te[t] = e;
+        ta[t] = a0;
+        tq[t] = b0;
+        tval[t] = true;
       }
-      __syncthreads();
-      // X: next round's rotations (wave 0) beside this round's V <- V J
-      const int r1 = r + 1 < nr ? r + 1 : 0;
-      if (NW == 1) {
-        rotations(r1, sl ^ 1);
-        vupdate(sl);
-      } else if (w == 0) {
-        rotations(r1, sl ^ 1);
-      } else {
-        vupdate(sl);
-      }
-      __syncthreads();
     }
   }
+  // the zero that replaces this round's pivot of pair tid in the next buffer
+  int zoff = 0;
+  if (tid < h) {
+    bool cj;
+    zoff = jac_slot_off(jac_sig(tid, ne), jac_sig(ne - 1 - tid, ne), ne, cj);
+  }
+  // eigenvector items: (pair vk, rows vj, vj + tpp, ...) on the last nvt threads
+  // (wave 0 keeps the rotations), at most vm rows each; consecutive lanes take
+  // consecutive rows, so an access spreads over the LDS banks
+  int vm, tpp;
+  jac_vplan(n, NTHR, vm, tpp);
+  const int nvt = h * tpp, vt = tid - (NTHR - nvt);
+  const bool vthread = vt >= 0;
+  const int vk = vthread ? vt / tpp : 0, vj = vthread ? vt - vk * tpp : 0;
+  const int vrows = vthread ? (n - vj + tpp - 1) / tpp : 0;
+  int vcp = vk, vcq = ne - 1 - vk;  // matrix columns of slots vk, ne-1-vk in round 0
+  int vop = vcp * ldv + vj, voq = vcq * ldv + vj;  // their element offsets (row vj)
+  int vro[4];  // offsets of the first four rows (the last valid row repeated)
+#pragma unroll
+  for (int u = 0; u < 4; ++u) vro[u] = min(u, max(vrows - 1, 0)) * tpp;
+  // per-wave roles (uniform): block slots, pivot tasks, eigenvector items
+  const int wlo = w * 64;
+  const bool wblk0 = wlo < nb, wblk1 = MAXB > 1 && wlo + NTHR < nb;
+  const bool wtask = w == 0;
+  const bool wv = wlo + 64 > NTHR - nvt;
+  __syncthreads();
+
+  // one block: A <- J_P^H A J_Q on its 2x2 entries, stored rotated and moved
+  auto block = [&](int j, const float2 *Sc, float2 *Sn, const Rot *rc, float2 (&x)[4], bool last,
+                   float &offp) {
+    const Rot RP = rc[bkp[j]], RQ = rc[bkq[j]];
+    rotate_block(RP, RQ, x[0], x[1], x[2], x[3]);
+    const unsigned m = cjm[j];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) x[e].y = __uint_as_float(__float_as_uint(x[e].y) ^ ((m << e) & 0x80000000u));
+    if (bval[j]) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) Sn[wr[j][e]] = x[e];
+    }
+    if (last) {  // uniform: the last round of a sweep
+      const float m2 = x[0].x * x[0].x + x[0].y * x[0].y + x[1].x * x[1].x + x[1].y * x[1].y +
+                       x[2].x * x[2].x + x[2].y * x[2].y + x[3].x * x[3].x + x[3].y * x[3].y;
+      offp += bval[j] ? 2.0f * m2 : 0.0f;
+    }
+  };
+  // V <- V J on up to 4 rows (stride tpp) of this thread's item
+  auto vrot = [&](const Rot &R, float2 *cp, float2 *cq, const float2 (&vp)[4], const float2 (&vq)[4], int rows) {
+    const float2 eb = make_float2(R.er, -R.ei);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float2 y = cmul(eb, vq[u]);
+      if (u < rows) {
+        cp[u * tpp] = make_float2(R.c * vp[u].x - R.s * y.x, R.c * vp[u].y - R.s * y.y);
+        cq[u * tpp] = make_float2(R.s * vp[u].x + R.c * y.x, R.s * vp[u].y + R.c * y.y);
+      }
+    }
+  };
+
+  // One round, straight-line per wave role: every load first (the next
+  // pivots' diagonal, block 0, the first eigenvector rows), then block 0 and
+  // the next rotations (wave 0), further blocks, the eigenvector update.
+  auto round = [&](auto parity, auto task, bool last) {
+    constexpr int PAR = decltype(parity)::value;
+    constexpr bool TASK = decltype(task)::value;
+    const float2 *Sc = PAR ? B1 : B0;
+    float2 *Sn = PAR ? B0 : B1;
+    const Rot *rc = L.rot + PAR * 32;
+    Rot *rn = L.rot + (1 - PAR) * 32;
+    const float *dc = L.dn + PAR * 64;
+    float *dnx = L.dn + (1 - PAR) * 64;
+    float tpa[2] = {0.f, 0.f}, tpq[2] = {0.f, 0.f};
+    if constexpr (TASK) {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        tpa[t] = dc[ta[t]];
+        tpq[t] = dc[tq[t]];
+      }
+    }
+    float2 x[4];
+    if (wblk0) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[e] = Sc[rd[0][e]];
+    }
+    const bool vact = vthread && vcp < n && vcq < n;
+    float2 *cp = V + vop, *cq = V + voq;
+    float2 vp[4], vq[4];
+    Rot RV;
+    if (wv) {
+      RV = rc[vk];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        vp[u] = cp[vro[u]];
+        vq[u] = cq[vro[u]];
+      }
+    }
+    float offp = 0.f;
+    if (wblk0) {
+      block(0, Sc, Sn, rc, x, last, offp);
+      if constexpr (TASK) {
+        // the next round's rotations of the pivots this thread holds
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          if (t == 0 || h == 2) {
+            const int e = te[t];
+            const float2 pv = e == 0 ? x[0] : (e == 1 ? x[1] : (e == 2 ? x[2] : (e == 3 ? x[3] : make_float2(0.f, 0.f))));
+            double tbd;
+            const Rot R = make_rotation_t(pv, tpa[t], tpq[t], tbd);
+            if (tval[t]) {
+              rn[tk[t]] = R;
+              dnx[tk[t]] = (float)((double)tpa[t] - tbd);
+              dnx[ne - 1 - tk[t]] = (float)((double)tpq[t] + tbd);
+            }
+          }
+        }
+      }
+    } else if constexpr (TASK) {
+      // n <= 2: no blocks; the next rotation is the identity on the rotated diagonal
+      if (tval[0]) {
+        rn[tk[0]] = Rot{1.0f, 0.0f, 1.0f, 0.0f};
+        dnx[tk[0]] = tpa[0];
+        dnx[ne - 1 - tk[0]] = tpq[0];
+      }
+    }
+    if (tid < h) Sn[zoff] = make_float2(0.f, 0.f);
+    if (wblk1) {
+#pragma unroll
+      for (int j = 1; j < MAXB; ++j) {
+        if (wlo + j * NTHR < nb) {
+          float2 y[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) y[e] = Sc[rd[j][e]];
+          block(j, Sc, Sn, rc, y, last, offp);
+        }
+      }
+    }
+    if (wv) {
+      if (vact) vrot(RV, cp, cq, vp, vq, vrows);
+      for (int r0 = 4; r0 < vrows; r0 += 4) {
+        float2 wp[4], wq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int o = min(r0 + u, vrows - 1) * tpp;
+          wp[u] = cp[o];
+          wq[u] = cq[o];
+        }
+        if (vact) vrot(RV, cp + r0 * tpp, cq + r0 * tpp, wp, wq, vrows - r0);
+      }
+    }
+    // next round: slot contents move down one slot, so the matrix column of a
+    // slot (other than 0) advances by one, ne - 1 wrapping to 1
+    if (vk != 0) {
+      const bool wp = vcp + 1 == ne;
+      vcp = wp ? 1 : vcp + 1;
+      vop = wp ? ldv + vj : vop + ldv;
+    }
+    {
+      const bool wq = vcq + 1 == ne;
+      vcq = wq ? 1 : vcq + 1;
+      voq = wq ? ldv + vj : voq + ldv;
+    }
+    if (last) {
+      offp = wave_sum(offp);
+      if (lane == 0) L.red[w] = offp;
+    }
+    __syncthreads();
+  };
+  auto round_any = [&](auto parity, bool last) {
+    if (wtask) round(parity, std::true_type{}, last);
+    else round(parity, std::false_type{}, last);
+  };
+  // converged after the last round of a sweep (table par holds its diagonal)
+  auto converged = [&](int par) {
+    float offs = 0.f;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) offs += L.red[i];
+    const float d = lane < ne ? L.dn[par * 64 + lane] : 0.f;
+    const float tot = offs + wave_sum(d * d);
+    return offs == 0.0f || offs <= tol * tol * tot;
+  };
+  int sweep = 0, r = 0, par_last = 0;
+  for (;;) {
+    round_any(std::integral_constant<int, 0>{}, r == nr - 1);
+    if (++r == nr) {
+      r = 0;
+      par_last = 0;
+      if (++sweep >= max_sweeps || converged(0)) break;
+    }
+    round_any(std::integral_constant<int, 1>{}, r == nr - 1);
+    if (++r == nr) {
+      r = 0;
+      par_last = 1;
+      if (++sweep >= max_sweeps || converged(1)) break;
+    }
+  }
+  // eigenvalues onto the diagonal of A: matrix index i sat in slot
+  // jac_sig_inv(i) of the last round
+  if (tid < n) A[tid * lda + tid] = make_float2(L.dn[par_last * 64 + jac_sig_inv(tid, ne)], 0.f);
+  __syncthreads();
   return sweep;
 }
 
 // Ascending order of the diagonal of A: perm[rank] = column, ev[rank] = value.
 __device__ __forceinline__ void sort_eigen(const float2 *A, int n, int *perm, float *ev) {
-  const int j = threadIdx.x, ld = n | 1;
+  const int j = threadIdx.x, lda = jac_ne(n);
   if (j < n) {
-    const float lj = A[j * ld + j].x;
+    const float lj = A[j * lda + j].x;
     int rank = 0;
     for (int k = 0; k < n; ++k) {
-      const float lk = A[k * ld + k].x;
+      const float lk = A[k * lda + k].x;
       rank += (lk < lj) || (lk == lj && k < j);
     }
     perm[rank] = j;
@@ -345,41 +604,46 @@ __device__ __forceinline__ float pct_linear(const float *v, int n, float q) {
 }
 
 struct EigSmem {
-  float2 *A, *V, *W, *G;
+  float2 *A, *V, *W, *G;           // A: leading dimension jac_ne(n); V: ldv
+  int ldv;
   float2 *A0, *Y, *hv, *t1, *t2;  // warm start: copy of A, proposal, reflectors, scratch
   float *ev, *d, *f, *red, *htau, *res;
   int *perm;
   int2 *rpq;                      // small (warm-start) Jacobi: pairs (shares pivo)
   Rot *rot;                       // small (warm-start) Jacobi: rotations (shares rotw)
   Rot *rot2;                      // Jacobi solver: rotations [2][32]
-  int2 *rpq2;                     // Jacobi solver: pairs [2][32]
+  int2 *rpq2;                     // Jacobi solver: rotated diagonals [2][64] (as floats)
   int *kdet, *mark;
-  __device__ JacobiLds jl() const { return JacobiLds{rot2, rpq2, red}; }
+  __device__ JacobiLds jl() const { return JacobiLds{W, rot2, reinterpret_cast<float *>(rpq2), red}; }
 };
 
-// LDS layout.  The warm-start buffers exist only in the warm kernel; its copy
-// of A shares the space of W and G (loaded after the eigen solve) when that
-// is large enough.  In the plain kernel G (formed after the sorted basis has
-// been copied out of V) reuses V: 32.7 KB instead of 37.3 KB at n = 36,
-// B = 16, so three eigen workgroups leave room on a CU for a voltage
-// workgroup of the neighbouring blocks (55 KB) instead of blocking it.
+// LDS layout.  The warm-start buffers exist only in the warm kernel.  W
+// (loaded after the eigen solve) also holds the solver's second matrix buffer
+// and the warm start's copy of A (used before any plain solve).  In the plain
+// kernel G (formed after the sorted basis has been copied out of V) reuses V:
+// 34.0 KB at n = 36, B = 16, so three eigen workgroups leave room on a CU for
+// a voltage workgroup of the neighbouring blocks (55 KB) instead of blocking it.
+__host__ __device__ inline int eig_w_elems(int n, int B, bool warm, int nthr) {
+  const int m = jac_ne(n) * jac_ne(n), mv = jac_ne(n) * jac_ldv(n, nthr);
+  const bool g_in_v = !warm && B * n <= mv;  // G (B x n) fits in V
+  const int wb = (g_in_v ? 2 : 3) * B * n;
+  return wb > m ? wb : m;
+}
+
 __device__ __forceinline__ EigSmem carve(char *base, int n, int B, bool warm, int nw) {
   EigSmem s;
-  const int ld = n | 1;  // odd column stride of A: conflict-free strided LDS access
+  const int m = jac_ne(n) * jac_ne(n);
+  s.ldv = jac_ldv(n, nw * 64);
+  const int mv = jac_ne(n) * s.ldv;
   s.A = reinterpret_cast<float2 *>(base);
-  s.V = s.A + ld * n;
-  s.W = s.V + ld * n;
-  const bool g_in_v = !warm && B <= ld;  // G (B x n) fits in V's ld x n
+  s.V = s.A + m;
+  s.W = s.V + mv;
+  const bool g_in_v = !warm && B * n <= mv;
   s.G = g_in_v ? s.V : s.W + 2 * B * n;
-  float2 *q = s.W + (g_in_v ? 2 : 3) * B * n;
+  float2 *q = s.W + eig_w_elems(n, B, warm, nw * 64);
   s.A0 = s.Y = s.hv = s.t1 = s.t2 = nullptr;
   if (warm) {
-    if (3 * B * n >= ld * n) {
-      s.A0 = s.W;
-    } else {
-      s.A0 = q;
-      q += ld * n;
-    }
+    s.A0 = s.W;
     s.Y = q;
     s.hv = s.Y + kMaxWarm * n;
     s.t1 = s.hv + kMaxWarm * n;
@@ -395,7 +659,10 @@ __device__ __forceinline__ EigSmem carve(char *base, int n, int B, bool warm, in
   s.res = fp + 216;  // kMaxWarm * 64 (warm only)
   s.perm = reinterpret_cast<int *>(s.res + (warm ? kMaxWarm * 64 : 0));
   s.mark = s.perm + 64;
-  s.rot2 = reinterpret_cast<Rot *>((reinterpret_cast<uintptr_t>(s.mark + 64) + 15) & ~uintptr_t(15));
+  // 16-byte alignment by offset arithmetic on the LDS base (an integer cast of
+  // the pointer would turn every table access into a flat access)
+  const long moff = (reinterpret_cast<char *>(s.mark + 64) - base + 15) & ~15L;
+  s.rot2 = reinterpret_cast<Rot *>(base + moff);
   s.rpq2 = reinterpret_cast<int2 *>(s.rot2 + 64);
   s.kdet = reinterpret_cast<int *>(s.rpq2 + 64);
   // the warm start's small Jacobi (wave 0, never beside the solver) reuses them
@@ -405,11 +672,18 @@ __device__ __forceinline__ EigSmem carve(char *base, int n, int B, bool warm, in
   return s;
 }
 
+size_t eig_smem_bytes(int n, int B, bool warm, int nw);
+// LDS of a workgroup that takes problems of any size up to max_n (the
+// eigenvector stride, hence the size, is not monotone in n)
+size_t eig_smem_bytes_upto(int max_n, int B, bool warm, int nw) {
+  size_t m = 0;
+  for (int n = 1; n <= max_n; ++n) m = std::max(m, eig_smem_bytes(n, B, warm, nw));
+  return m;
+}
+
 size_t eig_smem_bytes(int n, int B, bool warm, int nw) {
-  const int ld = n | 1;
-  const bool g_in_v = !warm && B <= ld;  // as in carve()
-  size_t f2 = 2 * (size_t)ld * n + (g_in_v ? 2 : 3) * (size_t)B * n;
-  if (warm) f2 += (3 * B * n >= ld * n ? 0 : (size_t)ld * n) + 2 * kMaxWarm * n + 128;
+  size_t f2 = (size_t)jac_ne(n) * (jac_ne(n) + jac_ldv(n, nw * 64)) + eig_w_elems(n, B, warm, nw * 64);
+  if (warm) f2 += 2 * kMaxWarm * n + 128;
   return sizeof(float2) * f2 + sizeof(float) * (216 + (warm ? kMaxWarm * 64 : 0)) + sizeof(int) * 128 +
          15 + sizeof(Rot) * 64 + sizeof(int2) * 64 + sizeof(int) * 4 + 0 * nw;
 }
@@ -659,7 +933,7 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
       z.x += t2.x;
       z.y += t2.y;
     }
-    S.V[(m + k) * ld + i] = u;
+    S.V[(m + k) * S.ldv + i] = u;
     z.x -= th * u.x;
     z.y -= th * u.y;
     S.res[k * 64 + i] = z.x * z.x + z.y * z.y;
@@ -672,7 +946,7 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
   if (tid < 64) {
     const int i = tid;
     for (int k = 0; k < KW; ++k)
-      if (i < n) Y[k * n + i] = S.V[(m + k) * ld + i];
+      if (i < n) Y[k * n + i] = S.V[(m + k) * S.ldv + i];
     for (int e = i; e < KW * KW; e += 64) Tm[e] = make_float2(0.f, 0.f);
     wave_sync();
     float2 vk[kMaxWarm];
@@ -751,14 +1025,14 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
     S.A0[e] = q;
   }
   __syncthreads();
-  for (int e = tid; e < m * n; e += nthr) S.V[(e / n) * ld + e % n] = S.A0[e];
+  for (int e = tid; e < m * n; e += nthr) S.V[(e / n) * S.ldv + e % n] = S.A0[e];
   __syncthreads();
   // Rayleigh quotients of the complement: lambda_j = q_j^H A q_j
   for (int e = tid; e < m * n; e += nthr) {  // W = A Q2 into A0
     const int j = e / n, i = e - j * n;
     float2 w = make_float2(0.f, 0.f);
     for (int c2 = 0; c2 < n; ++c2) {
-      const float2 t = cmul(A[c2 * ld + i], S.V[j * ld + c2]);
+      const float2 t = cmul(A[c2 * ld + i], S.V[j * S.ldv + c2]);
       w.x += t.x;
       w.y += t.y;
     }
@@ -770,7 +1044,7 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
     if (j < m) {
       float lam = 0.0f;
       for (int i = 0; i < n; ++i) {
-        const float2 q = S.V[j * ld + i], w = S.A0[j * n + i];
+        const float2 q = S.V[j * S.ldv + i], w = S.A0[j * n + i];
         lam += q.x * w.x + q.y * w.y;
       }
       evu[j] = lam;
@@ -862,7 +1136,7 @@ constexpr int kRefineModes = 4;
 template <typename Cov>
 __device__ __forceinline__ void refine_removed_eigenvalues(EigSmem &S, int n, int K, int whiten, Cov cov) {
   constexpr int kJc = 6;
-  const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1, lane = tid & 63, w = tid >> 6, nw = nthr >> 6;
+  const int tid = threadIdx.x, nthr = blockDim.x, lda = jac_ne(n), lane = tid & 63, w = tid >> 6, nw = nthr >> 6;
   const int nch = (n + kJc - 1) / kJc;
   float2 *red = S.A;  // [kRefineModes][waves] (hi, lo)
   for (int m0 = 0; m0 < K; m0 += kRefineModes) {
@@ -884,7 +1158,7 @@ __device__ __forceinline__ void refine_removed_eigenvalues(EigSmem &S, int n, in
 #pragma unroll
       for (int m = 0; m < kRefineModes; ++m) {
         if (m >= km) continue;
-        const float2 *u = S.V + S.perm[n - 1 - (m0 + m)] * ld;
+        const float2 *u = S.V + S.perm[n - 1 - (m0 + m)] * S.ldv;
         DF yr{0.f, 0.f}, yi{0.f, 0.f};
 #pragma unroll
         for (int jj = 0; jj < kJc; ++jj) {
@@ -920,7 +1194,7 @@ __device__ __forceinline__ void refine_removed_eigenvalues(EigSmem &S, int n, in
     if (tid < km) {
       DF t{0.f, 0.f};
       for (int ww = 0; ww < nw; ++ww) t = df_add(t, DF{red[tid * nw + ww].x, red[tid * nw + ww].y});
-      const float2 *u = S.V + S.perm[n - 1 - (m0 + tid)] * ld;
+      const float2 *u = S.V + S.perm[n - 1 - (m0 + tid)] * S.ldv;
       float nrm = 0.0f;
       for (int i = 0; i < n; ++i) nrm += u[i].x * u[i].x + u[i].y * u[i].y;
       S.ev[n - 1 - (m0 + tid)] = (t.hi + t.lo) / nrm;
@@ -970,7 +1244,7 @@ template <int NTHR, bool WARM>
 __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_raw, int c, int p) {
   const int n = P.R, B = P.B;
   EigSmem S = carve(smem_raw, n, B, WARM, NTHR / 64);
-  const int tid = threadIdx.x, nthr = blockDim.x, ld = n | 1;
+  const int tid = threadIdx.x, nthr = blockDim.x, lda = jac_ne(n);
   const long nbl = (long)n * (n + 1) / 2;
 
   // 1. dense Hermitian covariance of pol p (A(i,j) = V[bl(i,j)][p][p], i <= j)
@@ -986,21 +1260,21 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
       im += v.y;
     }
     const float2 a = make_float2((float)re, (float)im);
-    S.A[j * ld + i] = a;
-    S.A[i * ld + j] = make_float2(a.x, -a.y);
+    S.A[j * lda + i] = a;
+    S.A[i * lda + j] = make_float2(a.x, -a.y);
   }
   __syncthreads();
   // 2. whitening (spatial.cuh:1065-1093); off: the raw covariance, as the
   //    pulsar-fold pipeline's RFI branch decomposes it (d = 1 makes the
   //    unwhitening below a no-op)
-  if (tid < n) S.d[tid] = P.whiten ? sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f)) : 1.0f;
+  if (tid < n) S.d[tid] = P.whiten ? sqrtf(fmaxf(S.A[tid * lda + tid].x, 1e-10f)) : 1.0f;
   __syncthreads();
   if (P.whiten) {
     for (int i = tid; i < n * n; i += nthr) {
       const int col = i / n, row = i - col * n;
       const float den = S.d[row] * S.d[col];
-      S.A[col * ld + row].x /= den;
-      S.A[col * ld + row].y /= den;
+      S.A[col * lda + row].x /= den;
+      S.A[col * lda + row].y /= den;
     }
     __syncthreads();
   }
@@ -1014,7 +1288,7 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
     for (int e = tid; e < KW * n; e += nthr) S.Y[e] = P.proposal[cp * P.prop_stride + e];
     __syncthreads();
     K = max(0, min(n, fixed));
-    const bool ok = warm_solve(S.A, ld, n, KW, K, P.refine_sweeps, P.res_tol, S);
+    const bool ok = warm_solve(S.A, lda, n, KW, K, P.refine_sweeps, P.res_tol, S);
     sweeps = P.refine_sweeps;
     if (ok) {
       if (tid == 0) S.kdet[0] = K;
@@ -1058,7 +1332,7 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
   if (P.basis_out) {
     for (int e = tid; e < P.KW * n; e += nthr) {
       const int k = e / n, i = e - k * n;
-      P.basis_out[cp * P.KW * n + e] = k < n ? S.V[S.perm[n - 1 - k] * ld + i] : make_float2(0.f, 0.f);
+      P.basis_out[cp * P.KW * n + e] = k < n ? S.V[S.perm[n - 1 - k] * S.ldv + i] : make_float2(0.f, 0.f);
     }
   }
   if (tid == 0 && P.fallback) P.fallback[cp] = fell;
@@ -1072,7 +1346,7 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
       double re = 0.0, im = 0.0;
 #pragma unroll 1
       for (int i = 0; i < n; ++i) {
-        const float2 x = S.V[ca * ld + i], y = S.V[cb * ld + i];
+        const float2 x = S.V[ca * S.ldv + i], y = S.V[cb * S.ldv + i];
         re += (double)x.x * y.x + (double)x.y * y.y;
         im += (double)x.x * y.y - (double)x.y * y.x;
       }
@@ -1087,7 +1361,7 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
     const int src = S.perm[tid];
     float nrm = 0.0f;
     for (int i = 0; i < n; ++i) {
-      float2 v = S.V[src * ld + i];
+      float2 v = S.V[src * S.ldv + i];
       if (!P.shrink) {
         v.x *= S.d[i];
         v.y *= S.d[i];
@@ -1240,7 +1514,7 @@ __device__ __forceinline__ void subarray_problem(const SubParams &P, char *smem_
   if (n == 0) return;
   const int B = P.wop ? P.B : 0;
   EigSmem S = carve(smem_raw, n, B, false, kEigThreads / 64);
-  const int ld = n | 1;
+  const int lda = jac_ne(n);
   const int nb = n * (n + 1) / 2;
   auto cov = [&](int i, int j) {  // exact A(i, j), i <= j, of the group's antennas
     const int ai = s_ant[i], aj = s_ant[j];
@@ -1260,17 +1534,17 @@ __device__ __forceinline__ void subarray_problem(const SubParams &P, char *smem_
     const int i = idx - j * (j + 1) / 2;
     const longlong2 v = cov(i, j);
     const float2 a = make_float2((float)v.x, (float)v.y);
-    S.A[j * ld + i] = a;
-    S.A[i * ld + j] = make_float2(a.x, -a.y);
+    S.A[j * lda + i] = a;
+    S.A[i * lda + j] = make_float2(a.x, -a.y);
   }
   __syncthreads();
-  if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f));
+  if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * lda + tid].x, 1e-10f));
   __syncthreads();
   for (int i = tid; i < n * n; i += nthr) {
     const int col = i / n, row = i - col * n;
     const float den = S.d[row] * S.d[col];
-    S.A[col * ld + row].x /= den;
-    S.A[col * ld + row].y /= den;
+    S.A[col * lda + row].x /= den;
+    S.A[col * lda + row].y /= den;
   }
   __syncthreads();
   const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, P.max_sweeps, P.tol, S.jl());
@@ -1282,7 +1556,7 @@ __device__ __forceinline__ void subarray_problem(const SubParams &P, char *smem_
     const int src = S.perm[tid];
     float nrm = 0.0f;
     for (int i = 0; i < n; ++i) {
-      float2 v = S.V[src * ld + i];
+      float2 v = S.V[src * S.ldv + i];
       v.x *= S.d[i];
       v.y *= S.d[i];
       S.A[tid * n + i] = v;
@@ -1415,7 +1689,7 @@ __global__ void __launch_bounds__(kEigThreads) eig_list_kernel(EigParams E, SubP
 hipError_t launch_eig_queue(const EigParams &e, const SubParams &q, const int *order, int ntotal, int *counter,
                             int num_cus, int max_wgs, hipStream_t s) {
   const size_t sm = std::max(eig_smem_bytes(e.R, e.B, false, kEigThreads / 64),
-                             eig_smem_bytes(q.max_n, q.wop ? q.B : 0, false, kEigThreads / 64));
+                             eig_smem_bytes_upto(q.max_n, q.wop ? q.B : 0, false, kEigThreads / 64));
   if (sm > 65536) {
     const hipError_t er = hipFuncSetAttribute(reinterpret_cast<const void *>(eig_queue_kernel),
                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -1440,7 +1714,7 @@ hipError_t launch_eig_queue(const EigParams &e, const SubParams &q, const int *o
 
 hipError_t launch_subarray_null(const SubParams &p, int num_cus, int max_wgs, hipStream_t s) {
   // LDS for the largest group; as many workgroups as fit on the machine at once
-  const size_t sm = eig_smem_bytes(p.max_n, p.wop ? p.B : 0, false, kEigThreads / 64);
+  const size_t sm = eig_smem_bytes_upto(p.max_n, p.wop ? p.B : 0, false, kEigThreads / 64);
   if (sm > 65536) {
     const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(subarray_null_kernel),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
@@ -1460,17 +1734,17 @@ eigh_batched_kernel(const float2 *Ain, int n, int whiten, int max_sweeps, float 
                     float *evals, float2 *evecs, int *sweeps_out) {
   extern __shared__ __align__(16) char smem_raw[];
   EigSmem S = carve(smem_raw, n, 0, false, kEigThreads / 64);
-  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
-  for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * ld + i % n] = Ain[(long)m * n * n + i];
+  const int m = blockIdx.x, tid = threadIdx.x, lda = jac_ne(n);
+  for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * lda + i % n] = Ain[(long)m * n * n + i];
   __syncthreads();
   if (whiten) {
-    if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * ld + tid].x, 1e-10f));
+    if (tid < n) S.d[tid] = sqrtf(fmaxf(S.A[tid * lda + tid].x, 1e-10f));
     __syncthreads();
     for (int i = tid; i < n * n; i += blockDim.x) {
       const int col = i / n, row = i - col * n;
       const float den = S.d[row] * S.d[col];
-      S.A[col * ld + row].x /= den;
-      S.A[col * ld + row].y /= den;
+      S.A[col * lda + row].x /= den;
+      S.A[col * lda + row].y /= den;
     }
     __syncthreads();
   }
@@ -1479,7 +1753,7 @@ eigh_batched_kernel(const float2 *Ain, int n, int whiten, int max_sweeps, float 
   if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
   for (int i = tid; i < n * n; i += blockDim.x) {
     const int k = i / n, r = i % n;
-    evecs[(long)m * n * n + i] = S.V[S.perm[k] * ld + r];
+    evecs[(long)m * n * n + i] = S.V[S.perm[k] * S.ldv + r];
   }
   if (tid == 0 && sweeps_out) sweeps_out[m] = sw;
 }
@@ -1500,7 +1774,7 @@ pol_eig_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *gain
                int max_sweeps, float tol, float *evals, float2 *evecs, int *sweeps_out) {
   extern __shared__ __align__(16) char smem_raw[];
   EigSmem S = carve(smem_raw, n, 1, false, kEigThreads / 64);
-  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
+  const int m = blockIdx.x, tid = threadIdx.x, lda = jac_ne(n);
   const int c = m >> 2, pq = m & 3, p = pq >> 1, q = pq & 1;
   const int nbl = n * (n + 1) / 2;
   for (int idx = tid; idx < nbl; idx += blockDim.x) {
@@ -1518,14 +1792,14 @@ pol_eig_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *gain
     const float2 pi = phase[(long)c * n + i], pj = phase[(long)c * n + j];
     const float2 f = cmul(make_float2(gi.x * gj.x + gi.y * gj.y, gi.y * gj.x - gi.x * gj.y),
                           make_float2(pi.x * pj.x + pi.y * pj.y, pi.y * pj.x - pi.x * pj.y));
-    S.A[j * ld + i] = cmul(f, make_float2((float)re, (float)im));
+    S.A[j * lda + i] = cmul(f, make_float2((float)re, (float)im));
   }
   __syncthreads();
   const int sw = jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.jl());
   sort_eigen(S.A, n, S.perm, S.ev);
   float2 *ph = S.W;  // per output column: conj(v_max) / |v_max|
   if (tid < n) {
-    const float2 *col = S.V + S.perm[tid] * ld;
+    const float2 *col = S.V + S.perm[tid] * S.ldv;
     int best = 0;
     float bm = -1.0f;
     for (int r = 0; r < n; ++r) {
@@ -1543,7 +1817,7 @@ pol_eig_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *gain
   if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
   for (int e = tid; e < n * n; e += blockDim.x) {
     const int k = e / n, r = e - k * n;
-    evecs[(long)m * n * n + e] = cmul(S.V[S.perm[k] * ld + r], ph[k]);
+    evecs[(long)m * n * n + e] = cmul(S.V[S.perm[k] * S.ldv + r], ph[k]);
   }
   if (tid == 0 && sweeps_out) sweeps_out[m] = sw;
 }
@@ -1559,7 +1833,7 @@ proj_block_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *g
                   int K, int max_sweeps, float tol, float2 *acc) {
   extern __shared__ __align__(16) char smem_raw[];
   EigSmem S = carve(smem_raw, n, 1, false, kEigThreads / 64);
-  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
+  const int m = blockIdx.x, tid = threadIdx.x, lda = jac_ne(n);
   const int c = m >> 2, pq = m & 3, p = pq >> 1, q = pq & 1;
   const int nbl = n * (n + 1) / 2;
   for (int idx = tid; idx < nbl; idx += blockDim.x) {
@@ -1577,7 +1851,7 @@ proj_block_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *g
     const float2 pi = phase[(long)c * n + i], pj = phase[(long)c * n + j];
     const float2 f = cmul(make_float2(gi.x * gj.x + gi.y * gj.y, gi.y * gj.x - gi.x * gj.y),
                           make_float2(pi.x * pj.x + pi.y * pj.y, pi.y * pj.x - pi.x * pj.y));
-    S.A[j * ld + i] = cmul(f, make_float2((float)re, (float)im));
+    S.A[j * lda + i] = cmul(f, make_float2((float)re, (float)im));
   }
   __syncthreads();
   jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.jl());
@@ -1588,7 +1862,7 @@ proj_block_kernel(const int *vis_part, int nsplit, int C, int n, const float2 *g
     if (i > j) continue;
     float2 sum = make_float2(0.f, 0.f);
     for (int k = n - K; k < n; ++k) {  // U(i, k) conj(U(j, k))
-      const float2 *col = S.V + S.perm[k] * ld;
+      const float2 *col = S.V + S.perm[k] * S.ldv;
       const float2 t = cmulc(col[i], col[j]);
       sum.x += t.x;
       sum.y += t.y;
@@ -1606,20 +1880,20 @@ __global__ void __launch_bounds__(kEigThreads)
 proj_dump_kernel(float2 *acc, int n, float inv_runs, int max_sweeps, float tol, float *evals, float2 *evecs) {
   extern __shared__ __align__(16) char smem_raw[];
   EigSmem S = carve(smem_raw, n, 1, false, kEigThreads / 64);
-  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
+  const int m = blockIdx.x, tid = threadIdx.x, lda = jac_ne(n);
   float2 *a = acc + (long)m * n * n;
   for (int e = tid; e < n * n; e += blockDim.x) {
     const int j = e / n, i = e - j * n;
     const float2 v = a[e];
     a[e] = make_float2(0.f, 0.f);
-    if (i <= j) S.A[j * ld + i] = make_float2(v.x * inv_runs, v.y * inv_runs);
+    if (i <= j) S.A[j * lda + i] = make_float2(v.x * inv_runs, v.y * inv_runs);
   }
   __syncthreads();
   jacobi_lds<kEigThreads>(S.A, S.V, n, max_sweeps, tol, S.jl());
   sort_eigen(S.A, n, S.perm, S.ev);
   float2 *ph = S.W;
   if (tid < n) {
-    const float2 *col = S.V + S.perm[tid] * ld;
+    const float2 *col = S.V + S.perm[tid] * S.ldv;
     int best = 0;
     float bm = -1.0f;
     for (int r = 0; r < n; ++r) {
@@ -1637,7 +1911,7 @@ proj_dump_kernel(float2 *acc, int n, float inv_runs, int max_sweeps, float tol, 
   if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
   for (int e = tid; e < n * n; e += blockDim.x) {
     const int k = e / n, r = e - k * n;
-    evecs[(long)m * n * n + e] = cmul(S.V[S.perm[k] * ld + r], ph[k]);
+    evecs[(long)m * n * n + e] = cmul(S.V[S.perm[k] * S.ldv + r], ph[k]);
   }
 }
 
@@ -1689,15 +1963,15 @@ eigh_warm_kernel(const float2 *Ain, const float2 *prop, int n, int KW, int K, in
                  float *evals, float2 *evecs, float *ratio) {
   extern __shared__ __align__(16) char smem_raw[];
   EigSmem S = carve(smem_raw, n, 0, true, kEigThreads / 64);
-  const int m = blockIdx.x, tid = threadIdx.x, ld = n | 1;
-  for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * ld + i % n] = Ain[(long)m * n * n + i];
+  const int m = blockIdx.x, tid = threadIdx.x, lda = jac_ne(n);
+  for (int i = tid; i < n * n; i += blockDim.x) S.A[(i / n) * lda + i % n] = Ain[(long)m * n * n + i];
   for (int e = tid; e < KW * n; e += blockDim.x) S.Y[e] = prop[(long)m * KW * n + e];
   __syncthreads();
-  warm_solve(S.A, ld, n, KW, K, steps, tol, S);
+  warm_solve(S.A, lda, n, KW, K, steps, tol, S);
   if (tid < n) evals[(long)m * n + tid] = S.ev[tid];
   for (int i = tid; i < n * n; i += blockDim.x) {
     const int k = i / n, r = i % n;
-    evecs[(long)m * n * n + i] = S.V[S.perm[k] * ld + r];
+    evecs[(long)m * n * n + i] = S.V[S.perm[k] * S.ldv + r];
   }
   if (tid == 0) ratio[m] = S.red[1];
 }
