@@ -58,32 +58,57 @@ def algorithmic_bytes(shape, nbeams_out):
     return dict(volt=volt, vis=vis, beams=beams, cov=cov)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(shape, ring, scales, w, channels, min_seconds=10.0, max_passes=4):
     """FP64 oracle on the same block, `channels` channels at a time over the
     whole band, repeated until about `min_seconds` of host work (all host
-    cores through numpy)."""
+    cores through the oracle's OpenMP loops), then one call of `channels`
+    channels on one thread (the single-core figure)."""
     import oracle as orc
 
     s = orc.Shape(C=channels, F=shape.F, NT=shape.NT, R=shape.R, P=2, RPP=shape.RPP,
                   NP=shape.NP, B=shape.B)
+    threads = orc.num_threads()
+
+    def one(c0):
+        y = orc.ingest(s, ring[c0:c0 + channels], scales[c0:c0 + channels], orc.WINDOW_ADAPTIVE)
+        orc.adaptive_block(s, y, w[c0:c0 + channels], fixed_k=np.ones((channels, 2)))
+
     done = 0
     t0 = time.perf_counter()
     passes = 0
     while passes < max_passes and (passes == 0 or time.perf_counter() - t0 < min_seconds):
         for c0 in range(0, shape.C - channels + 1, channels):
-            y = orc.ingest(s, ring[c0:c0 + channels], scales[c0:c0 + channels], orc.WINDOW_ADAPTIVE)
-            orc.adaptive_block(s, y, w[c0:c0 + channels], fixed_k=np.ones((channels, 2)))
+            one(c0)
             done += channels
         passes += 1
     dt = time.perf_counter() - t0
-    ant_samples = shape.R * done * shape.T
-    return dict(value=ant_samples / dt, unit="ant-samples/s", cores=orc.num_threads(),
+    orc.set_threads(1)
+    t1 = time.perf_counter()
+    one(0)
+    dt1 = time.perf_counter() - t1
+    orc.set_threads(threads)
+    per_channel = shape.R * shape.T
+    model = cpu_model()
+    return dict(value=per_channel * done / dt, unit="ant-samples/s", cores=threads,
                 kind="port",
                 sample=f"{passes} pass(es) over the {shape.C}-channel block of {shape.T} samples "
                        f"({done} channel-blocks, {channels} channels per call), FP64 oracle: "
                        f"ingest+correlate+whiten+Jacobi+null+W.P+{2 * shape.B} beams; "
-                       f"{dt:.2f} s wall on {platform.processor() or platform.machine()}",
-                seconds=dt)
+                       f"{dt:.2f} s wall on {threads} threads of {model}",
+                seconds=dt, cpu_model=model,
+                single_core={"value": per_channel * channels / dt1, "unit": "ant-samples/s",
+                             "cores": 1, "seconds": dt1,
+                             "sample": f"one call of {channels} channels of the same block on one thread"})
 
 
 def run_spectra(p, shape, d_ring, d_scales, args):
@@ -619,8 +644,11 @@ def main():
     kt = p.kernel_times()
     p.set_profiling(False)
     orth_max = None
+    sweeps_mean = None
     if mode == lam.MODE_ADAPTIVE:
-        orth_max = float(p.eig_stats(with_orthogonality=True)[2].max())
+        sw_last, _, orth = p.eig_stats(with_orthogonality=True)
+        orth_max = float(orth.max())
+        sweeps_mean = float(sw_last.mean())
     host_fed = None
     if not args.no_host_fed:
         host_fed = run_host_fed(p, shape, ring, scales, args)
@@ -654,6 +682,20 @@ def main():
                     "kernel_ms": round(ms, 4), "algorithmic_bytes": per_kernel_bytes[dominant]}
     for k, v in kstats.items():
         v["GBps_algorithmic"] = round(per_kernel_bytes[k] / (v["ms_avg"] * 1e-3) / 1e9, 1)
+    # the eigen stage is bound by neither HBM nor MFMA: report it as matrices/s and
+    # as a fraction of the FP32 vector peak, counting the algorithmic Jacobi work
+    # N(N-1)/2 rotations x 6N complex updates x 8 flops per sweep and matrix
+    eig_roof = None
+    if "eig_null" in kstats and sweeps_mean is not None:
+        ms = kstats["eig_null"]["ms_avg"]
+        nmat = shape.C * 2
+        n = shape.R
+        flops = nmat * sweeps_mean * (n * (n - 1) / 2) * 6 * n * 8
+        tf = flops / (ms * 1e-3) / 1e12
+        eig_roof = {"bound": "fp32 valu / latency", "matrices_per_s": round(nmat / (ms * 1e-3), 1),
+                    "matrices": nmat, "n": n, "sweeps_mean": round(sweeps_mean, 2), "kernel_ms": round(ms, 4),
+                    "achieved_tflops": round(tf, 2), "peak_tflops": 157.3, "frac": round(tf / 157.3, 4),
+                    "peak_tflops_unpacked": 78.6, "frac_unpacked": round(tf / 78.6, 4)}
     # HBM bytes per launch measured with rocprofv3 PMC (FETCH_SIZE, WRITE_SIZE scaled by a copy
     # probe), per time-split count of the voltage workgroups
     tfile = os.path.join(REPO, "profiles", "adaptive_384ch_hbm_traffic.json")
@@ -718,6 +760,7 @@ def main():
         "real_time_factor_ref_convention": (shape.T / REF_RATE_HZ) / sec,
         "null_basis_orthogonality_fp64_max": orth_max,
         "roofline": roofline,
+        "eig_roofline": eig_roof,
         "kernels": kstats,
         "autotune": tuned,
         "learned_vs_plain": learned,

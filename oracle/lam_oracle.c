@@ -26,6 +26,15 @@ int orc_num_threads(void) {
 #endif
 }
 
+/* Threads of the following oracle calls (the CPU baseline's single-core figure). */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------ ingest */
 /* packet_formats.hpp:309-312 gives the ring layout; spatial.cuh:430-486 applies
  * the int16 scale of (channel, ring slot, receiver, pol) to the int8 sample;

@@ -152,6 +152,8 @@ void orc_bandpass(int rows, int n, int d, const double *x, double *out);
 
 /* Number of OpenMP threads the oracle would use (1 when built without it). */
 int orc_num_threads(void);
+/* Threads for the following calls (OpenMP; ignored without it). */
+void orc_set_threads(int n);
 
 #ifdef __cplusplus
 }

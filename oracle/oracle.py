@@ -97,6 +97,7 @@ def _declare(L):
                                      ctypes.c_double, ctypes.c_int, _p, _p, _p, _p, _p]
     L.orc_adaptive_block.restype = ctypes.c_int
     L.orc_num_threads.restype = ctypes.c_int
+    L.orc_set_threads.argtypes = [ctypes.c_int]
     L.orc_beam_spectra.argtypes = [ctypes.c_int] * 6 + [_p, _p]
     L.orc_bandpass.argtypes = [ctypes.c_int] * 3 + [_p, _p]
 
@@ -295,3 +296,7 @@ def bandpass(beams, d):
 
 def num_threads() -> int:
     return lib().orc_num_threads()
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(n)
