@@ -44,6 +44,7 @@ struct Buf {
   int *d_counts = nullptr, *d_sweeps = nullptr, *d_fallback = nullptr, *d_nsig = nullptr;
   float *d_orth = nullptr;
   hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};  // one per block phase
+  int graph_full = -1;  // the eigen outputs flag captured in graph[1]
   const void *graph_samples = nullptr, *graph_scales = nullptr;
   hipEvent_t done = nullptr;
   hipEvent_t eig_done = nullptr;  // the eigen stage of this buffer's latest block
@@ -119,6 +120,7 @@ struct lam_ctx {
   std::vector<ProfEvent> prof;
   // eigen-stage method
   int eig_method = LAM_EIG_PLAIN, refine_sweeps = 2, power = 0;
+  int eig_full = 1;  // the current block's eigen outputs are requested (EigParams.full_outputs)
   float res_tol = 0.1f;
   int eig_plain_threads = 256;
   int wprec = kWprecF16HiLo;
@@ -392,6 +394,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
     e.delta = ctx->delta;
     e.shrink = ctx->shrink;
     e.whiten = ctx->whiten;
+    e.full_outputs = ctx->eig_full;
     e.max_sweeps = ctx->max_sweeps;
     e.tol = ctx->tol;
     e.NMT = ctx->NMT2;
@@ -529,6 +532,13 @@ int run_block(lam_ctx *ctx, Buf &b, const void *samples, const void *scales) {
       b.graph_samples = samples;
       b.graph_scales = scales;
     }
+    // the eigen stage's graph holds whether eigen outputs are formed
+    if (b.graph[1] && b.graph_full != ctx->eig_full) {
+      HIPC(ctx, hipStreamSynchronize(b.stream));
+      (void)hipGraphExecDestroy(b.graph[1]);
+      b.graph[1] = nullptr;
+    }
+    b.graph_full = ctx->eig_full;
   }
   const bool chain = eig_chained(ctx) && ctx->last_buf >= 0 && ctx->last_buf != ctx->cur;
   for (int ph = 0; ph < kPhases; ++ph) {
@@ -1255,6 +1265,8 @@ static int execute_block(lam_ctx *ctx, const lam_block_io *io, bool &release_que
     smp = b.d_samples;
     scl = b.d_scales;
   }
+  // every eigenpair is formed only for a block whose eigen outputs are requested
+  ctx->eig_full = (io->eigenvalues_dst || io->eigenvectors_dst) ? 1 : 0;
   // never accumulate into a visibility buffer that a dump has not reset yet
   HIPC(ctx, hipStreamWaitEvent(s, ctx->reset_done, 0));
   HIPC(ctx, hipMemsetAsync(b.d_err, 0, sizeof(int), s));

@@ -747,13 +747,23 @@ __device__ __forceinline__ void apply_cols(const float2 *A, int ld, const float2
                                            int KW) {
   for (int e = threadIdx.x; e < KW * n; e += blockDim.x) {
     const int k = e / n, i = e - k * n;
-    float2 z = make_float2(0.f, 0.f);
-    for (int j = 0; j < n; ++j) {
-      const float2 t = cmul(A[j * ld + i], Y[k * n + j]);
-      z.x += t.x;
-      z.y += t.y;
+    // four independent partial sums: a short dependent chain per thread
+    float2 z[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float2 t = cmul(A[(j + u) * ld + i], Y[k * n + j + u]);
+        z[u].x += t.x;
+        z[u].y += t.y;
+      }
     }
-    Z[e] = z;
+    for (; j < n; ++j) {
+      const float2 t = cmul(A[j * ld + i], Y[k * n + j]);
+      z[0].x += t.x;
+      z[0].y += t.y;
+    }
+    Z[e] = make_float2((z[0].x + z[1].x) + (z[2].x + z[3].x), (z[0].y + z[1].y) + (z[2].y + z[3].y));
   }
 }
 
@@ -826,7 +836,8 @@ __device__ __forceinline__ void small_jacobi_wave(float2 *B, float2 *Vb, int KW,
           p = q;
           q = t;
         }
-        S.rot[l] = make_rotation(B[p * KW + q], B[p * KW + p].x, B[q * KW + q].x);
+        double tbd;
+        S.rot[l] = make_rotation_t(B[p * KW + q], B[p * KW + p].x, B[q * KW + q].x, tbd);
         S.rpq[l] = make_int2(p, q);
         S.mark[p] = 2 * l;
         S.mark[q] = 2 * l + 1;
@@ -867,14 +878,19 @@ __device__ __forceinline__ void small_jacobi_wave(float2 *B, float2 *Vb, int KW,
 //   (small Jacobi); Ritz vectors U = Y Vb; an orthonormal complement of U from
 //   its Householder QR with Rayleigh quotients as the remaining eigenvalues.
 // Fills S.V (columns: complement, then Ritz vectors by ascending Ritz value),
-// S.ev / S.perm (ascending) and returns the accepted flag of the leakage test
+// S.ev / S.perm (ascending) and returns the accepted flag of the leakage test.
+// With full == false (no eigen outputs requested) no complement is formed:
+// its columns of S.V are left unset, its eigenvalues all read as their mean
+// (trace(A) minus the Ritz values, over n - KW), and the complement's largest
+// eigenvalue for the check below is estimated by three power steps on
+// (I - U U^H) A (I - U U^H).
 //   sum_{top K} |A u_k - theta_k u_k|^2 / theta_k * (theta_k - mu) / theta_k <= tol * mu,
 // mu = mean of the remaining eigenvalues: the interference a mode leaks through
 // the null is its angle error squared times its power above the noise floor.
 // A complement direction ranked among the K largest fails the test when its
 // Rayleigh quotient exceeds 1.5 mu (an interferer the warm subspace missed).
 __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int KW, int K, int steps,
-                                           float tol, EigSmem &S) {
+                                           float tol, EigSmem &S, bool full = true) {
   const int tid = threadIdx.x, nthr = blockDim.x;
   float2 *Y = S.Y, *Z = S.hv;
   for (int st = 0; st < steps; ++st) {
@@ -895,13 +911,18 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
   float2 *Bm = S.t1, *Vb = S.t2;
   for (int e = tid; e < KW * KW; e += nthr) {
     const int ra = e / KW, cb = e - ra * KW;
-    float2 g = make_float2(0.f, 0.f);
-    for (int i = 0; i < n; ++i) {
-      const float2 t = cmulc(Z[cb * n + i], Y[ra * n + i]);  // conj(y_a) z_b
-      g.x += t.x;
-      g.y += t.y;
+    float2 g[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    for (int i0 = 0; i0 < n; i0 += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = min(i0 + u, n - 1);
+        const float2 t = cmulc(Z[cb * n + i], Y[ra * n + i]);  // conj(y_a) z_b
+        const float on = i0 + u < n ? 1.0f : 0.0f;
+        g[u].x += on * t.x;
+        g[u].y += on * t.y;
+      }
     }
-    Bm[ra * KW + cb] = g;
+    Bm[ra * KW + cb] = make_float2((g[0].x + g[1].x) + (g[2].x + g[3].x), (g[0].y + g[1].y) + (g[2].y + g[3].y));
   }
   __syncthreads();
   if (tid < 64) small_jacobi_wave(Bm, Vb, KW, 12, 1e-7f, S);
@@ -939,121 +960,199 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
     S.res[k * 64 + i] = z.x * z.x + z.y * z.y;
   }
   __syncthreads();
-  // complement: Householder QR of U (wave 0) and the compact form
-  // H_0 ... H_{KW-1} = I - Vh T Vh^H (T upper triangular, built in the same pass);
-  // V[:, 0..m) = (I - Vh T Vh^H)[:, KW..n)
-  float2 *Tm = S.t2;  // KW x KW, row-major (Vb is no longer needed)
-  if (tid < 64) {
-    const int i = tid;
-    for (int k = 0; k < KW; ++k)
-      if (i < n) Y[k * n + i] = S.V[(m + k) * S.ldv + i];
-    for (int e = i; e < KW * KW; e += 64) Tm[e] = make_float2(0.f, 0.f);
-    wave_sync();
-    float2 vk[kMaxWarm];
+  float *evu = S.f;  // unsorted eigenvalues by column
+  float cmax = 0.0f;  // complement's largest eigenvalue (estimate, full == false)
+  if (full) {
+    // complement: Householder QR of U (wave 0) and the compact form
+    // H_0 ... H_{KW-1} = I - Vh T Vh^H (T upper triangular, built in the same pass);
+    // V[:, 0..m) = (I - Vh T Vh^H)[:, KW..n)
+    float2 *Tm = S.t2;  // KW x KW, row-major (Vb is no longer needed)
+    if (tid < 64) {
+      const int i = tid;
+      for (int k = 0; k < KW; ++k)
+        if (i < n) Y[k * n + i] = S.V[(m + k) * S.ldv + i];
+      for (int e = i; e < KW * KW; e += 64) Tm[e] = make_float2(0.f, 0.f);
+      wave_sync();
+      float2 vk[kMaxWarm];
 #pragma unroll
-    for (int k = 0; k < kMaxWarm; ++k) {
-      if (k >= KW) break;
-      const float2 x = (i < n && i >= k) ? Y[k * n + i] : make_float2(0.f, 0.f);
-      const float nrm2 = wave_sum(x.x * x.x + x.y * x.y);
-      const float xkr = __shfl(x.x, k, 64), xki = __shfl(x.y, k, 64);
-      const float axk = sqrtf(xkr * xkr + xki * xki), nrm = sqrtf(nrm2);
-      const float2 alpha = axk > 0.0f ? make_float2(-xkr / axk * nrm, -xki / axk * nrm)
-                                      : make_float2(-nrm, 0.0f);
-      float2 v = x;
-      if (i == k) {
-        v.x -= alpha.x;
-        v.y -= alpha.y;
-      }
-      const float vn2 = wave_sum(v.x * v.x + v.y * v.y);
-      const float tau = (nrm2 > 1e-30f && vn2 > 1e-30f) ? 2.0f / vn2 : 0.0f;
-      vk[k] = v;
-      if (i < n) Z[k * n + i] = v;  // reflector k
-      // T(0:k, k) = -tau T(0:k, 0:k) (Vh(:, 0:k)^H v_k), T(k, k) = tau
-      float zr[kMaxWarm], zi[kMaxWarm];
-#pragma unroll
-      for (int j = 0; j < k; ++j) {
-        zr[j] = wave_sum(vk[j].x * v.x + vk[j].y * v.y);
-        zi[j] = wave_sum(vk[j].x * v.y - vk[j].y * v.x);
-      }
-      if (i < k) {
-        float2 acc = make_float2(0.f, 0.f);
+      for (int k = 0; k < kMaxWarm; ++k) {
+        if (k >= KW) break;
+        const float2 x = (i < n && i >= k) ? Y[k * n + i] : make_float2(0.f, 0.f);
+        const float nrm2 = wave_sum(x.x * x.x + x.y * x.y);
+        const float xkr = __shfl(x.x, k, 64), xki = __shfl(x.y, k, 64);
+        const float axk = sqrtf(xkr * xkr + xki * xki), nrm = sqrtf(nrm2);
+        const float2 alpha = axk > 0.0f ? make_float2(-xkr / axk * nrm, -xki / axk * nrm)
+                                        : make_float2(-nrm, 0.0f);
+        float2 v = x;
+        if (i == k) {
+          v.x -= alpha.x;
+          v.y -= alpha.y;
+        }
+        const float vn2 = wave_sum(v.x * v.x + v.y * v.y);
+        const float tau = (nrm2 > 1e-30f && vn2 > 1e-30f) ? 2.0f / vn2 : 0.0f;
+        vk[k] = v;
+        if (i < n) Z[k * n + i] = v;  // reflector k
+        // T(0:k, k) = -tau T(0:k, 0:k) (Vh(:, 0:k)^H v_k), T(k, k) = tau
+        float zr[kMaxWarm], zi[kMaxWarm];
 #pragma unroll
         for (int j = 0; j < k; ++j) {
-          const float2 t = cmul(Tm[i * KW + j], make_float2(zr[j], zi[j]));
-          acc.x += t.x;
-          acc.y += t.y;
+          zr[j] = wave_sum(vk[j].x * v.x + vk[j].y * v.y);
+          zi[j] = wave_sum(vk[j].x * v.y - vk[j].y * v.x);
         }
-        Tm[i * KW + k] = make_float2(-tau * acc.x, -tau * acc.y);
-      }
-      if (i == k) Tm[k * KW + k] = make_float2(tau, 0.f);
-      wave_sync();
-      // apply H_k to the remaining columns of U
+        if (i < k) {
+          float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int j = k + 1; j < kMaxWarm; ++j) {
-        if (j >= KW) break;
-        float2 y = i < n ? Y[j * n + i] : make_float2(0.f, 0.f);
-        const float gr = wave_sum(v.x * y.x + v.y * y.y), gi = wave_sum(v.x * y.y - v.y * y.x);
-        y.x -= tau * (gr * v.x - gi * v.y);
-        y.y -= tau * (gr * v.y + gi * v.x);
-        if (i < n) Y[j * n + i] = y;
+          for (int j = 0; j < k; ++j) {
+            const float2 t = cmul(Tm[i * KW + j], make_float2(zr[j], zi[j]));
+            acc.x += t.x;
+            acc.y += t.y;
+          }
+          Tm[i * KW + k] = make_float2(-tau * acc.x, -tau * acc.y);
+        }
+        if (i == k) Tm[k * KW + k] = make_float2(tau, 0.f);
+        wave_sync();
+        // apply H_k to the remaining columns of U
+#pragma unroll
+        for (int j = k + 1; j < kMaxWarm; ++j) {
+          if (j >= KW) break;
+          float2 y = i < n ? Y[j * n + i] : make_float2(0.f, 0.f);
+          const float gr = wave_sum(v.x * y.x + v.y * y.y), gi = wave_sum(v.x * y.y - v.y * y.x);
+          y.x -= tau * (gr * v.x - gi * v.y);
+          y.y -= tau * (gr * v.y + gi * v.x);
+          if (i < n) Y[j * n + i] = y;
+        }
+        wave_sync();
       }
-      wave_sync();
     }
-  }
-  __syncthreads();
-  // Sm(a, c) = sum_b T(a, b) conj(Vh(KW + c, b)), then Q2(i, c) = delta - sum_a Vh(i, a) Sm(a, c)
-  float2 *Sm = Y;  // KW x m, column c at Sm + c * KW
-  for (int e = tid; e < KW * m; e += nthr) {
-    const int c2 = e / KW, a2 = e - c2 * KW;
-    float2 acc = make_float2(0.f, 0.f);
-    for (int b2 = a2; b2 < KW; ++b2) {
-      const float2 t = cmulc(Tm[a2 * KW + b2], Z[b2 * n + KW + c2]);
-      acc.x += t.x;
-      acc.y += t.y;
+    __syncthreads();
+    // Sm(a, c) = sum_b T(a, b) conj(Vh(KW + c, b)), then Q2(i, c) = delta - sum_a Vh(i, a) Sm(a, c)
+    float2 *Sm = Y;  // KW x m, column c at Sm + c * KW
+    for (int e = tid; e < KW * m; e += nthr) {
+      const int c2 = e / KW, a2 = e - c2 * KW;
+      float2 acc = make_float2(0.f, 0.f);
+      for (int b2 = a2; b2 < KW; ++b2) {
+        const float2 t = cmulc(Tm[a2 * KW + b2], Z[b2 * n + KW + c2]);
+        acc.x += t.x;
+        acc.y += t.y;
+      }
+      Sm[c2 * KW + a2] = acc;
     }
-    Sm[c2 * KW + a2] = acc;
-  }
-  __syncthreads();
-  for (int e = tid; e < m * n; e += nthr) {
-    const int c2 = e / n, i = e - c2 * n;
-    float2 q = make_float2(i == KW + c2 ? 1.0f : 0.0f, 0.0f);
-    for (int a2 = 0; a2 < KW; ++a2) {
-      const float2 t = cmul(Z[a2 * n + i], Sm[c2 * KW + a2]);
-      q.x -= t.x;
-      q.y -= t.y;
+    __syncthreads();
+    for (int e = tid; e < m * n; e += nthr) {
+      const int c2 = e / n, i = e - c2 * n;
+      float2 q = make_float2(i == KW + c2 ? 1.0f : 0.0f, 0.0f);
+      for (int a2 = 0; a2 < KW; ++a2) {
+        const float2 t = cmul(Z[a2 * n + i], Sm[c2 * KW + a2]);
+        q.x -= t.x;
+        q.y -= t.y;
+      }
+      S.A0[e] = q;
     }
-    S.A0[e] = q;
-  }
-  __syncthreads();
-  for (int e = tid; e < m * n; e += nthr) S.V[(e / n) * S.ldv + e % n] = S.A0[e];
-  __syncthreads();
-  // Rayleigh quotients of the complement: lambda_j = q_j^H A q_j
-  for (int e = tid; e < m * n; e += nthr) {  // W = A Q2 into A0
-    const int j = e / n, i = e - j * n;
-    float2 w = make_float2(0.f, 0.f);
-    for (int c2 = 0; c2 < n; ++c2) {
-      const float2 t = cmul(A[c2 * ld + i], S.V[j * S.ldv + c2]);
-      w.x += t.x;
-      w.y += t.y;
+    __syncthreads();
+    for (int e = tid; e < m * n; e += nthr) S.V[(e / n) * S.ldv + e % n] = S.A0[e];
+    __syncthreads();
+    // Rayleigh quotients of the complement: lambda_j = q_j^H A q_j
+    for (int e = tid; e < m * n; e += nthr) {  // W = A Q2 into A0
+      const int j = e / n, i = e - j * n;
+      float2 w[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+      for (int c0 = 0; c0 < n; c0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int c2 = min(c0 + u, n - 1);
+          const float2 t = cmul(A[c2 * ld + i], S.V[j * S.ldv + c2]);
+          const float on = c0 + u < n ? 1.0f : 0.0f;
+          w[u].x += on * t.x;
+          w[u].y += on * t.y;
+        }
+      }
+      S.A0[e] = make_float2((w[0].x + w[1].x) + (w[2].x + w[3].x), (w[0].y + w[1].y) + (w[2].y + w[3].y));
     }
-    S.A0[e] = w;
-  }
-  __syncthreads();
-  float *evu = S.f;  // unsorted eigenvalues by column
-  for (int j = tid; j < n; j += nthr) {
-    if (j < m) {
+    __syncthreads();
+    for (int j = tid; j < n; j += nthr) {
+      if (j < m) {
+        float lam[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int i0 = 0; i0 < n; i0 += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int i = min(i0 + u, n - 1);
+            const float2 q = S.V[j * S.ldv + i], w = S.A0[j * n + i];
+            lam[u] += i0 + u < n ? q.x * w.x + q.y * w.y : 0.0f;
+          }
+        }
+        evu[j] = (lam[0] + lam[1]) + (lam[2] + lam[3]);
+      } else {
+        const int bcol = rk[j - m];
+        evu[j] = Bm[bcol * KW + bcol].x;
+      }
+    }
+    __syncthreads();
+  } else {
+    // complement eigenvalues as their mean, and its largest one by power steps
+    // (wave 0, lane = row): v <- (I - U U^H) A v from pseudo-random phases
+    if (tid < 64) {
+      const int i = tid;
+      float tr = 0.0f, sth = 0.0f;
+      for (int c2 = 0; c2 < n; ++c2) tr += A[c2 * ld + c2].x;
+      for (int k = 0; k < KW; ++k) sth += Bm[k * KW + k].x;
+      const float mu = m > 0 ? (tr - sth) / (float)m : 0.0f;
+      float sn, cs;
+      __sincosf(6.2831853f * (float)((i * 40503u) & 0xffffu) * (1.0f / 65536.0f), &sn, &cs);
+      float2 v = i < n ? make_float2(cs, sn) : make_float2(0.f, 0.f);
+      float2 *vt = Y;  // v broadcast through LDS (Y is free without a complement)
       float lam = 0.0f;
-      for (int i = 0; i < n; ++i) {
-        const float2 q = S.V[j * S.ldv + i], w = S.A0[j * n + i];
-        lam += q.x * w.x + q.y * w.y;
+      for (int it = 0; it < 4; ++it) {
+        // project out the KW Ritz vectors (columns m .. n-1 of V)
+        float cr[kMaxWarm], ci[kMaxWarm];
+#pragma unroll
+        for (int k = 0; k < kMaxWarm; ++k) {
+          const float2 u = (k < KW && i < n) ? S.V[(m + k) * S.ldv + i] : make_float2(0.f, 0.f);
+          cr[k] = wave_sum(u.x * v.x + u.y * v.y);
+          ci[k] = wave_sum(u.x * v.y - u.y * v.x);
+        }
+#pragma unroll
+        for (int k = 0; k < kMaxWarm; ++k) {
+          const float2 u = (k < KW && i < n) ? S.V[(m + k) * S.ldv + i] : make_float2(0.f, 0.f);
+          v.x -= cr[k] * u.x - ci[k] * u.y;
+          v.y -= cr[k] * u.y + ci[k] * u.x;
+        }
+        if (it == 3) break;
+        const float nv = wave_sum(v.x * v.x + v.y * v.y);
+        const float inv = nv > 1e-30f ? rsqrtf(nv) : 0.0f;
+        v.x *= inv;
+        v.y *= inv;
+        if (i < n) vt[i] = v;
+        wave_sync();
+        // w = A v, four partial sums
+        float2 w4[4] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+        for (int j0 = 0; j0 < n; j0 += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int j = min(j0 + u, n - 1);
+            const float2 t = cmul(A[j * ld + min(i, n - 1)], vt[j]);
+            const float on = j0 + u < n ? 1.0f : 0.0f;
+            w4[u].x += on * t.x;
+            w4[u].y += on * t.y;
+          }
+        }
+        wave_sync();
+        float2 w = i < n ? make_float2((w4[0].x + w4[1].x) + (w4[2].x + w4[3].x), (w4[0].y + w4[1].y) + (w4[2].y + w4[3].y))
+                         : make_float2(0.f, 0.f);
+        lam = wave_sum(v.x * w.x + v.y * w.y);  // Rayleigh quotient of the unit v
+        v = w;
       }
-      evu[j] = lam;
-    } else {
-      const int bcol = rk[j - m];
-      evu[j] = Bm[bcol * KW + bcol].x;
+      if (tid == 0) S.red[2] = lam;
+      for (int j = tid; j < n; j += 64) {
+        if (j < m) {
+          evu[j] = mu;
+        } else {
+          const int bcol = rk[j - m];
+          evu[j] = Bm[bcol * KW + bcol].x;
+        }
+      }
     }
+    __syncthreads();
+    cmax = S.red[2];
   }
-  __syncthreads();
   if (tid < n) {
     const float lj = evu[tid];
     int rank = 0;
@@ -1068,6 +1167,8 @@ __device__ __forceinline__ bool warm_solve(const float2 *A, int ld, int n, int K
     bool ok = K <= KW;
     for (int k = 0; k < n - K; ++k) rest += S.ev[k];
     rest = (n > K) ? rest / (float)(n - K) : 0.0f;
+    // without the complement basis: its largest eigenvalue must stay near the floor
+    if (!full && cmax > 1.5f * rest) ok = false;
     for (int k = 0; k < K && ok; ++k) {
       const int col = S.perm[n - 1 - k];
       const float th = S.ev[n - 1 - k];
@@ -1288,7 +1389,7 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
     for (int e = tid; e < KW * n; e += nthr) S.Y[e] = P.proposal[cp * P.prop_stride + e];
     __syncthreads();
     K = max(0, min(n, fixed));
-    const bool ok = warm_solve(S.A, lda, n, KW, K, P.refine_sweeps, P.res_tol, S);
+    const bool ok = warm_solve(S.A, lda, n, KW, K, P.refine_sweeps, P.res_tol, S, P.full_outputs || P.shrink);
     sweeps = P.refine_sweeps;
     if (ok) {
       if (tid == 0) S.kdet[0] = K;
@@ -1396,10 +1497,12 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
     S.W[idx] = __half22float2(w);
   }
   __syncthreads();
-  // 6. eigen outputs
-  if (tid < n) P.evals[cp * n + tid] = S.ev[tid];
-  for (int i = tid; i < n * n; i += nthr)
-    reinterpret_cast<float2 *>(P.evecs)[cp * n * n + i] = S.A[i];
+  // 6. eigen outputs (when requested)
+  if (P.full_outputs) {
+    if (tid < n) P.evals[cp * n + tid] = S.ev[tid];
+    for (int i = tid; i < n * n; i += nthr)
+      reinterpret_cast<float2 *>(P.evecs)[cp * n * n + i] = S.A[i];
+  }
   if (tid == 0) {
     P.counts[cp] = K;
     if (P.sweeps) P.sweeps[cp] = sweeps;

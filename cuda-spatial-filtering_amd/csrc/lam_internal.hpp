@@ -123,6 +123,8 @@ struct EigParams {
   int plain_threads;       // workgroup width of the plain solver (64, 128, 256)
   int wprec;               // precision of the [W; W'] beam operands (kWprec*)
   int whiten;              // 1: whiten the covariance first (adaptive); 0: raw (pulsar-fold RFI branch)
+  int full_outputs;        // 1: every eigenpair is written (eigen outputs requested); 0: only what the
+                           // null needs (the warm start then forms no complement basis)
 };
 
 // Eigen stage of subarrays (uneven sizes), one persistent launch.
