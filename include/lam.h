@@ -176,7 +176,11 @@ typedef struct lam_block_io {
   void *beams_dst;         /* host or device landing pointer for beams (NULL: skip) */
   lam_host_fn beams_done_fn;
   void *beams_done_user;
-  float *eigenvalues_dst;  /* adaptive or LAMBDA_GPU mode (NULL: skip) */
+  float *eigenvalues_dst;  /* adaptive or LAMBDA_GPU mode (NULL: skip).  In the adaptive
+                            * mode a block with neither eigen landing pointer forms only
+                            * what its null needs: no eigen outputs are written and the
+                            * warm starts form no complement basis (the 2B beams, counts
+                            * and statistics are unchanged). */
   float *eigenvectors_dst;
   int32_t *counts_dst;
   lam_host_fn eigen_done_fn;
