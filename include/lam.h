@@ -70,7 +70,8 @@ enum {
    * B beams and accumulated visibilities as in corr-beam mode, on the delay-aligned
    * window of the adaptive pipeline (slot 1 + per-FPGA delay, stream permutation),
    * plus the eigendecomposition of all four pol-pair covariances of every block
-   * (layout "pol-pair eigen" below).  Its bandpass FFT is not computed. */
+   * (layout "pol-pair eigen" below); with lam_set_bandpass, also its bandpass (the
+   * block-length FFT of every beam row, mean |X|^2 over runs of bins). */
   LAM_MODE_LAMBDA_GPU = 2,
   /* LambdaProjectionPipeline (lambda_projection_pipeline.hpp:307-645): per block the
    * four pol-pair covariances (ring slots 0..NP-1, gains applied) are decomposed and
