@@ -617,6 +617,29 @@ struct EigSmem {
   __device__ JacobiLds jl() const { return JacobiLds{W, rot2, reinterpret_cast<float *>(rpq2), red}; }
 };
 
+// ---- plain solve without eigen outputs: Householder reduction to a real
+// tridiagonal, all eigenvalues by multisection on Sturm counts, the removed
+// modes' eigenvectors by inverse iteration and the reflectors ----
+// The eigenvectors of a problem are wanted only for its removed modes (and
+// the next block's warm start), so the O(n^3) work of the Jacobi solver's
+// accumulated rotations is replaced by one reduction (about 16/3 n^3 real
+// flops, one barrier per column) and O(n^2) per vector.
+constexpr int kTriK = 8;  // eigenvectors per problem (more: the Jacobi solver)
+
+// LDS of the tridiagonal path inside the W region (nothing else there is
+// live until the weights are loaded): the reduction's partial sums, then the
+// elimination rows and the tridiagonal's eigenvectors; the tables after them.
+struct TriLds {
+  float2 *part;          // [2][waves][64] partial products A22 u, then [2][64] next columns
+  float *fac;            // [n][4][kTriK] elimination rows: 1/pivot, u1, u2, rhs
+  float *z;              // [n][kTriK] eigenvectors of the real tridiagonal
+  float *ta, *tb, *hc;   // [64] diagonal, off-diagonal (>= 0), reflector scales
+  float2 *ph;            // [64] off-diagonal phases, then the diagonal similarity
+  int *flag;             // [1] vectors finite
+};
+__host__ __device__ inline int tri_w_base(int n) { return 20 * n > 640 ? 20 * n : 640; }
+__host__ __device__ inline int tri_w_elems(int n) { return tri_w_base(n) + 256; }
+
 // LDS layout.  The warm-start buffers exist only in the warm kernel.  W
 // (loaded after the eigen solve) also holds the solver's second matrix buffer
 // and the warm start's copy of A (used before any plain solve).  In the plain
@@ -626,7 +649,8 @@ struct EigSmem {
 __host__ __device__ inline int eig_w_elems(int n, int B, bool warm, int nthr) {
   const int m = jac_ne(n) * jac_ne(n), mv = jac_ne(n) * jac_ldv(n, nthr);
   const bool g_in_v = !warm && B * n <= mv;  // G (B x n) fits in V
-  const int wb = (g_in_v ? 2 : 3) * B * n;
+  int wb = (g_in_v ? 2 : 3) * B * n;
+  if (!warm && tri_w_elems(n) > wb) wb = tri_w_elems(n);  // the tridiagonal path's scratch
   return wb > m ? wb : m;
 }
 
@@ -1304,6 +1328,388 @@ __device__ __forceinline__ void refine_removed_eigenvalues(EigSmem &S, int n, in
   }
 }
 
+__device__ __forceinline__ TriLds tri_lds(float2 *W, int n) {
+  TriLds t;
+  t.part = W;
+  t.fac = reinterpret_cast<float *>(W);
+  t.z = t.fac + 4 * kTriK * n;
+  float *tab = reinterpret_cast<float *>(W + tri_w_base(n));
+  t.ta = tab;
+  t.tb = tab + 64;
+  t.hc = tab + 128;
+  t.ph = reinterpret_cast<float2 *>(tab + 192);
+  t.flag = reinterpret_cast<int *>(tab + 320);
+  return t;
+}
+
+__device__ __forceinline__ float rlf(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ float2 rl2(float2 v, int l) { return make_float2(rlf(v.x, l), rlf(v.y, l)); }
+// Inclusive prefix maximum over the lanes (any sign): the DPP pattern of
+// wave_sum with -3e38 where a move has no source.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_move_neg(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, -3e38f),
+                                                               __builtin_bit_cast(int, v), CTRL, ROW_MASK, 0xf,
+                                                               false));
+}
+__device__ __forceinline__ float wave_scan_max(float v) {
+  v = fmaxf(v, dpp_move_neg<0x111, 0xf>(v));  // row_shr:1
+  v = fmaxf(v, dpp_move_neg<0x112, 0xf>(v));  // row_shr:2
+  v = fmaxf(v, dpp_move_neg<0x114, 0xf>(v));  // row_shr:4
+  v = fmaxf(v, dpp_move_neg<0x118, 0xf>(v));  // row_shr:8
+  v = fmaxf(v, dpp_move_neg<0x142, 0xa>(v));  // row_bcast:15 into rows 1 and 3
+  v = fmaxf(v, dpp_move_neg<0x143, 0xc>(v));  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+// Maximum over the 64 lanes (any sign).
+__device__ __forceinline__ float wave_max_any(float v) { return rlf(wave_scan_max(v), 63); }
+
+// Columns of the matrix a wave holds in registers during the reduction
+// (column k in wave k mod waves); the tridiagonal path takes n up to
+// waves x tri_cols.
+template <int NW>
+constexpr int tri_cols() { return NW >= 4 ? 16 : 32; }
+
+// Reduction of the Hermitian A (3 <= n <= waves x tri_cols, full storage,
+// leading dimension jac_ne(n)) to T = Q^H A Q tridiagonal with
+// Q = H_0 ... H_{n-3}, H_j = I - c_j u_j u_j^H acting on rows j+1..
+// (Hermitian reflectors): H_j maps column j below the diagonal, x, to
+// -e^{i arg x_{j+1}} |x| e_{j+1}.  Step j updates the trailing matrix as
+// A22 <- A22 - u w^H - w u^H with y = c A22 u, w = y - (c/2)(u^H y) u.
+// The matrix lives in registers (lane i: row i of the wave's columns; the
+// lowest column leaves when it becomes the next x and the slots shift
+// down).  Each step: every wave applies the previous step's update to its
+// columns and forms its part of A22 u in the same pass, u and w read as LDS
+// broadcasts (barrier); wave 0 alone forms y, w, the next column and the
+// next reflector (barrier).  u_j is left in column j of A (rows j+1..), the
+// tridiagonal in T: ta (real diagonal), tb |e_j|, ph the phase of e_j, hc c_j.
+template <int NTHR>
+__device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
+  constexpr int NW = NTHR / 64, NC = tri_cols<NW>();
+  const int i = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lda = jac_ne(n);
+  float2 *ub = T.part + NW * 64;  // [2][64] u_j by parity
+  float2 *wb = ub + 128;          // [64] w of the last step
+  float2 *xb = wb + 64;           // [64] the next column before its update
+  float2 *pp = T.part;            // [NW][64] partial products
+  int c0 = w == 0 ? NW : w;       // slot t: column c0 + NW t
+  float2 a[NC];
+#pragma unroll
+  for (int t = 0; t < NC; ++t) {
+    const int jc = c0 + NW * t;
+    a[t] = jc < n && i < n ? A[jc * lda + i] : make_float2(0.f, 0.f);
+  }
+  // wave 0: the reflector of column x (rows > j) for step j, into ub / A / T
+  auto reflector = [&](int j, float2 x, float dj) {
+    const float s = wave_sum(i > j + 1 ? x.x * x.x + x.y * x.y : 0.0f);
+    const float2 al = rl2(x, j + 1);
+    const float a2 = al.x * al.x + al.y * al.y;
+    const float av = __builtin_amdgcn_sqrtf(a2), nx = __builtin_amdgcn_sqrtf(a2 + s);
+    const bool refl = s > 0.0f;
+    const float ia = av > 0.0f ? __builtin_amdgcn_rcpf(av) : 0.0f;
+    const float2 ph = av > 0.0f ? make_float2(al.x * ia, al.y * ia) : make_float2(1.f, 0.f);
+    const float c = refl ? __builtin_amdgcn_rcpf(nx * (nx + av)) : 0.0f;
+    float2 u = i == j + 1 ? make_float2(ph.x * (av + nx), ph.y * (av + nx)) : x;
+    if (i <= j || i >= n) u = make_float2(0.f, 0.f);
+    ub[(j & 1) * 64 + i] = u;
+    if (i > j && i < n) A[j * lda + i] = u;
+    if (i == 0) {
+      T.ta[j] = dj;
+      T.tb[j] = refl ? nx : av;
+      T.ph[j] = refl ? make_float2(-ph.x, -ph.y) : ph;
+      T.hc[j] = c;
+    }
+  };
+  if (w == 0) reflector(0, i > 0 && i < n ? A[i] : make_float2(0.f, 0.f), A[0].x);
+  __syncthreads();
+  // the previous step's update on this wave's columns (every slot: the retired
+  // ones have shifted out)
+  // (u, w per lane, the column's entries by readlane: no memory wait per column)
+  auto update = [&](int j, int cnt) {
+    const float2 ui = ub[((j - 1) & 1) * 64 + i], wi = wb[i];
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
+      if (t < cnt) {
+        const int jc = c0 + NW * t;
+        const float2 uc = rl2(ui, jc), wc = rl2(wi, jc);
+        const float2 t1 = cmulc(ui, wc), t2 = cmulc(wi, uc);
+        a[t].x -= t1.x + t2.x;
+        a[t].y -= t1.y + t2.y;
+      }
+    }
+  };
+  for (int j = 0; j + 2 < n; ++j) {
+    if (c0 == j) {  // column j became step j's x: shift it out
+#pragma unroll
+      for (int t = 0; t + 1 < NC; ++t) a[t] = a[t + 1];
+      c0 += NW;
+    }
+    const int cnt = c0 < n ? (n - c0 + NW - 1) / NW : 0;
+    if (j > 0) update(j, cnt);
+    // this wave's part of A22 u_j; the owner of column j+1 hands it over
+    float2 yp = make_float2(0.f, 0.f);
+    const float2 uj = ub[(j & 1) * 64 + i];
+#pragma unroll
+    for (int t = 0; t < NC; ++t) {
+      if (t < cnt) {
+        const float2 uc = rl2(uj, c0 + NW * t);
+        yp.x += a[t].x * uc.x - a[t].y * uc.y;
+        yp.y += a[t].x * uc.y + a[t].y * uc.x;
+      }
+    }
+    if (c0 == j + 1) xb[i] = a[0];
+    pp[w * 64 + i] = yp;
+    __syncthreads();
+    if (w == 0) {
+      const float2 u = ub[(j & 1) * 64 + i];
+      const float c = T.hc[j];
+      float2 y = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int q = 0; q < NW; ++q) {
+        const float2 t2 = pp[q * 64 + i];
+        y.x += t2.x;
+        y.y += t2.y;
+      }
+      y.x *= c;
+      y.y *= c;
+      const float kap = 0.5f * c * wave_sum(u.x * y.x + u.y * y.y);
+      const float2 wv = i > j && i < n ? make_float2(y.x - kap * u.x, y.y - kap * u.y) : make_float2(0.f, 0.f);
+      wb[i] = wv;
+      // column j+1 with step j's update: the next x and diagonal entry
+      const float2 xo = xb[i];
+      const float2 u1 = rl2(u, j + 1), w1 = rl2(wv, j + 1);
+      const float2 t1 = cmulc(u, w1), t2 = cmulc(wv, u1);
+      const float2 xn = i > j + 1 && i < n ? make_float2(xo.x - t1.x - t2.x, xo.y - t1.y - t2.y)
+                                           : make_float2(0.f, 0.f);
+      const float dn = rlf(xo.x, j + 1) - 2.0f * (u1.x * w1.x + u1.y * w1.y);
+      if (j + 3 < n) {
+        reflector(j + 1, xn, dn);
+      } else {  // the last 2 x 2: a_{n-2,n-2} and e_{n-2}
+        if (i == 0) {
+          T.ta[n - 2] = dn;
+          T.hc[n - 2] = 0.0f;
+        }
+        if (i == n - 1) {
+          const float ae = sqrtf(xn.x * xn.x + xn.y * xn.y);
+          T.tb[n - 2] = ae;
+          T.ph[n - 2] = ae > 0.0f ? make_float2(xn.x / ae, xn.y / ae) : make_float2(1.f, 0.f);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // a_{n-1,n-1}: the last update on its owner's column
+  if (c0 == n - 2) {
+#pragma unroll
+    for (int t = 0; t + 1 < NC; ++t) a[t] = a[t + 1];
+    c0 += NW;
+  }
+  if (c0 == n - 1) {
+    update(n - 2, 1);
+    if (i == n - 1) T.ta[n - 1] = a[0].x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // diagonal similarity D (phi_0 = 1, phi_{j+1} = phi_j e_j / |e_j|): D^H T D
+    // is real with off-diagonal |e_j|
+    float2 cur = make_float2(1.f, 0.f);
+    for (int j = 0; j < n; ++j) {
+      const float2 pe = j + 1 < n ? T.ph[j] : make_float2(1.f, 0.f);
+      T.ph[j] = cur;
+      cur = cmul(cur, pe);
+      const float s2 = rsqrtf(cur.x * cur.x + cur.y * cur.y);
+      cur.x *= s2;
+      cur.y *= s2;
+    }
+  }
+  __syncthreads();
+}
+
+// Eigenvalues of the real tridiagonal (ta, tb) into ev, ascending (perm =
+// identity): the top nsel by multisection, eigenvalue k in one group of
+// lanes of a wave, each lane a Sturm count at its point (the number of
+// eigenvalues below it, from the LDL^T pivots with the usual minimum
+// pivot); the others (nsel < n) as their mean, trace minus the computed
+// ones over n - nsel.  Returns the Gershgorin bound of |T|.
+template <int NTHR>
+__device__ __forceinline__ float tri_eigenvalues(int n, int nsel, const TriLds &T, float *ev, int *perm) {
+  constexpr int NW = NTHR / 64;
+  const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const float bl = r >= 1 && r < n ? T.tb[r - 1] : 0.f, br = r + 1 < n ? T.tb[r] : 0.f;
+  const float ar = r < n ? T.ta[r] : 0.f;
+  float glo = -wave_max_any(r < n ? -(ar - bl - br) : -3e38f);
+  float ghi = wave_max_any(r < n ? ar + bl + br : -3e38f);
+  const float tn = fmaxf(fabsf(glo), fabsf(ghi));
+  const float pivmin = 1e-30f * fmaxf(1.0f, wave_max(br * br));
+  const float pad = 2.4e-7f * tn * (float)n + pivmin;
+  glo -= pad;
+  ghi += pad;
+  const int k0 = n - nsel;
+  const int G = (nsel + NW - 1) / NW, nt = min(63, max(1, 64 / max(G, 1)));
+  const int g = r / nt, t = r - g * nt, k = k0 + w * G + g;
+  const bool act = nsel > 0 && g < G && k < n;
+  int iters = 1;
+  for (long p = 1; p < (1L << 30); p *= nt + 1) ++iters;
+  const float inv = 1.0f / (float)(nt + 1), fr = (float)(t + 1) * inv;
+  float lo = glo, hi = ghi;
+  if (nsel > 0 && w * G < nsel) {
+    for (int it = 0; it < iters; ++it) {
+      const float x = fmaf(hi - lo, fr, lo);
+      float q = T.ta[0] - x;
+      int cnt = q < 0.0f;
+#pragma unroll 4
+      for (int i = 1; i < n; ++i) {
+        const float b = T.tb[i - 1];
+        q = fabsf(q) < pivmin ? -pivmin : q;
+        q = (T.ta[i] - x) - b * b * __builtin_amdgcn_rcpf(q);
+        cnt += q < 0.0f;
+      }
+      const unsigned long long bal = __ballot(act && cnt >= k + 1);
+      const unsigned long long bits = (bal >> (g * nt)) & ((nt >= 64 ? 0ull : (1ull << nt)) - 1ull);
+      const int t1 = bits ? (int)__builtin_ctzll(bits) : nt;
+      const float nlo = t1 > 0 ? fmaf(hi - lo, (float)t1 * inv, lo) : lo;
+      const float nhi = t1 < nt ? fmaf(hi - lo, (float)(t1 + 1) * inv, lo) : hi;
+      lo = nlo;
+      hi = nhi;
+    }
+  }
+  if (act && t == 0) ev[k] = 0.5f * (lo + hi);
+  if (threadIdx.x < n) perm[threadIdx.x] = threadIdx.x;
+  __syncthreads();
+  if (w == 0) {
+    // ascending through ties (prefix maximum); the uncomputed ones: their mean
+    float v = r >= k0 && r < n ? ev[r] : -3e38f;
+    const float top = wave_sum(r >= k0 && r < n ? ev[r] : 0.0f), tr = wave_sum(ar);
+    const float mean = k0 > 0 ? (tr - top) / (float)k0 : 0.0f;
+    v = wave_scan_max(v);
+    if (r < n) ev[r] = r < k0 ? fminf(mean, k0 < n ? ev[k0] : mean) : v;
+  }
+  __syncthreads();
+  return tn;
+}
+
+// Eigenvectors of the `need` largest eigenvalues into columns n-1, n-2, ...
+// of V (the other columns zero): inverse iteration on the real tridiagonal
+// (two solves of (T - lambda I) z = b, Gaussian elimination with partial
+// pivoting, pivots kept away from zero, pseudo-random start), modified
+// Gram-Schmidt over the vectors, then v = H_0 ... H_{n-3} D z.  Returns
+// false (uniform) if a vector is not finite.
+template <int NTHR>
+__device__ __forceinline__ bool tri_vectors(const float2 *A, int n, int need, const TriLds &T, const float *ev,
+                                            float tn, float2 *V, int ldv) {
+  constexpr int NW = NTHR / 64;
+  const int tid = threadIdx.x, r = tid & 63, w = tid >> 6, lda = jac_ne(n);
+  const float tiny = fmaxf(2.4e-7f * tn, 1e-20f);  // a zero matrix: z = 1e20 b
+  if (tid == 0) T.flag[0] = 1;
+  if (w == 0) {
+    const int k = r;
+    const bool own = k < need;
+    float *z = T.z + k, *f = T.fac + k;
+    if (own) {
+      const float sig = ev[n - 1 - k];
+      const unsigned mul = 0x9E3779B1u + 0x632BE5ABu * (unsigned)k;
+      bool fin = true;
+      float sc = 1.0f;  // the right-hand side of the second solve: the first solution over its norm
+      for (int it = 0; it < 2; ++it) {
+        auto rhs = [&](int ii) {
+          return it == 0 ? (float)(((unsigned)(ii + 1) * mul) >> 8) * (1.0f / 8388608.0f) - 1.0f
+                         : z[ii * kTriK] * sc;
+        };
+        float p = T.ta[0] - sig, q = T.tb[0], rr = rhs(0);
+        for (int i = 0; i + 1 < n; ++i) {
+          const float bi = T.tb[i], dn = T.ta[i + 1] - sig, bn = i + 2 < n ? T.tb[i + 1] : 0.f;
+          const float rn = rhs(i + 1);
+          float *fi = f + i * 4 * kTriK;
+          if (fabsf(bi) > fabsf(p)) {  // row i+1 is the pivot row
+            const float ib = __builtin_amdgcn_rcpf(bi), mlt = p * ib;
+            fi[0] = ib;
+            fi[kTriK] = dn;
+            fi[2 * kTriK] = bn;
+            fi[3 * kTriK] = rn;
+            p = q - mlt * dn;
+            q = -mlt * bn;
+            rr = rr - mlt * rn;
+          } else {
+            if (fabsf(p) < tiny) p = p >= 0.0f ? tiny : -tiny;
+            const float ip = __builtin_amdgcn_rcpf(p), mlt = bi * ip;
+            fi[0] = ip;
+            fi[kTriK] = q;
+            fi[2 * kTriK] = 0.0f;
+            fi[3 * kTriK] = rr;
+            p = dn - mlt * q;
+            q = bn;
+            rr = rn - mlt * rr;
+          }
+        }
+        if (fabsf(p) < tiny) p = p >= 0.0f ? tiny : -tiny;
+        float *fl = f + (n - 1) * 4 * kTriK;
+        fl[0] = __builtin_amdgcn_rcpf(p);
+        fl[kTriK] = 0.0f;
+        fl[2 * kTriK] = 0.0f;
+        fl[3 * kTriK] = rr;
+        float z1 = 0.f, z2 = 0.f;
+        double ss = 0.0;  // |z| up to ~1e20 |b|
+        for (int i = n - 1; i >= 0; --i) {
+          const float *fi = f + i * 4 * kTriK;
+          const float zi = (fi[3 * kTriK] - fi[kTriK] * z1 - fi[2 * kTriK] * z2) * fi[0];
+          z[i * kTriK] = zi;
+          ss += (double)zi * zi;
+          z2 = z1;
+          z1 = zi;
+        }
+        fin = fin && ss > 0.0 && ss < 1e300;
+        sc = fin ? (float)(1.0 / sqrt(ss)) : 0.0f;
+      }
+#pragma unroll 4
+      for (int i = 0; i < n; ++i) z[i * kTriK] *= sc;
+      if (!fin) T.flag[0] = 0;
+    }
+    // modified Gram-Schmidt, descending eigenvalues (vector k after 0 .. k-1)
+    for (int j = 0; j + 1 < need; ++j) {
+      wave_sync();
+      if (own && k > j) {
+        const float *zj = T.z + j;
+        float dot = 0.0f;
+        for (int i = 0; i < n; ++i) dot += zj[i * kTriK] * z[i * kTriK];
+        float ss = 0.0f;
+        for (int i = 0; i < n; ++i) {
+          const float v = z[i * kTriK] - dot * zj[i * kTriK];
+          z[i * kTriK] = v;
+          ss += v * v;
+        }
+        const float sc = ss > 0.0f ? rsqrtf(ss) : 0.0f;
+        if (k == j + 1)
+          for (int i = 0; i < n; ++i) z[i * kTriK] *= sc;
+      }
+    }
+  }
+  for (int e = tid; e < (n - need) * n; e += NTHR) {
+    const int col = e / n, i = e - col * n;
+    V[col * ldv + i] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  for (int k = w; k < need; k += NW) {
+    const float zi = r < n ? T.z[r * kTriK + k] : 0.f;
+    const float2 phr = r < n ? T.ph[r] : make_float2(0.f, 0.f);
+    float2 v = make_float2(phr.x * zi, phr.y * zi);
+    for (int j = n - 3; j >= 0; --j) {
+      const float c = T.hc[j];
+      if (c == 0.0f) continue;
+      const float2 uj = r > j && r < n ? A[j * lda + r] : make_float2(0.f, 0.f);
+      const float dr = wave_sum(uj.x * v.x + uj.y * v.y), di = wave_sum(uj.x * v.y - uj.y * v.x);
+      const float2 s = make_float2(c * dr, c * di);  // c u^H v
+      v.x -= s.x * uj.x - s.y * uj.y;
+      v.y -= s.x * uj.y + s.y * uj.x;
+    }
+    if (r < n) V[(n - 1 - k) * ldv + r] = v;
+  }
+  __syncthreads();
+  const bool ok = T.flag[0] != 0;
+  __syncthreads();
+  return ok;
+}
+
 // MFMA A-operand fragments of beam rows [row0, row0 + nrows) of one (channel,
 // pol) in the layout [NMT][NK][hi, lo][64 lanes] x 8 halves that the beam
 // kernel reads (rows of [Re; Im] pairs, K = (antenna, re/im)); weight(e, a)
@@ -1349,36 +1755,39 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
   const long nbl = (long)n * (n + 1) / 2;
 
   // 1. dense Hermitian covariance of pol p (A(i,j) = V[bl(i,j)][p][p], i <= j)
-  for (int idx = tid; idx < nbl; idx += nthr) {
-    int j = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
-    while ((long)j * (j + 1) / 2 > idx) --j;
-    while ((long)(j + 1) * (j + 2) / 2 <= idx) ++j;
-    const int i = idx - j * (j + 1) / 2;
-    long re = 0, im = 0;
-    for (int k = 0; k < P.nsplit; ++k) {
-      const int2 v = reinterpret_cast<const int2 *>(P.vis_part)[(((long)k * P.C + c) * 4 + 3 * p) * nbl + idx];
-      re += v.x;
-      im += v.y;
-    }
-    const float2 a = make_float2((float)re, (float)im);
-    S.A[j * lda + i] = a;
-    S.A[i * lda + j] = make_float2(a.x, -a.y);
-  }
-  __syncthreads();
   // 2. whitening (spatial.cuh:1065-1093); off: the raw covariance, as the
   //    pulsar-fold pipeline's RFI branch decomposes it (d = 1 makes the
   //    unwhitening below a no-op)
-  if (tid < n) S.d[tid] = P.whiten ? sqrtf(fmaxf(S.A[tid * lda + tid].x, 1e-10f)) : 1.0f;
-  __syncthreads();
-  if (P.whiten) {
-    for (int i = tid; i < n * n; i += nthr) {
-      const int col = i / n, row = i - col * n;
-      const float den = S.d[row] * S.d[col];
-      S.A[col * lda + row].x /= den;
-      S.A[col * lda + row].y /= den;
+  auto load_matrix = [&]() {
+    for (int idx = tid; idx < nbl; idx += nthr) {
+      int j = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
+      while ((long)j * (j + 1) / 2 > idx) --j;
+      while ((long)(j + 1) * (j + 2) / 2 <= idx) ++j;
+      const int i = idx - j * (j + 1) / 2;
+      long re = 0, im = 0;
+      for (int k = 0; k < P.nsplit; ++k) {
+        const int2 v = reinterpret_cast<const int2 *>(P.vis_part)[(((long)k * P.C + c) * 4 + 3 * p) * nbl + idx];
+        re += v.x;
+        im += v.y;
+      }
+      const float2 a = make_float2((float)re, (float)im);
+      S.A[j * lda + i] = a;
+      S.A[i * lda + j] = make_float2(a.x, -a.y);
     }
     __syncthreads();
-  }
+    if (tid < n) S.d[tid] = P.whiten ? sqrtf(fmaxf(S.A[tid * lda + tid].x, 1e-10f)) : 1.0f;
+    __syncthreads();
+    if (P.whiten) {
+      for (int i = tid; i < n * n; i += nthr) {
+        const int col = i / n, row = i - col * n;
+        const float den = S.d[row] * S.d[col];
+        S.A[col * lda + row].x /= den;
+        S.A[col * lda + row].y /= den;
+      }
+      __syncthreads();
+    }
+  };
+  load_matrix();
   // 3. eigen decomposition: plain cyclic Jacobi, or a warm start refined by a
   //    few block sweeps and accepted only if the top-K residual is within tolerance
   const long cp = (long)c * 2 + p;
@@ -1403,9 +1812,27 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
       fell = 1;
     }
   } else {
-    sweeps = jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.jl());
-    sort_eigen(S.A, n, S.perm, S.ev);
-    K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
+    // without eigen outputs only the removed modes' vectors are needed: the
+    // tridiagonal path, unless it needs more than kTriK vectors (or one is
+    // not finite): then the matrix is formed again for the Jacobi solver
+    bool solved = false;
+    if (!P.full_outputs && !P.shrink && n >= 3 && n <= (NTHR / 64) * tri_cols<NTHR / 64>()) {
+      const TriLds T = tri_lds(S.W, n);
+      tri_reduce<NTHR>(S.A, n, T);
+      // with detection all eigenvalues (percentiles); else the top ones used
+      const int kw = P.basis_out ? min(P.KW, n) : 0;
+      const int nsel = P.detect ? n : min(n, max(max(0, min(n, fixed)), kw));
+      const float tn = tri_eigenvalues<NTHR>(n, nsel, T, S.ev, S.perm);
+      K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
+      const int need = max(K, kw);
+      if (need <= kTriK) solved = tri_vectors<NTHR>(S.A, n, need, T, S.ev, tn, S.V, S.ldv);
+      if (!solved) load_matrix();
+    }
+    if (!solved) {
+      sweeps = jacobi_lds<NTHR>(S.A, S.V, n, P.max_sweeps, P.tol, S.jl());
+      sort_eigen(S.A, n, S.perm, S.ev);
+      K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
+    }
   }
   // 4. eigenvalues of the removed modes to the precision of the exact covariance
   if (K > 0)

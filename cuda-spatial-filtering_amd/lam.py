@@ -595,8 +595,11 @@ class Pipeline:
         s = self.shape
         return np.zeros((s.C, 2, self.n_out_beams, s.T, 2), np.float16)
 
-    def run_host(self, ring, scales):
-        """Convenience: one block from host arrays, returns fp16 beams (+eigen)."""
+    def run_host(self, ring, scales, eigen=True):
+        """Convenience: one block from host arrays, returns fp16 beams (+eigen).
+
+        eigen=False (adaptive mode): only the mode counts are requested, so
+        no eigenvalues or eigenvectors land anywhere."""
         if self.mode == MODE_PROJECTION:  # no per-block product: only the accumulator
             self.execute(np.ascontiguousarray(ring, np.int8), np.ascontiguousarray(scales, np.int16))
             self.sync()
@@ -608,6 +611,8 @@ class Pipeline:
             extra = dict(evals_out=np.zeros((s.C, 2, s.R), np.float32),
                          evecs_out=np.zeros((s.C, 2, s.R, s.R, 2), np.float32),
                          counts_out=np.zeros((s.C, 2), np.int32))
+            if not eigen:
+                del extra["evals_out"], extra["evecs_out"]
             if self.spectra:
                 extra["spectra_out"] = self.spectra_array()
         elif self.mode == MODE_LAMBDA_GPU:

@@ -497,6 +497,47 @@ def test_adaptive_scene_vs_oracle(shrink):
             assert np.max(np.abs(Ug @ Ug.conj().T - Ur @ Ur.conj().T)) < 1e-4
 
 
+@pytest.mark.parametrize("R,K,detect,tones", [(36, 1, False, 1), (36, 2, False, 2), (36, 4, False, 2),
+                                               (36, 0, True, 3), (12, 2, False, 2), (3, 1, False, 1),
+                                               (64, 3, False, 3), (36, 10, False, 2)])
+def test_adaptive_without_eigen_outputs(R, K, detect, tones):
+    """A block that requests no eigenvalues or eigenvectors takes the
+    tridiagonal solve (reduction, multisection, inverse iteration; more than
+    8 removed modes: the Jacobi solver): K and the projected beams against
+    the FP64 oracle, and against the same block with eigen outputs (the
+    Jacobi solver).  An all-zero channel rides along."""
+    shape = syn.BlockShape(C=4, F=1, NT=64, R=R, P=2, RPP=R, NP=4, B=4)
+    _log(f"adaptive without eigen outputs R={R} K={K} detect={detect}")
+    scene = syn.make_scene(shape, seed=31 + R + K, n_tones=tones, inr_db=(8, 30))
+    ring, scales = syn.make_block(shape, scene, seed=32 + R)
+    ring[3] = 0  # channel 3 silent: zero covariance
+    w16, w = syn.random_weights(shape, 33)
+    outs = []
+    for eigen in (False, True):
+        p = lam.Pipeline(shape, lam.MODE_ADAPTIVE)
+        p.set_weights(w16)
+        p.set_fixed_k(np.full((shape.C, 2), K))
+        p.set_nulling(detect=detect, delta=3.0)
+        p.set_eig_params(20, 1e-7)
+        outs.append(p.run_host(ring, scales, eigen=eigen))
+        outs[-1]["sweeps"] = p.eig_stats()[0]
+        p.close()
+    tri, jac = outs
+    assert "evals_out" not in tri
+    # the tridiagonal solve reports no sweeps; beyond 8 vectors the Jacobi solver ran
+    assert (tri["sweeps"][:3] > 0).all() if K > 8 else (tri["sweeps"] == 0).all()
+    s = _o(shape)
+    y = orc.ingest(s, ring, scales, orc.WINDOW_ADAPTIVE)
+    ref = orc.adaptive_block(s, y, w, fixed_k=np.full((shape.C, 2), K), detect=detect, delta=3.0)
+    assert list(tri["counts_out"].ravel()) == list(ref["counts"].ravel())
+    assert list(tri["counts_out"].ravel()) == list(jac["counts_out"].ravel())
+    B = shape.B
+    for lo, hi in ((0, B), (B, 2 * B)):
+        assert rel_err(tri["beams"][:, :, lo:hi], ref["beams2b"][:, :, lo:hi]) < BEAM_TOL
+        assert rel_err(tri["beams"][:, :, lo:hi], jac["beams"][:, :, lo:hi]) < BEAM_TOL
+    assert not tri["beams"][3].any()
+
+
 def test_adaptive_spectra_vs_oracle():
     """Post-beam spectra (register_fft_block product, adaptive 808-848): the
     GPU spectra of the GPU's own fp16 beams match the FP64 DFT oracle within
