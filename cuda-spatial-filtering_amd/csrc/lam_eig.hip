@@ -720,6 +720,30 @@ __device__ __forceinline__ float dpp_move(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
                                                                ROW_MASK, 0xf, false));
 }
+// Lane l of v (l uniform).
+__device__ __forceinline__ float rlf(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ float2 rl2(float2 v, int l) { return make_float2(rlf(v.x, l), rlf(v.y, l)); }
+// Sum over the 64 lanes in FP64 (the DPP pattern of wave_sum on both halves).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_move_f64(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+  v += dpp_move_f64<0x111, 0xf>(v);
+  v += dpp_move_f64<0x112, 0xf>(v);
+  v += dpp_move_f64<0x114, 0xf>(v);
+  v += dpp_move_f64<0x118, 0xf>(v);
+  v += dpp_move_f64<0x142, 0xa>(v);
+  v += dpp_move_f64<0x143, 0xc>(v);
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
 __device__ __forceinline__ float wave_sum(float v) {
   v += dpp_move<0x111, 0xf>(v);  // row_shr:1
   v += dpp_move<0x112, 0xf>(v);  // row_shr:2
@@ -1301,13 +1325,20 @@ __device__ __forceinline__ void refine_removed_eigenvalues(EigSmem &S, int n, in
         acc[m] = df_add(acc[m], df_add(df_mul(yr, u[i].x * si), df_mul(yi, u[i].y * si)));
       }
     }
-    // block reduction per mode: across the wave by shuffles, then across waves
+    // block reduction per mode: across the wave by DPP moves (the pattern of
+    // wave_sum, compensated adds; the total in lane 63), then across waves
 #pragma unroll
     for (int m = 0; m < kRefineModes; ++m) {
-      DF v = acc[m];
-      for (int off = 32; off > 0; off >>= 1)
-        v = df_add(v, DF{__shfl_xor(v.hi, off, 64), __shfl_xor(v.lo, off, 64)});
-      acc[m] = v;
+      if (m < km) {
+        DF v = acc[m];
+        v = df_add(v, DF{dpp_move<0x111, 0xf>(v.hi), dpp_move<0x111, 0xf>(v.lo)});
+        v = df_add(v, DF{dpp_move<0x112, 0xf>(v.hi), dpp_move<0x112, 0xf>(v.lo)});
+        v = df_add(v, DF{dpp_move<0x114, 0xf>(v.hi), dpp_move<0x114, 0xf>(v.lo)});
+        v = df_add(v, DF{dpp_move<0x118, 0xf>(v.hi), dpp_move<0x118, 0xf>(v.lo)});
+        v = df_add(v, DF{dpp_move<0x142, 0xa>(v.hi), dpp_move<0x142, 0xa>(v.lo)});
+        v = df_add(v, DF{dpp_move<0x143, 0xc>(v.hi), dpp_move<0x143, 0xc>(v.lo)});
+        acc[m] = DF{rlf(v.hi, 63), rlf(v.lo, 63)};
+      }
     }
     __syncthreads();  // the previous users of S.A are done
     if (lane == 0) {
@@ -1342,10 +1373,6 @@ __device__ __forceinline__ TriLds tri_lds(float2 *W, int n) {
   return t;
 }
 
-__device__ __forceinline__ float rlf(float v, int l) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-}
-__device__ __forceinline__ float2 rl2(float2 v, int l) { return make_float2(rlf(v.x, l), rlf(v.y, l)); }
 // Inclusive prefix maximum over the lanes (any sign): the DPP pattern of
 // wave_sum with -3e38 where a move has no source.
 template <int CTRL, int ROW_MASK>
@@ -1366,41 +1393,36 @@ __device__ __forceinline__ float wave_scan_max(float v) {
 // Maximum over the 64 lanes (any sign).
 __device__ __forceinline__ float wave_max_any(float v) { return rlf(wave_scan_max(v), 63); }
 
-// Columns of the matrix a wave holds in registers during the reduction
-// (column k in wave k mod waves); the tridiagonal path takes n up to
-// waves x tri_cols.
+// Waves that own columns of the matrix during the reduction (with 4 waves,
+// wave 0 keeps to the serial part).
 template <int NW>
-constexpr int tri_cols() { return NW >= 4 ? 16 : 32; }
+constexpr int tri_owners() { return NW >= 4 ? NW - 1 : NW; }
 
-// Reduction of the Hermitian A (3 <= n <= waves x tri_cols, full storage,
-// leading dimension jac_ne(n)) to T = Q^H A Q tridiagonal with
-// Q = H_0 ... H_{n-3}, H_j = I - c_j u_j u_j^H acting on rows j+1..
-// (Hermitian reflectors): H_j maps column j below the diagonal, x, to
-// -e^{i arg x_{j+1}} |x| e_{j+1}.  Step j updates the trailing matrix as
-// A22 <- A22 - u w^H - w u^H with y = c A22 u, w = y - (c/2)(u^H y) u.
-// The matrix lives in registers (lane i: row i of the wave's columns; the
-// lowest column leaves when it becomes the next x and the slots shift
-// down).  Each step: every wave applies the previous step's update to its
-// columns and forms its part of A22 u in the same pass, u and w read as LDS
-// broadcasts (barrier); wave 0 alone forms y, w, the next column and the
-// next reflector (barrier).  u_j is left in column j of A (rows j+1..), the
-// tridiagonal in T: ta (real diagonal), tb |e_j|, ph the phase of e_j, hc c_j.
+// Reduction of the Hermitian A (n >= 3, full storage, leading dimension
+// jac_ne(n)) to T = Q^H A Q tridiagonal with Q = H_0 ... H_{n-3},
+// H_j = I - c_j u_j u_j^H acting on rows j+1.. (Hermitian reflectors): H_j
+// maps column j below the diagonal, x, to -e^{i arg x_{j+1}} |x| e_{j+1}.
+// Step j: y = c A22 u, w = y - (c/2)(u^H y) u, A22 <- A22 - u w^H - w u^H.
+// The matrix stays in LDS one update behind: the product of step j uses the
+// columns before step j-1's update and is corrected by -u'(w'^H u) -
+// w'(u'^H u) (u', w' of step j-1); the owner waves apply that update to the
+// columns above j+1 while wave 0 forms y, w, the next column (column j+1 with
+// both pending updates) and the next reflector.  Owners walk their columns
+// in loops whose loads batch, the vectors' entries read as LDS broadcasts
+// (no cross-lane moves).  Two barriers per step.  u_j is left in column j
+// of A (rows j+1..), the tridiagonal in T: ta (real diagonal), tb |e_j|, ph
+// the phase of e_j, hc c_j.
 template <int NTHR>
 __device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
-  constexpr int NW = NTHR / 64, NC = tri_cols<NW>();
+  constexpr int NW = NTHR / 64, OW = tri_owners<NW>();
   const int i = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lda = jac_ne(n);
-  float2 *ub = T.part + NW * 64;  // [2][64] u_j by parity
-  float2 *wb = ub + 128;          // [64] w of the last step
-  float2 *xb = wb + 64;           // [64] the next column before its update
-  float2 *pp = T.part;            // [NW][64] partial products
-  int c0 = w == 0 ? NW : w;       // slot t: column c0 + NW t
-  float2 a[NC];
-#pragma unroll
-  for (int t = 0; t < NC; ++t) {
-    const int jc = c0 + NW * t;
-    a[t] = jc < n && i < n ? A[jc * lda + i] : make_float2(0.f, 0.f);
-  }
-  // wave 0: the reflector of column x (rows > j) for step j, into ub / A / T
+  const int ow = NW >= 4 ? w - 1 : w;    // owner index (-1: wave 0 with 4 waves)
+  float2 *pp = T.part;                   // [OW][64] partial products
+  float4 *uw = reinterpret_cast<float4 *>(pp + OW * 64);  // [2][64] (u_j, w_j) by parity
+  float2 *un = reinterpret_cast<float2 *>(uw + 2 * 64);   // [2][64] u_j by parity (the product's)
+  // wave 0's vectors: u_j, u_{j-1}, w_{j-1} (lane i: row i)
+  float2 uc = make_float2(0.f, 0.f), up = make_float2(0.f, 0.f), wp = make_float2(0.f, 0.f);
+  // wave 0: the reflector of column x (rows > j) for step j, into un / A / T
   auto reflector = [&](int j, float2 x, float dj) {
     const float s = wave_sum(i > j + 1 ? x.x * x.x + x.y * x.y : 0.0f);
     const float2 al = rl2(x, j + 1);
@@ -1412,7 +1434,7 @@ __device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
     const float c = refl ? __builtin_amdgcn_rcpf(nx * (nx + av)) : 0.0f;
     float2 u = i == j + 1 ? make_float2(ph.x * (av + nx), ph.y * (av + nx)) : x;
     if (i <= j || i >= n) u = make_float2(0.f, 0.f);
-    ub[(j & 1) * 64 + i] = u;
+    un[(j & 1) * 64 + i] = u;
     if (i > j && i < n) A[j * lda + i] = u;
     if (i == 0) {
       T.ta[j] = dj;
@@ -1420,71 +1442,63 @@ __device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
       T.ph[j] = refl ? make_float2(-ph.x, -ph.y) : ph;
       T.hc[j] = c;
     }
+    return u;
   };
-  if (w == 0) reflector(0, i > 0 && i < n ? A[i] : make_float2(0.f, 0.f), A[0].x);
+  if (w == 0) uc = reflector(0, i > 0 && i < n ? A[i] : make_float2(0.f, 0.f), A[0].x);
   __syncthreads();
-  // the previous step's update on this wave's columns (every slot: the retired
-  // ones have shifted out)
-  // (u, w per lane, the column's entries by readlane: no memory wait per column)
-  auto update = [&](int j, int cnt) {
-    const float2 ui = ub[((j - 1) & 1) * 64 + i], wi = wb[i];
-#pragma unroll
-    for (int t = 0; t < NC; ++t) {
-      if (t < cnt) {
-        const int jc = c0 + NW * t;
-        const float2 uc = rl2(ui, jc), wc = rl2(wi, jc);
-        const float2 t1 = cmulc(ui, wc), t2 = cmulc(wi, uc);
-        a[t].x -= t1.x + t2.x;
-        a[t].y -= t1.y + t2.y;
-      }
-    }
+  // this owner's first column >= lo
+  auto first_col = [&](int lo) {
+    const int m = NW >= 4 ? ow : w;  // columns c = m (mod OW)
+    return lo + ((m - lo) % OW + OW) % OW;
   };
   for (int j = 0; j + 2 < n; ++j) {
-    if (c0 == j) {  // column j became step j's x: shift it out
-#pragma unroll
-      for (int t = 0; t + 1 < NC; ++t) a[t] = a[t + 1];
-      c0 += NW;
-    }
-    const int cnt = c0 < n ? (n - c0 + NW - 1) / NW : 0;
-    if (j > 0) update(j, cnt);
-    // this wave's part of A22 u_j; the owner of column j+1 hands it over
-    float2 yp = make_float2(0.f, 0.f);
-    const float2 uj = ub[(j & 1) * 64 + i];
-#pragma unroll
-    for (int t = 0; t < NC; ++t) {
-      if (t < cnt) {
-        const float2 uc = rl2(uj, c0 + NW * t);
-        yp.x += a[t].x * uc.x - a[t].y * uc.y;
-        yp.y += a[t].x * uc.y + a[t].y * uc.x;
+    if (ow >= 0) {
+      // this wave's part of A22 u_j (columns above j, one update behind)
+      const float2 *uj = un + (j & 1) * 64;
+      float2 yp = make_float2(0.f, 0.f);
+#pragma unroll 4
+      for (int c = first_col(j + 1); c < n; c += OW) {
+        const float2 e = uj[c];
+        const float2 av = i < n ? A[c * lda + i] : make_float2(0.f, 0.f);
+        yp.x += av.x * e.x - av.y * e.y;
+        yp.y += av.x * e.y + av.y * e.x;
       }
+      pp[ow * 64 + i] = yp;
     }
-    if (c0 == j + 1) xb[i] = a[0];
-    pp[w * 64 + i] = yp;
     __syncthreads();
     if (w == 0) {
-      const float2 u = ub[(j & 1) * 64 + i];
       const float c = T.hc[j];
       float2 y = make_float2(0.f, 0.f);
 #pragma unroll
-      for (int q = 0; q < NW; ++q) {
+      for (int q = 0; q < OW; ++q) {
         const float2 t2 = pp[q * 64 + i];
         y.x += t2.x;
         y.y += t2.y;
       }
+      const float2 xo = i < n ? A[(j + 1) * lda + i] : make_float2(0.f, 0.f);  // column j+1, before step j-1's update
+      // y = c (P - u' al - w' be) with al = w'^H u, be = u'^H u;
+      // Re(u^H y) / c = Re(u^H P) - 2 Re(conj(be) al)
+      const float al_r = wave_sum(wp.x * uc.x + wp.y * uc.y), al_i = wave_sum(wp.x * uc.y - wp.y * uc.x);
+      const float be_r = wave_sum(up.x * uc.x + up.y * uc.y), be_i = wave_sum(up.x * uc.y - up.y * uc.x);
+      const float g0 = wave_sum(uc.x * y.x + uc.y * y.y);
+      y.x -= up.x * al_r - up.y * al_i + wp.x * be_r - wp.y * be_i;
+      y.y -= up.x * al_i + up.y * al_r + wp.x * be_i + wp.y * be_r;
       y.x *= c;
       y.y *= c;
-      const float kap = 0.5f * c * wave_sum(u.x * y.x + u.y * y.y);
-      const float2 wv = i > j && i < n ? make_float2(y.x - kap * u.x, y.y - kap * u.y) : make_float2(0.f, 0.f);
-      wb[i] = wv;
-      // column j+1 with step j's update: the next x and diagonal entry
-      const float2 xo = xb[i];
-      const float2 u1 = rl2(u, j + 1), w1 = rl2(wv, j + 1);
-      const float2 t1 = cmulc(u, w1), t2 = cmulc(wv, u1);
-      const float2 xn = i > j + 1 && i < n ? make_float2(xo.x - t1.x - t2.x, xo.y - t1.y - t2.y)
+      const float kap = 0.5f * c * c * (g0 - 2.0f * (be_r * al_r + be_i * al_i));
+      const float2 wv = i > j && i < n ? make_float2(y.x - kap * uc.x, y.y - kap * uc.y) : make_float2(0.f, 0.f);
+      uw[(j & 1) * 64 + i] = make_float4(uc.x, uc.y, wv.x, wv.y);
+      // column j+1 with the updates of steps j-1 and j: the next x and diagonal entry
+      const float2 u1 = rl2(uc, j + 1), w1 = rl2(wv, j + 1), u0 = rl2(up, j + 1), w0 = rl2(wp, j + 1);
+      const float2 t1 = cmulc(uc, w1), t2 = cmulc(wv, u1), t3 = cmulc(up, w0), t4 = cmulc(wp, u0);
+      const float2 xn = i > j + 1 && i < n ? make_float2(xo.x - t1.x - t2.x - t3.x - t4.x,
+                                                         xo.y - t1.y - t2.y - t3.y - t4.y)
                                            : make_float2(0.f, 0.f);
-      const float dn = rlf(xo.x, j + 1) - 2.0f * (u1.x * w1.x + u1.y * w1.y);
+      const float dn = rlf(xo.x, j + 1) - 2.0f * (u1.x * w1.x + u1.y * w1.y) - 2.0f * (u0.x * w0.x + u0.y * w0.y);
+      up = uc;
+      wp = wv;
       if (j + 3 < n) {
-        reflector(j + 1, xn, dn);
+        uc = reflector(j + 1, xn, dn);
       } else {  // the last 2 x 2: a_{n-2,n-2} and e_{n-2}
         if (i == 0) {
           T.ta[n - 2] = dn;
@@ -1497,17 +1511,28 @@ __device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
         }
       }
     }
+    // step j-1's update on the owners' columns above j+1 (beside wave 0's part)
+    if (ow >= 0 && j > 0) {
+      const float4 *uwp = uw + ((j - 1) & 1) * 64;
+      const float4 me = uwp[i];  // (u'_i, w'_i)
+#pragma unroll 4
+      for (int c = first_col(j + 2); c < n; c += OW) {
+        const float4 e = uwp[c];
+        if (i < n) {
+          // a -= u'_i conj(w'_c) + w'_i conj(u'_c)
+          const float2 av = A[c * lda + i];
+          A[c * lda + i] = make_float2(av.x - (me.x * e.z + me.y * e.w + me.z * e.x + me.w * e.y),
+                                       av.y - (me.y * e.z - me.x * e.w + me.w * e.x - me.z * e.y));
+        }
+      }
+    }
     __syncthreads();
   }
-  // a_{n-1,n-1}: the last update on its owner's column
-  if (c0 == n - 2) {
-#pragma unroll
-    for (int t = 0; t + 1 < NC; ++t) a[t] = a[t + 1];
-    c0 += NW;
-  }
-  if (c0 == n - 1) {
-    update(n - 2, 1);
-    if (i == n - 1) T.ta[n - 1] = a[0].x;
+  // a_{n-1,n-1}: step n-4's update is in, step n-3's (wave 0's u', w') is applied here
+  if (w == 0) {
+    const float2 un1 = rl2(up, n - 1), wn1 = rl2(wp, n - 1);
+    const float d = A[(n - 1) * lda + n - 1].x - 2.0f * (un1.x * wn1.x + un1.y * wn1.y);
+    if (i == 0) T.ta[n - 1] = d;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1554,6 +1579,10 @@ __device__ __forceinline__ float tri_eigenvalues(int n, int nsel, const TriLds &
   const float inv = 1.0f / (float)(nt + 1), fr = (float)(t + 1) * inv;
   float lo = glo, hi = ghi;
   if (nsel > 0 && w * G < nsel) {
+    // pivots q_i = (a_i - x) - b_{i-1}^2 / q_{i-1}; with b^2 kept >= 1e-30 a
+    // zero pivot gives -inf next (counted) and then a_i - x again, the count
+    // of the limit q -> +0, so the chain needs no pivot guard
+    (void)pivmin;
     for (int it = 0; it < iters; ++it) {
       const float x = fmaf(hi - lo, fr, lo);
       float q = T.ta[0] - x;
@@ -1561,8 +1590,7 @@ __device__ __forceinline__ float tri_eigenvalues(int n, int nsel, const TriLds &
 #pragma unroll 4
       for (int i = 1; i < n; ++i) {
         const float b = T.tb[i - 1];
-        q = fabsf(q) < pivmin ? -pivmin : q;
-        q = (T.ta[i] - x) - b * b * __builtin_amdgcn_rcpf(q);
+        q = (T.ta[i] - x) - fmaxf(b * b, 1e-30f) * __builtin_amdgcn_rcpf(q);
         cnt += q < 0.0f;
       }
       const unsigned long long bal = __ballot(act && cnt >= k + 1);
@@ -1609,51 +1637,52 @@ __device__ __forceinline__ bool tri_vectors(const float2 *A, int n, int need, co
     if (own) {
       const float sig = ev[n - 1 - k];
       const unsigned mul = 0x9E3779B1u + 0x632BE5ABu * (unsigned)k;
+      // distinct arrays (restrict: the loads of a step need not wait for the
+      // previous step's stores)
+      const float *__restrict__ ta = T.ta;
+      const float *__restrict__ tb = T.tb;
+      float *__restrict__ zz = T.z + k;
+      float *__restrict__ ff = T.fac + k;
       bool fin = true;
       float sc = 1.0f;  // the right-hand side of the second solve: the first solution over its norm
       for (int it = 0; it < 2; ++it) {
         auto rhs = [&](int ii) {
           return it == 0 ? (float)(((unsigned)(ii + 1) * mul) >> 8) * (1.0f / 8388608.0f) - 1.0f
-                         : z[ii * kTriK] * sc;
+                         : zz[ii * kTriK] * sc;
         };
-        float p = T.ta[0] - sig, q = T.tb[0], rr = rhs(0);
+        float p = ta[0] - sig, q = tb[0], rr = rhs(0);
+#pragma unroll 4
         for (int i = 0; i + 1 < n; ++i) {
-          const float bi = T.tb[i], dn = T.ta[i + 1] - sig, bn = i + 2 < n ? T.tb[i + 1] : 0.f;
+          const float bi = tb[i], dn = ta[i + 1] - sig, bn = i + 2 < n ? tb[i + 1] : 0.f;
           const float rn = rhs(i + 1);
-          float *fi = f + i * 4 * kTriK;
-          if (fabsf(bi) > fabsf(p)) {  // row i+1 is the pivot row
-            const float ib = __builtin_amdgcn_rcpf(bi), mlt = p * ib;
-            fi[0] = ib;
-            fi[kTriK] = dn;
-            fi[2 * kTriK] = bn;
-            fi[3 * kTriK] = rn;
-            p = q - mlt * dn;
-            q = -mlt * bn;
-            rr = rr - mlt * rn;
-          } else {
-            if (fabsf(p) < tiny) p = p >= 0.0f ? tiny : -tiny;
-            const float ip = __builtin_amdgcn_rcpf(p), mlt = bi * ip;
-            fi[0] = ip;
-            fi[kTriK] = q;
-            fi[2 * kTriK] = 0.0f;
-            fi[3 * kTriK] = rr;
-            p = dn - mlt * q;
-            q = bn;
-            rr = rn - mlt * rr;
-          }
+          float *fi = ff + i * 4 * kTriK;
+          const bool sw = fabsf(bi) > fabsf(p);  // row i+1 is the pivot row
+          const float pv = sw ? bi : (fabsf(p) < tiny ? (p >= 0.0f ? tiny : -tiny) : p);
+          const float ip = __builtin_amdgcn_rcpf(pv), mlt = (sw ? p : bi) * ip;
+          fi[0] = ip;
+          fi[kTriK] = sw ? dn : q;
+          fi[2 * kTriK] = sw ? bn : 0.0f;
+          fi[3 * kTriK] = sw ? rn : rr;
+          const float np = sw ? q - mlt * dn : dn - mlt * q;
+          const float nq = sw ? -mlt * bn : bn;
+          const float nr = sw ? rr - mlt * rn : rn - mlt * rr;
+          p = np;
+          q = nq;
+          rr = nr;
         }
         if (fabsf(p) < tiny) p = p >= 0.0f ? tiny : -tiny;
-        float *fl = f + (n - 1) * 4 * kTriK;
+        float *fl = ff + (n - 1) * 4 * kTriK;
         fl[0] = __builtin_amdgcn_rcpf(p);
         fl[kTriK] = 0.0f;
         fl[2 * kTriK] = 0.0f;
         fl[3 * kTriK] = rr;
         float z1 = 0.f, z2 = 0.f;
         double ss = 0.0;  // |z| up to ~1e20 |b|
+#pragma unroll 4
         for (int i = n - 1; i >= 0; --i) {
-          const float *fi = f + i * 4 * kTriK;
+          const float *fi = ff + i * 4 * kTriK;
           const float zi = (fi[3 * kTriK] - fi[kTriK] * z1 - fi[2 * kTriK] * z2) * fi[0];
-          z[i * kTriK] = zi;
+          zz[i * kTriK] = zi;
           ss += (double)zi * zi;
           z2 = z1;
           z1 = zi;
@@ -1662,7 +1691,7 @@ __device__ __forceinline__ bool tri_vectors(const float2 *A, int n, int need, co
         sc = fin ? (float)(1.0 / sqrt(ss)) : 0.0f;
       }
 #pragma unroll 4
-      for (int i = 0; i < n; ++i) z[i * kTriK] *= sc;
+      for (int i = 0; i < n; ++i) zz[i * kTriK] *= sc;
       if (!fin) T.flag[0] = 0;
     }
     // modified Gram-Schmidt, descending eigenvalues (vector k after 0 .. k-1)
@@ -1693,9 +1722,9 @@ __device__ __forceinline__ bool tri_vectors(const float2 *A, int n, int need, co
     const float zi = r < n ? T.z[r * kTriK + k] : 0.f;
     const float2 phr = r < n ? T.ph[r] : make_float2(0.f, 0.f);
     float2 v = make_float2(phr.x * zi, phr.y * zi);
+#pragma unroll 2
     for (int j = n - 3; j >= 0; --j) {
-      const float c = T.hc[j];
-      if (c == 0.0f) continue;
+      const float c = T.hc[j];  // 0: no reflector (the product vanishes)
       const float2 uj = r > j && r < n ? A[j * lda + r] : make_float2(0.f, 0.f);
       const float dr = wave_sum(uj.x * v.x + uj.y * v.y), di = wave_sum(uj.x * v.y - uj.y * v.x);
       const float2 s = make_float2(c * dr, c * di);  // c u^H v
@@ -1758,6 +1787,16 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
   // 2. whitening (spatial.cuh:1065-1093); off: the raw covariance, as the
   //    pulsar-fold pipeline's RFI branch decomposes it (d = 1 makes the
   //    unwhitening below a no-op)
+  // the beam weights, loaded now so their latency hides behind the solve
+  constexpr int kWPre = 3;
+  const bool wfit = B * n <= kWPre * nthr;
+  __half2 wpre[kWPre];
+#pragma unroll
+  for (int q = 0; q < kWPre; ++q) {
+    const int idx = tid + q * nthr;
+    wpre[q] = wfit && idx < B * n ? reinterpret_cast<const __half2 *>(P.weights)[((long)(c * 2 + p) * B) * n + idx]
+                                  : __half2{};
+  }
   auto load_matrix = [&]() {
     for (int idx = tid; idx < nbl; idx += nthr) {
       int j = (int)((sqrtf(8.0f * idx + 1.0f) - 1.0f) * 0.5f);
@@ -1816,7 +1855,7 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
     // tridiagonal path, unless it needs more than kTriK vectors (or one is
     // not finite): then the matrix is formed again for the Jacobi solver
     bool solved = false;
-    if (!P.full_outputs && !P.shrink && n >= 3 && n <= (NTHR / 64) * tri_cols<NTHR / 64>()) {
+    if (!P.full_outputs && !P.shrink && n >= 3) {
       const TriLds T = tri_lds(S.W, n);
       tri_reduce<NTHR>(S.A, n, T);
       // with detection all eigenvalues (percentiles); else the top ones used
@@ -1848,13 +1887,11 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
     });
   // modes clearly above the noise floor (> 4x the mean of the remaining eigenvalues):
   // the label count of the warm-start training set
-  if (tid == 0 && P.nsig) {
-    float rest = 0.0f;
-    for (int i = 0; i < n - K; ++i) rest += S.ev[i];
-    rest = n > K ? rest / (float)(n - K) : 0.0f;
-    int ns = 0;
-    for (int k = 0; k < K; ++k) ns += S.ev[n - 1 - k] > 4.0f * rest;
-    P.nsig[cp] = ns;
+  if (P.nsig && tid < 64) {
+    const float rest = wave_sum(tid < n - K ? S.ev[tid] : 0.0f);
+    const float mean = n > K ? rest / (float)(n - K) : 0.0f;
+    const float ns = wave_sum(tid >= n - K && tid < n && S.ev[tid] > 4.0f * mean ? 1.0f : 0.0f);
+    if (tid == 0) P.nsig[cp] = (int)ns;
   }
   // top-KW whitened eigenvectors (descending): the next block's warm start / training labels
   if (P.basis_out) {
@@ -1865,45 +1902,44 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
   }
   if (tid == 0 && P.fallback) P.fallback[cp] = fell;
   // FP64 orthogonality check of the null basis: max |u_a^H u_b - delta_ab| over the
-  // K removed (whitened) eigenvectors, one (a, b) pair per thread
+  // K removed (whitened) eigenvectors, one (a, b) pair per wave (lanes: rows)
   if (P.orth) {
     double e = 0.0;
-    for (int ab = tid; ab < K * K; ab += nthr) {
+    const int lane = tid & 63, wq = tid >> 6, nwq = nthr >> 6;
+    for (int ab = wq; ab < K * K; ab += nwq) {
       const int a = ab / K, b = ab - a * K;
       const int ca = S.perm[n - 1 - a], cb = S.perm[n - 1 - b];
-      double re = 0.0, im = 0.0;
-#pragma unroll 1
-      for (int i = 0; i < n; ++i) {
-        const float2 x = S.V[ca * S.ldv + i], y = S.V[cb * S.ldv + i];
-        re += (double)x.x * y.x + (double)x.y * y.y;
-        im += (double)x.x * y.y - (double)x.y * y.x;
-      }
+      const float2 x = lane < n ? S.V[ca * S.ldv + lane] : make_float2(0.f, 0.f);
+      const float2 y = lane < n ? S.V[cb * S.ldv + lane] : make_float2(0.f, 0.f);
+      double re = wave_sum_f64((double)x.x * y.x + (double)x.y * y.y);
+      const double im = wave_sum_f64((double)x.x * y.y - (double)x.y * y.x);
       if (a == b) re -= 1.0;
       e = fmax(e, sqrt(re * re + im * im));
     }
     const float ef = block_max((float)e, S.red);
     if (tid == 0) P.orth[cp] = ef;
   }
-  // 5. output basis U (sorted columns) into S.A: unwhitened+normalised (null) or whitened (shrink)
-  if (tid < n) {
-    const int src = S.perm[tid];
-    float nrm = 0.0f;
-    for (int i = 0; i < n; ++i) {
-      float2 v = S.V[src * S.ldv + i];
+  // 5. output basis U (sorted columns) into S.A: unwhitened+normalised (null) or whitened (shrink);
+  //    without eigen outputs the null needs only the K removed columns.  One wave per column.
+  {
+    const int lane = tid & 63, wq = tid >> 6, nwq = nthr >> 6;
+    const int kfirst = (P.full_outputs || P.shrink) ? 0 : n - K;
+    for (int col = kfirst + wq; col < n; col += nwq) {
+      const int src = S.perm[col];
+      float2 v = lane < n ? S.V[src * S.ldv + lane] : make_float2(0.f, 0.f);
       if (!P.shrink) {
-        v.x *= S.d[i];
-        v.y *= S.d[i];
+        const float dl = lane < n ? S.d[lane] : 0.0f;
+        v.x *= dl;
+        v.y *= dl;
+        const float nrm = wave_sum(v.x * v.x + v.y * v.y);
+        const float inv = nrm > 0.0f ? rsqrtf(nrm) : 0.0f;
+        v.x *= inv;
+        v.y *= inv;
       }
-      S.A[tid * n + i] = v;
-      nrm += v.x * v.x + v.y * v.y;
+      if (lane < n) S.A[col * n + lane] = v;
     }
-    if (!P.shrink) {
-      const float inv = nrm > 0.0f ? rsqrtf(nrm) : 0.0f;
-      for (int i = 0; i < n; ++i) {
-        S.A[tid * n + i].x *= inv;
-        S.A[tid * n + i].y *= inv;
-      }
-    }
+  }
+  if (tid < n) {
     // per-mode factor: null -> -1 on the K largest, shrink -> lambda_bar / lambda_k
     float fk;
     if (!P.shrink) {
@@ -1918,10 +1954,16 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
     }
     S.f[tid] = fk;
   }
-  // weights of this (c, p): rows 0..B-1 original
-  for (int idx = tid; idx < B * n; idx += nthr) {
-    const __half2 w = reinterpret_cast<const __half2 *>(P.weights)[((long)(c * 2 + p) * B) * n + idx];
-    S.W[idx] = __half22float2(w);
+  // weights of this (c, p): rows 0..B-1 original (prefetched at the start)
+  if (wfit) {
+#pragma unroll
+    for (int q = 0; q < kWPre; ++q)
+      if (tid + q * nthr < B * n) S.W[tid + q * nthr] = __half22float2(wpre[q]);
+  } else {
+    for (int idx = tid; idx < B * n; idx += nthr) {
+      const __half2 w = reinterpret_cast<const __half2 *>(P.weights)[((long)(c * 2 + p) * B) * n + idx];
+      S.W[idx] = __half22float2(w);
+    }
   }
   __syncthreads();
   // 6. eigen outputs (when requested)
