@@ -1618,11 +1618,17 @@ __device__ __forceinline__ float tri_eigenvalues(int n, int nsel, const TriLds &
 }
 
 // Eigenvectors of the `need` largest eigenvalues into columns n-1, n-2, ...
-// of V (the other columns zero): inverse iteration on the real tridiagonal
-// (two solves of (T - lambda I) z = b, Gaussian elimination with partial
-// pivoting, pivots kept away from zero, pseudo-random start), modified
-// Gram-Schmidt over the vectors, then v = H_0 ... H_{n-3} D z.  Returns
-// false (uniform) if a vector is not finite.
+// of V (the other columns zero), on the real tridiagonal, then
+// v = H_0 ... H_{n-3} D z.  An isolated eigenvalue (gap to its neighbours at
+// least 1e-3 of the Gershgorin bound) takes the twisted factorization of
+// T - lambda I: forward pivots D+ and backward pivots D- (two lanes, two
+// short independent chains), the twist r = argmin |D+_r + D-_r - (a_r - lambda)|,
+// then z_r = 1, z_i = -(b_i / D+_i) z_{i+1} upwards and
+// z_{i+1} = -(b_i / D-_{i+1}) z_i downwards.  The others take inverse
+// iteration (two solves of (T - lambda I) z = b, Gaussian elimination with
+// partial pivoting, pseudo-random start).  Pivots are kept away from zero;
+// modified Gram-Schmidt over the vectors.  Returns false (uniform) if a
+// vector is not finite.
 template <int NTHR>
 __device__ __forceinline__ bool tri_vectors(const float2 *A, int n, int need, const TriLds &T, const float *ev,
                                             float tn, float2 *V, int ldv) {
@@ -1630,9 +1636,46 @@ __device__ __forceinline__ bool tri_vectors(const float2 *A, int n, int need, co
   const int tid = threadIdx.x, r = tid & 63, w = tid >> 6, lda = jac_ne(n);
   const float tiny = fmaxf(2.4e-7f * tn, 1e-20f);  // a zero matrix: z = 1e20 b
   if (tid == 0) T.flag[0] = 1;
+  int *iso = T.flag + 16, *rtw = T.flag + 24;  // [kTriK] isolated, twist index
   if (w == 0) {
+    if (r < need) {
+      const int q = n - 1 - r;
+      const float glo = q >= 1 ? ev[q] - ev[q - 1] : 3e38f, ghi = q + 1 < n ? ev[q + 1] - ev[q] : 3e38f;
+      iso[r] = fminf(glo, ghi) >= 1e-3f * tn ? 1 : 0;
+    }
+    wave_sync();
+    // twisted pivots: lane 32 + 2k forward, 33 + 2k backward (D+ and D- of
+    // vector k at fac rows 4i and 4i + 1)
+    const int kt = (r - 32) >> 1;
+    const bool twl = r >= 32 && kt < need && iso[kt & (kTriK - 1)];
+    if (twl) {
+      const float sig = ev[n - 1 - kt];
+      const float *__restrict__ ta = T.ta;
+      const float *__restrict__ tb = T.tb;
+      float *__restrict__ dp = T.fac + kt;
+      auto guard = [&](float d) { return fabsf(d) < tiny ? (d >= 0.0f ? tiny : -tiny) : d; };
+      if (((r - 32) & 1) == 0) {
+        float d = guard(ta[0] - sig);
+        dp[0] = d;
+#pragma unroll 4
+        for (int i = 0; i + 1 < n; ++i) {
+          const float b = tb[i];
+          d = guard((ta[i + 1] - sig) - b * b * __builtin_amdgcn_rcpf(d));
+          dp[(i + 1) * 4 * kTriK] = d;
+        }
+      } else {
+        float d = guard(ta[n - 1] - sig);
+        dp[((n - 1) * 4 + 1) * kTriK] = d;
+#pragma unroll 4
+        for (int i = n - 2; i >= 0; --i) {
+          const float b = tb[i];
+          d = guard((ta[i] - sig) - b * b * __builtin_amdgcn_rcpf(d));
+          dp[(i * 4 + 1) * kTriK] = d;
+        }
+      }
+    }
     const int k = r;
-    const bool own = k < need;
+    const bool own = k < need && !iso[k & (kTriK - 1)];
     float *z = T.z + k, *f = T.fac + k;
     if (own) {
       const float sig = ev[n - 1 - k];
@@ -1693,6 +1736,48 @@ __device__ __forceinline__ bool tri_vectors(const float2 *A, int n, int need, co
 #pragma unroll 4
       for (int i = 0; i < n; ++i) zz[i * kTriK] *= sc;
       if (!fin) T.flag[0] = 0;
+    }
+    wave_sync();
+    // the twist of each isolated vector
+    for (int kk = 0; kk < need; ++kk) {
+      if (iso[kk]) {
+        const float sig = ev[n - 1 - kk];
+        const float g = r < n ? fabsf(T.fac[(r * 4) * kTriK + kk] + T.fac[(r * 4 + 1) * kTriK + kk] - (T.ta[r] - sig))
+                              : 3e38f;
+        const float gm = -wave_max_any(-g);
+        const unsigned long long bal = __ballot(r < n && g == gm);
+        if (r == 0) rtw[kk] = bal ? (int)__builtin_ctzll(bal) : 0;
+      }
+    }
+    wave_sync();
+    if (twl) {
+      const int rt = rtw[kt];
+      const float *__restrict__ tb = T.tb;
+      const float *__restrict__ dp = T.fac + kt;
+      float *__restrict__ zz = T.z + kt;
+      float zc = 1.0f;
+      if (((r - 32) & 1) == 0) {
+        zz[rt * kTriK] = 1.0f;
+        for (int i = rt - 1; i >= 0; --i) {
+          zc = -(tb[i] * __builtin_amdgcn_rcpf(dp[(i * 4) * kTriK])) * zc;
+          zz[i * kTriK] = zc;
+        }
+      } else {
+        for (int i = rt; i + 1 < n; ++i) {
+          zc = -(tb[i] * __builtin_amdgcn_rcpf(dp[((i + 1) * 4 + 1) * kTriK])) * zc;
+          zz[(i + 1) * kTriK] = zc;
+        }
+      }
+    }
+    wave_sync();
+    for (int kk = 0; kk < need; ++kk) {
+      if (iso[kk]) {
+        const float v = r < n ? T.z[r * kTriK + kk] : 0.0f;
+        const float ss = wave_sum(v * v);
+        const bool fin = ss > 0.0f && ss < 3e38f;
+        if (r < n) T.z[r * kTriK + kk] = fin ? v * rsqrtf(ss) : 0.0f;
+        if (r == 0 && !fin) T.flag[0] = 0;
+      }
     }
     // modified Gram-Schmidt, descending eigenvalues (vector k after 0 .. k-1)
     for (int j = 0; j + 1 < need; ++j) {
@@ -1860,7 +1945,8 @@ __device__ __forceinline__ void eig_null_problem(const EigParams &P, char *smem_
       tri_reduce<NTHR>(S.A, n, T);
       // with detection all eigenvalues (percentiles); else the top ones used
       const int kw = P.basis_out ? min(P.KW, n) : 0;
-      const int nsel = P.detect ? n : min(n, max(max(0, min(n, fixed)), kw));
+      // (one more than the vectors need: the gap below the last one)
+      const int nsel = P.detect ? n : min(n, max(max(0, min(n, fixed)), kw) + 1);
       const float tn = tri_eigenvalues<NTHR>(n, nsel, T, S.ev, S.perm);
       K = select_k(S.ev, n, fixed, P.detect, P.delta, S.kdet);
       const int need = max(K, kw);
