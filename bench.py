@@ -213,6 +213,44 @@ def run_lambda_gpu(shape, d_ring, d_scales, w16, args):
             "eigenproblems_per_block": shape.C * 4}
 
 
+def run_eigen_outputs(shape, d_ring, d_scales, w16, args):
+    """The headline workload with every block's eigenvalues and eigenvectors
+    landing in device buffers: such blocks take the cyclic Jacobi solver
+    (all n eigenvectors), where blocks without eigen outputs take the
+    tridiagonal solve for the removed modes only."""
+    n = max(10, args.steps // 3)
+    p = lam.Pipeline(shape, lam.MODE_ADAPTIVE, num_buffers=args.buffers)
+    p.set_weights(w16)
+    p.set_fixed_k(np.ones((shape.C, 2), np.int32))
+    p.set_eig_params(args.sweeps, 1e-6)
+    p.set_tuning(args.spw)
+    evals = lam.DeviceArray((shape.C, 2, shape.R), np.float32)
+    evecs = lam.DeviceArray((shape.C, 2, shape.R, shape.R, 2), np.float32)
+    for _ in range(5):
+        p.execute(d_ring, d_scales, evals_out=evals, evecs_out=evecs, on_device=True)
+    p.sync()
+    p.dump_visibilities_i64()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        p.execute(d_ring, d_scales, evals_out=evals, evecs_out=evecs, on_device=True)
+    p.sync()
+    t1 = time.perf_counter()
+    p.dump_visibilities_i64()
+    p.set_profiling(True)
+    p.kernel_times()
+    for _ in range(n):
+        p.execute(d_ring, d_scales, evals_out=evals, evecs_out=evecs, on_device=True)
+        p.sync()
+    kt = p.kernel_times()
+    sweeps = float(p.eig_stats()[0].mean())
+    p.close()
+    sec = (t1 - t0) / n
+    return {"ms_per_step": sec * 1e3, "real_time_factor": (shape.T / CHANNEL_RATE_HZ) / sec,
+            "ant_samples_per_s": shape.R * shape.C * shape.T / sec, "solver": "cyclic Jacobi",
+            "sweeps_mean": round(sweeps, 2),
+            "kernels_us": {k: round(v[0] / max(1, v[1]) * 1e3, 1) for k, v in kt.items()}}
+
+
 def run_projection(shape, d_ring, d_scales, args, k=1, runs=4):
     """LambdaProjectionPipeline on the same block: per block the projector onto
     the K largest eigenvectors of all four pol-pair covariances is accumulated
@@ -576,6 +614,8 @@ def main():
     ap.add_argument("--no-spectra", action="store_true")
     ap.add_argument("--no-host-fed", action="store_true")
     ap.add_argument("--no-lambda-gpu", action="store_true")
+    ap.add_argument("--no-eigen-outputs", action="store_true",
+                    help="skip the run with eigenvalues/eigenvectors requested (Jacobi solver)")
     ap.add_argument("--spw", type=int, default=0, help="samples per voltage workgroup (0: automatic)")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -683,16 +723,26 @@ def main():
     for k, v in kstats.items():
         v["GBps_algorithmic"] = round(per_kernel_bytes[k] / (v["ms_avg"] * 1e-3) / 1e9, 1)
     # the eigen stage is bound by neither HBM nor MFMA: report it as matrices/s and
-    # as a fraction of the FP32 vector peak, counting the algorithmic Jacobi work
-    # N(N-1)/2 rotations x 6N complex updates x 8 flops per sweep and matrix
+    # as a fraction of the FP32 vector peak.  Blocks without eigen outputs take the
+    # tridiagonal solve (0 sweeps): its algorithmic work is the Householder
+    # reduction of a complex Hermitian matrix, 16/3 N^3 real flops (the
+    # eigenvalues and the removed modes' vectors are O(N^2)); blocks with eigen
+    # outputs take cyclic Jacobi: N(N-1)/2 rotations x 6N complex updates x 8
+    # flops per sweep and matrix
     eig_roof = None
     if "eig_null" in kstats and sweeps_mean is not None:
         ms = kstats["eig_null"]["ms_avg"]
         nmat = shape.C * 2
         n = shape.R
-        flops = nmat * sweeps_mean * (n * (n - 1) / 2) * 6 * n * 8
+        if sweeps_mean == 0:
+            solver = "householder tridiagonal + multisection + twisted factorization"
+            flops = nmat * 16.0 / 3.0 * n ** 3
+        else:
+            solver = "cyclic Jacobi"
+            flops = nmat * sweeps_mean * (n * (n - 1) / 2) * 6 * n * 8
         tf = flops / (ms * 1e-3) / 1e12
-        eig_roof = {"bound": "fp32 valu / latency", "matrices_per_s": round(nmat / (ms * 1e-3), 1),
+        eig_roof = {"bound": "fp32 valu / latency", "solver": solver,
+                    "matrices_per_s": round(nmat / (ms * 1e-3), 1),
                     "matrices": nmat, "n": n, "sweeps_mean": round(sweeps_mean, 2), "kernel_ms": round(ms, 4),
                     "achieved_tflops": round(tf, 2), "peak_tflops": 157.3, "frac": round(tf / 157.3, 4),
                     "peak_tflops_unpacked": 78.6, "frac_unpacked": round(tf / 78.6, 4)}
@@ -710,6 +760,11 @@ def main():
         if ent:
             roofline["traffic"] = ent["bytes_per_launch"].get(dominant)
             roofline["traffic_splits"] = nsplit
+
+    eigen_outputs = None
+    if mode == lam.MODE_ADAPTIVE and not args.no_eigen_outputs:
+        eigen_outputs = run_eigen_outputs(shape, d_ring, d_scales, w16, args)
+        log(f"eigen outputs: {eigen_outputs}")
 
     lambda_gpu = projection = None
     if not args.no_lambda_gpu:
@@ -747,11 +802,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int8->int32 corr, fp16 MFMA beams, fp32 eig (fp64 angles)",
+        "dtype": "int8->int32 corr, fp16 MFMA beams, fp32 eig",
         "data": "synthetic (seeded Gaussian sigma=16 + 1 CW tone/channel, int8)",
         "config": {"workload": f"{args.mode}: {shape.C} ch x 2 pol x 36 ant (64 rx), "
                                f"{shape.T} samples/block, {shape.B} beams -> {nb_out} out, "
-                               f"plain Jacobi ({args.sweeps} sweeps max) rank-1 null, "
+                               "plain eigen solve without eigen outputs (Householder tridiagonal, "
+                               "top eigenvalues by multisection, removed vector by twisted factorization) "
+                               "rank-1 null, "
                                f"int64 vis accumulated every block (dumped outside the timed region), {args.buffers} pipeline buffers "
                                "(consecutive blocks overlap)",
                    "channels": shape.C, "samples_per_block": shape.T, "beams": shape.B,
@@ -761,6 +818,7 @@ def main():
         "null_basis_orthogonality_fp64_max": orth_max,
         "roofline": roofline,
         "eig_roofline": eig_roof,
+        "with_eigen_outputs": eigen_outputs,
         "kernels": kstats,
         "autotune": tuned,
         "learned_vs_plain": learned,
