@@ -179,9 +179,12 @@ typedef struct lam_block_io {
   void *beams_done_user;
   float *eigenvalues_dst;  /* adaptive or LAMBDA_GPU mode (NULL: skip).  In the adaptive
                             * mode a block with neither eigen landing pointer forms only
-                            * what its null needs: no eigen outputs are written and the
-                            * warm starts form no complement basis (the 2B beams, counts
-                            * and statistics are unchanged). */
+                            * what its null needs: no eigen outputs are written, the
+                            * plain solve finds the eigenvalues K needs and only the
+                            * removed modes' vectors (tridiagonal reduction instead of
+                            * Jacobi, except in shrink mode), and the warm starts form no
+                            * complement basis; the 2B beams and counts agree with a block
+                            * that requests the outputs to the beams' tolerance. */
   float *eigenvectors_dst;
   int32_t *counts_dst;
   lam_host_fn eigen_done_fn;
