@@ -1420,6 +1420,7 @@ __device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
   float2 *pp = T.part;                   // [OW][64] partial products
   float4 *uw = reinterpret_cast<float4 *>(pp + OW * 64);  // [2][64] (u_j, w_j) by parity
   float2 *un = reinterpret_cast<float2 *>(uw + 2 * 64);   // [2][64] u_j by parity (the product's)
+  float2 *sink = un + 2 * 64;            // [64] stores of the lanes past n
   // wave 0's vectors: u_j, u_{j-1}, w_{j-1} (lane i: row i)
   float2 uc = make_float2(0.f, 0.f), up = make_float2(0.f, 0.f), wp = make_float2(0.f, 0.f);
   // wave 0: the reflector of column x (rows > j) for step j, into un / A / T
@@ -1452,14 +1453,34 @@ __device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
     return lo + ((m - lo) % OW + OW) % OW;
   };
   for (int j = 0; j + 2 < n; ++j) {
+    // wave 0, while the owners form the product: the sums and the part of the
+    // next column that do not need the product
+    float al_r = 0.f, al_i = 0.f, be_r = 0.f, be_i = 0.f, dn0 = 0.f;
+    float2 u1 = make_float2(0.f, 0.f), xq = make_float2(0.f, 0.f);
+    if (w == 0) {
+      al_r = wave_sum(wp.x * uc.x + wp.y * uc.y);
+      al_i = wave_sum(wp.x * uc.y - wp.y * uc.x);
+      be_r = wave_sum(up.x * uc.x + up.y * uc.y);
+      be_i = wave_sum(up.x * uc.y - up.y * uc.x);
+      // column j+1 (before step j-1's update) with step j-1's update
+      const float2 xo = i < n ? A[(j + 1) * lda + i] : make_float2(0.f, 0.f);
+      const float2 u0 = rl2(up, j + 1), w0 = rl2(wp, j + 1);
+      u1 = rl2(uc, j + 1);
+      const float2 t3 = cmulc(up, w0), t4 = cmulc(wp, u0);
+      xq = make_float2(xo.x - t3.x - t4.x, xo.y - t3.y - t4.y);
+      dn0 = rlf(xo.x, j + 1) - 2.0f * (u0.x * w0.x + u0.y * w0.y);
+    }
     if (ow >= 0) {
       // this wave's part of A22 u_j (columns above j, one update behind)
+      // (rows past n read row n-1: finite values that only meet zero entries
+      // of u, w and x; no exec-mask branch around the loads)
       const float2 *uj = un + (j & 1) * 64;
+      const int ir = min(i, n - 1);
       float2 yp = make_float2(0.f, 0.f);
 #pragma unroll 4
       for (int c = first_col(j + 1); c < n; c += OW) {
         const float2 e = uj[c];
-        const float2 av = i < n ? A[c * lda + i] : make_float2(0.f, 0.f);
+        const float2 av = A[c * lda + ir];
         yp.x += av.x * e.x - av.y * e.y;
         yp.y += av.x * e.y + av.y * e.x;
       }
@@ -1475,11 +1496,8 @@ __device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
         y.x += t2.x;
         y.y += t2.y;
       }
-      const float2 xo = i < n ? A[(j + 1) * lda + i] : make_float2(0.f, 0.f);  // column j+1, before step j-1's update
       // y = c (P - u' al - w' be) with al = w'^H u, be = u'^H u;
       // Re(u^H y) / c = Re(u^H P) - 2 Re(conj(be) al)
-      const float al_r = wave_sum(wp.x * uc.x + wp.y * uc.y), al_i = wave_sum(wp.x * uc.y - wp.y * uc.x);
-      const float be_r = wave_sum(up.x * uc.x + up.y * uc.y), be_i = wave_sum(up.x * uc.y - up.y * uc.x);
       const float g0 = wave_sum(uc.x * y.x + uc.y * y.y);
       y.x -= up.x * al_r - up.y * al_i + wp.x * be_r - wp.y * be_i;
       y.y -= up.x * al_i + up.y * al_r + wp.x * be_i + wp.y * be_r;
@@ -1488,13 +1506,12 @@ __device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
       const float kap = 0.5f * c * c * (g0 - 2.0f * (be_r * al_r + be_i * al_i));
       const float2 wv = i > j && i < n ? make_float2(y.x - kap * uc.x, y.y - kap * uc.y) : make_float2(0.f, 0.f);
       uw[(j & 1) * 64 + i] = make_float4(uc.x, uc.y, wv.x, wv.y);
-      // column j+1 with the updates of steps j-1 and j: the next x and diagonal entry
-      const float2 u1 = rl2(uc, j + 1), w1 = rl2(wv, j + 1), u0 = rl2(up, j + 1), w0 = rl2(wp, j + 1);
-      const float2 t1 = cmulc(uc, w1), t2 = cmulc(wv, u1), t3 = cmulc(up, w0), t4 = cmulc(wp, u0);
-      const float2 xn = i > j + 1 && i < n ? make_float2(xo.x - t1.x - t2.x - t3.x - t4.x,
-                                                         xo.y - t1.y - t2.y - t3.y - t4.y)
+      // the next x and diagonal entry: step j's update on top
+      const float2 w1 = rl2(wv, j + 1);
+      const float2 t1 = cmulc(uc, w1), t2 = cmulc(wv, u1);
+      const float2 xn = i > j + 1 && i < n ? make_float2(xq.x - t1.x - t2.x, xq.y - t1.y - t2.y)
                                            : make_float2(0.f, 0.f);
-      const float dn = rlf(xo.x, j + 1) - 2.0f * (u1.x * w1.x + u1.y * w1.y) - 2.0f * (u0.x * w0.x + u0.y * w0.y);
+      const float dn = dn0 - 2.0f * (u1.x * w1.x + u1.y * w1.y);
       up = uc;
       wp = wv;
       if (j + 3 < n) {
@@ -1515,15 +1532,16 @@ __device__ __forceinline__ void tri_reduce(float2 *A, int n, const TriLds &T) {
     if (ow >= 0 && j > 0) {
       const float4 *uwp = uw + ((j - 1) & 1) * 64;
       const float4 me = uwp[i];  // (u'_i, w'_i)
+      const int ir = min(i, n - 1);
 #pragma unroll 4
       for (int c = first_col(j + 2); c < n; c += OW) {
         const float4 e = uwp[c];
-        if (i < n) {
-          // a -= u'_i conj(w'_c) + w'_i conj(u'_c)
-          const float2 av = A[c * lda + i];
-          A[c * lda + i] = make_float2(av.x - (me.x * e.z + me.y * e.w + me.z * e.x + me.w * e.y),
-                                       av.y - (me.y * e.z - me.x * e.w + me.w * e.x - me.z * e.y));
-        }
+        // a -= u'_i conj(w'_c) + w'_i conj(u'_c) (lanes past n: into the sink)
+        const float2 av = A[c * lda + ir];
+        const float2 nv = make_float2(av.x - (me.x * e.z + me.y * e.w + me.z * e.x + me.w * e.y),
+                                      av.y - (me.y * e.z - me.x * e.w + me.w * e.x - me.z * e.y));
+        float2 *dst = i < n ? A + c * lda + i : sink + i;
+        *dst = nv;
       }
     }
     __syncthreads();

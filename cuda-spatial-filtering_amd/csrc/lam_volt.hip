@@ -182,6 +182,45 @@ __device__ __forceinline__ void corr_chunk(const uint32_t *L, int gA, int gB, in
   }
 }
 
+// Operand tiles of one chunk, transposed once per workgroup: tile
+// (G, ks, v) holds op[v] of antenna group G at k-step ks (rows 64 ks ..) for
+// every lane, [group][k-step][component][lane].  Wave u < 2 NG builds tile
+// set u = 2 G + ks (the correlator waves, and the loader wave when NG = 1),
+// so each group is transposed once per chunk instead of once per group pair.
+template <int RSD, int NG>
+__device__ __forceinline__ void transpose_chunk(const uint32_t *L, v4i *tiles, int wave, int lane) {
+  if (wave < 2 * NG) {
+    v4i op[4];
+    load_group<RSD>(L, wave >> 1, 64 * (wave & 1), lane, op);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) tiles[(wave * 4 + v) * 64 + lane] = op[v];
+  }
+}
+
+// One chunk of correlation for the wave's group pair, from the transposed tiles.
+__device__ __forceinline__ void corr_chunk_tiles(const v4i *tiles, int gA, int gB, int lane, v4i acc[12]) {
+  static_assert(kChunk == 128, "two 64-row k-steps per chunk");
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    v4i opA[4], opB[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      opA[v] = tiles[((2 * gA + ks) * 4 + v) * 64 + lane];
+      opB[v] = tiles[((2 * gB + ks) * 4 + v) * 64 + lane];
+    }
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        v4i *a3 = &acc[(p * 2 + q) * 3];
+        a3[0] = mfma_i8(opA[2 * p], opB[2 * q], a3[0]);
+        a3[0] = mfma_i8(opA[2 * p + 1], opB[2 * q + 1], a3[0]);
+        a3[1] = mfma_i8(opA[2 * p + 1], opB[2 * q], a3[1]);
+        a3[2] = mfma_i8(opA[2 * p], opB[2 * q + 1], a3[2]);
+      }
+  }
+}
+
 // Partial visibilities of this workgroup: vis_part[split][c][p*2+q][bl(a,b)][re,im]
 // (polarisation pair major, so the XX and YY covariances are contiguous runs).
 __device__ __forceinline__ void corr_store(const VoltParams &P, const v4i acc[12], int gA, int gB,
@@ -407,6 +446,7 @@ volt_fast_kernel(VoltParams P) {
   static_assert(NI * (NS - 2) < 64, "the wait count for the oldest chunk must fit vmcnt");
   // slack past the last stage for the padding-column reads of a TIGHT layout
   __shared__ __align__(16) uint32_t lds[NS * STAGE + (TIGHT ? 16 : 0)];
+  __shared__ v4i tiles[DO_CORR ? NG * 2 * 4 * 64 : 1];  // the chunk's correlator operands
   __shared__ int s_flags;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -514,9 +554,14 @@ volt_fast_kernel(VoltParams P) {
       }
     }
     chunk_barrier();
+    if constexpr (DO_CORR) {
+      // (the previous chunk's tile reads ended before the barrier above)
+      transpose_chunk<RSD, NG>(L, tiles, wave, lane);
+      chunk_barrier();
+    }
     if (is_loader && ch + NS - 1 < nchunks) issue(ch + NS - 1);
     if constexpr (DO_CORR) {
-      if (is_corr) corr_chunk<RSD>(L, gA, gB, lane, acc);
+      if (is_corr) corr_chunk_tiles(tiles, gA, gB, lane, acc);
     }
     if constexpr (DO_BEAM) {
       if (!is_corr && !is_loader)
