@@ -1825,7 +1825,6 @@ __device__ __forceinline__ bool tri_vectors(const float2 *A, int n, int need, co
     const float zi = r < n ? T.z[r * kTriK + k] : 0.f;
     const float2 phr = r < n ? T.ph[r] : make_float2(0.f, 0.f);
     float2 v = make_float2(phr.x * zi, phr.y * zi);
-#pragma unroll 2
     for (int j = n - 3; j >= 0; --j) {
       const float c = T.hc[j];  // 0: no reflector (the product vanishes)
       const float2 uj = r > j && r < n ? A[j * lda + r] : make_float2(0.f, 0.f);
