@@ -343,6 +343,7 @@ int enqueue_phase(lam_ctx *ctx, Buf &b, const void *samples, const void *scales,
   }
   if (phase == 2) {
     VoltParams pb = p;
+    pick_split(ctx->T, ctx->C, beam_samples_per_wg(ctx->T, ctx->C, ctx->num_cus), pb.samples_per_wg, pb.nsplit);
     pb.wop = b.d_wop2;
     pb.nbeams_out = ctx->Bout;  // 2B, plus B per subarray group
     pb.NMT = ctx->NMT2;

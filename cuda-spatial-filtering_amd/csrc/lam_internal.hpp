@@ -57,6 +57,19 @@ inline void pick_split(int T, int C, int requested, int &spw, int &nsplit) {
   if (spw > kMaxSamplesPerSplit) spw = kMaxSamplesPerSplit;
   nsplit = (T + spw - 1) / spw;
 }
+// Samples per workgroup of a beam-only voltage launch.  It writes nothing per split,
+// so it takes more, shorter splits than the correlator (whose split count multiplies
+// its partial visibilities): about 12 workgroups per CU, between one chunk and 1024
+// samples (2B beams at 384 channels x 8192 samples on 256 CUs: 0.291 ms with the
+// correlator's 8192-sample split, 0.252 ms with 1024).
+inline int beam_samples_per_wg(int T, int C, int num_cus) {
+  const long want = 12L * (num_cus > 0 ? num_cus : 256);
+  long spw = ((long)T * C + want - 1) / want;
+  spw = ((spw + kChunk - 1) / kChunk) * kChunk;
+  if (spw < kChunk) spw = kChunk;
+  if (spw > 1024) spw = 1024;
+  return (int)spw;
+}
 #ifndef LAM_EIG_THREADS
 #define LAM_EIG_THREADS 256
 #endif

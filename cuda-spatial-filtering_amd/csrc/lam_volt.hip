@@ -444,6 +444,11 @@ volt_fast_kernel(VoltParams P) {
   constexpr int NS = (LAM_VOLT_STAGES >= 4 && 2 * NI < 64) ? 4 : 3;
   static_assert(NI < 64, "loads of one chunk must fit the vmcnt counter");
   static_assert(NI * (NS - 2) < 64, "the wait count for the oldest chunk must fit vmcnt");
+  // Chunks the loader keeps in flight.  A correlator-only workgroup is done with a raw
+  // stage once the stage is transposed into the operand tiles, so every stage can hold
+  // a chunk in flight; the beam waves read the raw stage while they work on it, so
+  // kernels with beams keep one stage for the chunk being processed.
+  constexpr int PF = (DO_CORR && !DO_BEAM && NI * (NS - 1) < 64) ? NS : NS - 1;
   // slack past the last stage for the padding-column reads of a TIGHT layout
   __shared__ __align__(16) uint32_t lds[NS * STAGE + (TIGHT ? 16 : 0)];
   __shared__ v4i tiles[DO_CORR ? NG * 2 * 4 * 64 : 1];  // the chunk's correlator operands
@@ -520,14 +525,19 @@ volt_fast_kernel(VoltParams P) {
 
   if (is_loader) {
 #pragma unroll
-    for (int k = 0; k < NS - 1; ++k)
+    for (int k = 0; k < PF; ++k)
       if (k < nchunks) issue(k);
   }
   for (int ch = 0; ch < nchunks; ++ch) {
     if (is_loader) {
-      // chunk ch is complete when at most min(NS - 2, chunks issued after it) chunk loads remain
-      const int after = min(NS - 2, nchunks - 1 - ch);
-      if constexpr (NS >= 4) {
+      // chunk ch is complete when at most min(PF - 1, chunks issued after it) chunk loads remain
+      const int after = min(PF - 1, nchunks - 1 - ch);
+      if constexpr (PF >= 4) {
+        if (after >= 3) wait_vmcnt<(PF >= 4 ? 3 * NI : 0)>();
+        else if (after == 2) wait_vmcnt<2 * NI>();
+        else if (after == 1) wait_vmcnt<NI>();
+        else wait_vmcnt<0>();
+      } else if constexpr (PF == 3) {
         if (after >= 2) wait_vmcnt<2 * NI>();
         else if (after == 1) wait_vmcnt<NI>();
         else wait_vmcnt<0>();
@@ -559,7 +569,9 @@ volt_fast_kernel(VoltParams P) {
       transpose_chunk<RSD, NG>(L, tiles, wave, lane);
       chunk_barrier();
     }
-    if (is_loader && ch + NS - 1 < nchunks) issue(ch + NS - 1);
+    // (the refilled stage's last readers, the transpose or the previous chunk's beam
+    // waves, finished before the barrier above)
+    if (is_loader && ch + PF < nchunks) issue(ch + PF);
     if constexpr (DO_CORR) {
       if (is_corr) corr_chunk_tiles(tiles, gA, gB, lane, acc);
     }
