@@ -2119,16 +2119,38 @@ __global__ void __launch_bounds__(NTHR) eig_null_kernel(EigParams P) {
   eig_null_problem<NTHR, WARM>(P, smem_raw, blockIdx.x >> 1, blockIdx.x & 1);
 }
 
-template <int NTHR, bool WARM>
-hipError_t launch_eig_t(const EigParams &p, hipStream_t s) {
-  const size_t sm = eig_smem_bytes(p.R, p.B, WARM, NTHR / 64);
+// The same solve for blocks that take the tridiagonal path (no eigen outputs, no
+// shrink, n >= 3; its rare Jacobi fallback still runs here), held to 96 VGPRs (five
+// waves per SIMD): three of these workgroups then leave a CU room for a correlator
+// workgroup of the next block (2 waves x 104 VGPRs on a SIMD), where at 144 VGPRs the
+// eigen stage kept voltage work off every CU it held for its ~95 us.  The cap spills
+// into scratch and costs the kernel ~5 us on its own (384 ch x 36 antennas), but the
+// overlapped step gains ~2 %; the Jacobi path (eigen outputs) loses far more under the
+// same cap (251 -> 297 us) and keeps the uncapped kernel.
+template <int NTHR>
+__global__ void __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(5))) eig_tri_kernel(EigParams P) {
+  extern __shared__ __align__(16) char smem_raw[];
+  eig_null_problem<NTHR, false>(P, smem_raw, blockIdx.x >> 1, blockIdx.x & 1);
+}
+
+template <typename K>
+hipError_t launch_eig_kernel(K kern, const EigParams &p, int nthr, size_t sm, hipStream_t s) {
   if (sm > 65536) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(eig_null_kernel<NTHR, WARM>),
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((eig_null_kernel<NTHR, WARM>), dim3(p.C * 2), dim3(NTHR), sm, s, p);
+  hipLaunchKernelGGL(kern, dim3(p.C * 2), dim3(nthr), sm, s, p);
   return hipGetLastError();
+}
+
+template <int NTHR, bool WARM>
+hipError_t launch_eig_t(const EigParams &p, hipStream_t s) {
+  const size_t sm = eig_smem_bytes(p.R, p.B, WARM, NTHR / 64);
+  if constexpr (!WARM && NTHR == 256) {
+    if (!p.full_outputs && !p.shrink && p.R >= 3) return launch_eig_kernel(eig_tri_kernel<NTHR>, p, NTHR, sm, s);
+  }
+  return launch_eig_kernel(eig_null_kernel<NTHR, WARM>, p, NTHR, sm, s);
 }
 
 hipError_t launch_eig_plain(const EigParams &p, int nthr, hipStream_t s) {

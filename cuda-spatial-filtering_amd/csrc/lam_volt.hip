@@ -434,6 +434,10 @@ __device__ __forceinline__ void wg_coords(const VoltParams &P, int &c, int &spli
 #ifndef LAM_VOLT_STAGES
 #define LAM_VOLT_STAGES 3
 #endif
+// ring depth of correlator-only workgroups (every stage holds a chunk in flight)
+#ifndef LAM_CORR_STAGES
+#define LAM_CORR_STAGES 3
+#endif
 template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT, bool TIGHT>
 __global__ void __launch_bounds__((Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, true, TIGHT>::NT))
 volt_fast_kernel(VoltParams P) {
@@ -441,7 +445,8 @@ volt_fast_kernel(VoltParams P) {
   constexpr int RSD = G::RSD, NK = G::NK, STAGE = G::STAGE;
   constexpr int NI = STAGE / 256;  // dwordx4 wave loads per chunk
   // LDS ring depth: up to LAM_VOLT_STAGES, as long as the wait count stays encodable
-  constexpr int NS = (LAM_VOLT_STAGES >= 4 && 2 * NI < 64) ? 4 : 3;
+  constexpr int NS = (DO_CORR && !DO_BEAM) ? LAM_CORR_STAGES : ((LAM_VOLT_STAGES >= 4 && 2 * NI < 64) ? 4 : 3);
+  static_assert(NS >= 2 && NS <= 4, "ring depth");
   static_assert(NI < 64, "loads of one chunk must fit the vmcnt counter");
   static_assert(NI * (NS - 2) < 64, "the wait count for the oldest chunk must fit vmcnt");
   // Chunks the loader keeps in flight.  A correlator-only workgroup is done with a raw
