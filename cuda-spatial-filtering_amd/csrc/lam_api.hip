@@ -120,7 +120,9 @@ struct lam_ctx {
   std::vector<ProfEvent> prof;
   // eigen-stage method
   int eig_method = LAM_EIG_PLAIN, refine_sweeps = 2, power = 0;
-  int eig_full = 1;  // the current block's eigen outputs are requested (EigParams.full_outputs)
+  // the current block's eigen outputs are requested (EigParams.full_outputs); set per
+  // block by lam_execute, and what the autotuner times: a fresh context has none
+  int eig_full = 0;
   float res_tol = 0.1f;
   int eig_plain_threads = 256;
   int wprec = kWprecF16HiLo;

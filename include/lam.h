@@ -278,7 +278,8 @@ enum { LAM_WPREC_F16_HILO = 0, LAM_WPREC_F16 = 1, LAM_WPREC_BF16 = 2 };
 int lam_set_weight_precision(lam_ctx *ctx, int precision);
 
 /* Workgroup width (64, 128 or 256 threads: one to four wavefronts per
- * matrix) of the plain Jacobi launch; 256 by default.  Results are
+ * matrix) of the plain eigen launch (Jacobi or the tridiagonal solve); 256 by
+ * default.  Results are
  * identical bit for bit across launches of the same width. */
 int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads);
 
@@ -288,7 +289,7 @@ int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads);
  *   - time samples per voltage workgroup: automatic, 1024, 2048, 3072, 4096,
  *     8192, judged on the whole block sequence (more splits speed the voltage
  *     kernels but multiply the partial visibilities the eigen stage reads);
- *   - plain Jacobi workgroup width: 64, 128, 256 (adaptive mode);
+ *   - plain eigen solve workgroup width: 64, 128, 256 (adaptive mode);
  *   with several buffers both are judged on streaming time per block (blocks
  *   overlapping, at least 16 timed passes), with one buffer on kernel time;
  *   - warm-start refinement steps 1..3 when LAM_EIG_LEARNED / PREVIOUS is set
@@ -298,6 +299,9 @@ int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads);
  * rotated per round) after a warm-up of about 48 blocks, judges each by its
  * median, and keeps the setting in force before the call unless the best
  * median beats it by more than that setting's own spread over the rounds.
+ * The eigen stage is timed as the context's most recent lam_execute block ran it
+ * (with or without eigen outputs: Jacobi or the tridiagonal solve; a context that
+ * has executed no block yet is tuned without eigen outputs).
  * The blocks do not reach the visibility accumulator, and training capture
  * (lam_learn_capture) is suspended while tuning.  result[3] (may be NULL)
  * = {samples_per_wg, plain_threads, refine_steps}. */
