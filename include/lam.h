@@ -279,8 +279,7 @@ int lam_set_weight_precision(lam_ctx *ctx, int precision);
 
 /* Workgroup width (64, 128 or 256 threads: one to four wavefronts per
  * matrix) of the plain eigen launch (Jacobi or the tridiagonal solve); 256 by
- * default.  Results are
- * identical bit for bit across launches of the same width. */
+ * default.  Results are identical bit for bit across launches of the same width. */
 int lam_set_eig_tuning(lam_ctx *ctx, int plain_threads);
 
 /* Autotuner (no counterpart in the reference): times this context's kernels
