@@ -434,9 +434,12 @@ __device__ __forceinline__ void wg_coords(const VoltParams &P, int &c, int &spli
 #ifndef LAM_VOLT_STAGES
 #define LAM_VOLT_STAGES 3
 #endif
-// ring depth of correlator-only workgroups (every stage holds a chunk in flight)
+// ring depth of correlator-only workgroups (every stage holds a chunk in flight):
+// two stages (61.5 KB of LDS with the operand tiles, against 80 KB for three) ran the
+// correlator 147 against 154 us (384 ch x 8192 samples, one split) with the
+// overlapped step the same or ~1 % shorter
 #ifndef LAM_CORR_STAGES
-#define LAM_CORR_STAGES 3
+#define LAM_CORR_STAGES 2
 #endif
 template <int NG, int KODD, bool DO_CORR, bool DO_BEAM, int NMT, bool TIGHT>
 __global__ void __launch_bounds__((Geo<NG, KODD, DO_CORR, DO_BEAM, NMT, true, TIGHT>::NT))
