@@ -187,7 +187,12 @@ VoltParams base_volt(lam_ctx *ctx, const Buf &b, const void *samples, const void
   p.nt_log2 = -1;
   for (int l = 0; l < 30; ++l)
     if ((1 << l) == ctx->NT) p.nt_log2 = l;
-  pick_split(ctx->T, ctx->C, ctx->samples_per_wg, p.samples_per_wg, p.nsplit);
+  // automatic split: in the adaptive mode about one correlator workgroup per CU (at
+  // 384 channels one split: 0.425 ms per overlapped block against 0.438 with the
+  // three splits of ~1024 workgroups, as the split count multiplies the partials the
+  // eigen stage and the accumulator read); elsewhere about 1024 workgroups
+  pick_split(ctx->T, ctx->C, ctx->samples_per_wg, p.samples_per_wg, p.nsplit,
+             ctx->mode == LAM_MODE_ADAPTIVE ? ctx->num_cus : 1024);
   p.xcd_map = (ctx->C % 8 == 0) ? 1 : 0;
   p.err = b.d_err;
   p.NK = ctx->NK;

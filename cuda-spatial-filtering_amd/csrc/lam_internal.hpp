@@ -41,13 +41,13 @@ constexpr int kMaxSamplesPerSplit = 32768;
 
 // Time samples per voltage workgroup (a multiple of kChunk) and the number of
 // splits per channel for T samples and C channels.  requested <= 0 picks about
-// 1024 workgroups in total; the result keeps nsplit <= kMaxSplits and every
+// target_wgs workgroups in total; the result keeps nsplit <= kMaxSplits and every
 // split within kMaxSamplesPerSplit.
-inline void pick_split(int T, int C, int requested, int &spw, int &nsplit) {
+inline void pick_split(int T, int C, int requested, int &spw, int &nsplit, int target_wgs = 1024) {
   const int Tr = ((T + kChunk - 1) / kChunk) * kChunk;
   spw = requested;
   if (spw <= 0) {
-    const int target = (1024 + C - 1) / C > 1 ? (1024 + C - 1) / C : 1;
+    const int target = (target_wgs + C - 1) / C > 1 ? (target_wgs + C - 1) / C : 1;
     spw = (T + target - 1) / target;
   }
   spw = ((spw + kChunk - 1) / kChunk) * kChunk;
