@@ -754,8 +754,8 @@ def main():
     tfile = os.path.join(REPO, "profiles", "adaptive_384ch_hbm_traffic.json")
     if roofline and mode == lam.MODE_ADAPTIVE and shape.C == 384 and os.path.exists(tfile):
         spw = tuned["samples_per_wg"] if tuned else args.spw
-        if spw <= 0:  # the pipeline's default: about 1024 workgroups over the channels
-            target = max(1, -(-1024 // shape.C))
+        if spw <= 0:  # the adaptive pipeline's default: about one workgroup per CU (256 CUs)
+            target = max(1, -(-256 // shape.C))
             spw = -(-shape.T // target)
         spw = -(-spw // 128) * 128
         nsplit = -(-shape.T // spw)
