@@ -265,7 +265,11 @@ int lam_set_profiling(lam_ctx *ctx, int enable);
 int lam_kernel_times(lam_ctx *ctx, double *ms, int32_t *launches);
 
 /* Tunables picked by the autotuner: time samples per workgroup of the
- * voltage kernels (multiple of 64; 0 = automatic). */
+ * correlating voltage kernels, rounded up to a multiple of 128; 0 = automatic
+ * (about one correlator workgroup per CU in the adaptive mode, about 1024
+ * workgroups in the others).  The adaptive mode's 2B-beam launch, which writes
+ * nothing per split, always takes its own finer split (about 12 workgroups per CU,
+ * at most 1024 samples each). */
 int lam_set_tuning(lam_ctx *ctx, int samples_per_wg);
 /* Precision of the beamformer's weight operands (no counterpart in the
  * reference, whose ccglib GEMM takes FP16 weights):
